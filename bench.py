@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Benchmark of the collective hot path on one MI355X.
+
+Workload (the headline metric's configuration): the team reduction
+op/type matrix at 64 emulated PEs x 4,194,304 elements per PE --
+and/or/xor on uint64 (random bits), min/max/sum on int64 (uniform in
+[-2^40, 2^40]), prod on int64 ({-1, 1, 2}), sum/min/max on float32 and
+float64 (N(0, 1)), prod on float32 and float64 (uniform [0.999, 1.001]).
+Inputs are generated on the device and resident in HBM before timing.  A
+step is one pass over the 15 cells (15 reductions on SHMEM_TEAM_WORLD).
+
+value = algorithmic bytes (2 * P * n * sizeof(T) per reduction: every
+source slice read once, every destination slice written once) summed over
+the timed steps / elapsed seconds, in GB/s (1e9 B/s).  Each reduction is
+bracketed by HIP events on the library stream it runs on; the roofline
+object reports the kernel with the largest total time.
+
+    python bench.py [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "osss-ucx_amd"))
+
+import ctypes as C  # noqa: E402
+
+import osc  # noqa: E402
+
+PEAK_HBM_GBPS = 8000.0  # MI355X HBM3E, AMD published
+P, N = 64, 4194304
+CELLS = [  # (op, dtype, dist, a, b)
+    ("and", "u64", "bits", 0, 0), ("or", "u64", "bits", 0, 0),
+    ("xor", "u64", "bits", 0, 0),
+    ("min", "i64", "int_range", -2.0 ** 40, 2.0 ** 40),
+    ("max", "i64", "int_range", -2.0 ** 40, 2.0 ** 40),
+    ("sum", "i64", "int_range", -2.0 ** 40, 2.0 ** 40),
+    ("prod", "i64", "set_m1_1_2", 0, 0),
+    ("sum", "f32", "normal", 0.0, 1.0), ("min", "f32", "normal", 0.0, 1.0),
+    ("max", "f32", "normal", 0.0, 1.0),
+    ("prod", "f32", "real_range", 0.999, 1.001),
+    ("sum", "f64", "normal", 0.0, 1.0), ("min", "f64", "normal", 0.0, 1.0),
+    ("max", "f64", "normal", 0.0, 1.0),
+    ("prod", "f64", "real_range", 0.999, 1.001),
+]
+ESIZE = {"u64": 8, "i64": 8, "f32": 4, "f64": 8}
+
+
+def kernel_name(op, dtype, mode):
+    pol = {"f32": "PolF32Acc", "f64": "PolF64Acc"}
+    if dtype in pol and op in ("sum", "prod"):
+        return "k_reduce<%s,%s>" % (dtype, pol[dtype] if mode == 0
+                                    else "PolTree")
+    return "k_reduce<%s,PolSeq>" % dtype
+
+
+def cpu_baseline(seconds_budget=20.0):
+    """Oracle CPU restatement of the reference algorithm (rec_dbl, PEs as
+    threads, memcpy puts) on a bounded sample of the same matrix."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    threads = max(1, min(16, cores))
+    n = N // 16
+    total_b = 0.0
+    total_s = 0.0
+    t0 = time.time()
+    for op, dt, _, _, _ in CELLS:
+        sec = O.cpu_bench_reduce(op, dt, P, n, 3, threads)
+        total_b += 2.0 * P * n * ESIZE[dt]
+        total_s += sec
+        if time.time() - t0 > seconds_budget:
+            break
+    return {"value": round(total_b / total_s / 1e9, 3), "unit": "GB/s",
+            "cores": threads, "kind": "port",
+            "sample": "rec_dbl restatement (oracle/orc.c), 64 PEs as %d "
+                      "threads, nreduce=%d (1/16 of the workload), op/type "
+                      "matrix cells, median of 3" % (threads, n)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--fp-mode", choices=["accurate", "reference"],
+                    default="accurate")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the emulated PEs all live in one "
+              "MI355X's HBM; --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    L = osc.lib()
+    mode = 0 if args.fp_mode == "accurate" else 1
+    dists = sorted({(dt, dist, a, b) for _, dt, dist, a, b in CELLS})
+    per_pe = sum(N * ESIZE[d[0]] for d in dists) + N * 8 + (8 << 20)
+    h = osc.Heap(npes=P, heap_bytes=per_pe)
+    src = {}
+    for k, (dt, dist, a, b) in enumerate(dists):
+        src[(dt, dist, a, b)] = h.malloc(N * ESIZE[dt])
+        h.fill(None, src[(dt, dist, a, b)], N, dt, dist, a, b, seed=k + 1)
+    dest = h.malloc(N * 8)
+    h.sync()
+    osc.check(L.osc_set_fp_mode(mode), "fp mode")
+    stream = L.osc_default_stream()
+    evs = []
+    for _ in range(len(CELLS) + 1):
+        e = C.c_void_p()
+        osc.check(L.osc_event_create(C.byref(e)), "event")
+        evs.append(e)
+
+    def step(record):
+        for i, (op, dt, dist, a, b) in enumerate(CELLS):
+            if record:
+                L.osc_event_record(evs[i], stream)
+            rc = L.osc_reduce(h.world, osc.OPS[op], osc.DTYPES[dt], dest,
+                              src[(dt, dist, a, b)], N, stream)
+            osc.check(rc, "osc_reduce")
+        if record:
+            L.osc_event_record(evs[-1], stream)
+
+    for _ in range(args.warmup):
+        step(False)
+    h.sync()
+    cell_ms = [0.0] * len(CELLS)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+        h.sync()
+        ms = C.c_float()
+        for i in range(len(CELLS)):
+            L.osc_event_elapsed_ms(evs[i], evs[i + 1], C.byref(ms))
+            cell_ms[i] += ms.value
+    wall = time.perf_counter() - t0
+    kern_s = sum(cell_ms) / 1e3
+
+    step_bytes = sum(2.0 * P * N * ESIZE[dt] for _, dt, _, _, _ in CELLS)
+    value = step_bytes * args.steps / kern_s / 1e9
+    cells = []
+    for i, (op, dt, _, _, _) in enumerate(CELLS):
+        avg = cell_ms[i] / args.steps
+        b = 2.0 * P * N * ESIZE[dt]
+        cells.append({"op": op, "type": dt, "ms": round(avg, 4),
+                      "GBps": round(b / (avg / 1e3) / 1e9, 1)})
+    dom = max(range(len(CELLS)), key=lambda i: cell_ms[i])
+    dom_avg = cell_ms[dom] / args.steps / 1e3
+    dom_b = 2.0 * P * N * ESIZE[CELLS[dom][1]]
+    achieved = dom_b / dom_avg / 1e9
+
+    # plain-copy ceiling on the same arena: 2 GiB read + 2 GiB write
+    cp_bytes = 1 << 31
+    a_ = h.malloc(N * 8)  # 32 MiB per PE -> 2 GiB contiguous over 64 PEs
+    base = L.osc_heap_base()
+    half = (P // 2) * L.osc_slice_bytes()
+    t = osc.Timer(stream)
+    L.osc_device_copy(base + half, base, cp_bytes // 2, stream)
+    h.sync()
+    t.start()
+    for _ in range(5):
+        L.osc_device_copy(base + half, base, cp_bytes // 2, stream)
+    t.stop()
+    copy_gbps = 5 * cp_bytes / (t.ms() / 1e3) / 1e9
+    h.free(a_)
+
+    out = {
+        "metric": "reduce HBM GB/s (% peak) at 64 PEs x 4M; p50/p99 latency "
+                  "served vs per-call",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(kern_s * 1e3 / args.steps, 4),
+        "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "pct_peak": round(100.0 * value / PEAK_HBM_GBPS, 2),
+        "dtype": "u64/i64/f32/f64",
+        "data": "synthetic (seeded SplitMix64 on device)",
+        "config": {"workload": "team reduce op/type matrix, 15 cells, "
+                               "SHMEM_TEAM_WORLD of 64 PEs x 4194304 "
+                               "elements per PE",
+                   "npes": P, "nreduce": N, "fp_mode": args.fp_mode,
+                   "parallelism": "1 GPU, 64 emulated PEs"},
+        "roofline": {"bound": "hbm", "kernel": kernel_name(
+            CELLS[dom][0], CELLS[dom][1], mode) + " %s" % CELLS[dom][0],
+            "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS,
+            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": dom_b},
+        "copy_ceiling_GBps": round(copy_gbps, 1),
+        "cells": cells,
+    }
+    h.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

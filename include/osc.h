@@ -1,0 +1,163 @@
+/*
+ * osc.h - MI355X (gfx950) collective library: the thin launch API.
+ *
+ * The OpenSHMEM entry points in <shmem.h> are the drop-in boundary.  This
+ * header is the lower, stream-aware layer those entry points are built on,
+ * plus the pieces a benchmark, a test harness or a serving front end needs:
+ * explicit initialisation, float-accuracy mode, asynchronous team
+ * collectives on a HIP stream, host<->PE copies, a seeded synthetic-input
+ * generator, HIP-event timing, the batched (persistent, graph-captured)
+ * executor and the served request loop.
+ *
+ * Model.  All PEs of the job are emulated inside one GPU: PE p owns the
+ * symmetric-heap slice [base + p*slice, base + (p+1)*slice) of one device
+ * arena.  A symmetric address is any address inside PE p's slice; its
+ * offset from the slice start is the same on every PE.  One host call of a
+ * collective acts for every member of the team ("team-call" mode).
+ *
+ * Every function returns 0 (OSC_OK) on success or a negative OSC_ERR_* code;
+ * osc_strerror() names the code and osc_last_error() gives the last message.
+ */
+#ifndef OSC_H
+#define OSC_H 1
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+enum {
+  OSC_OK = 0,
+  OSC_ERR_ARG = -1,         /* bad argument (size, op, type, overlap)      */
+  OSC_ERR_NOTINIT = -2,     /* library not initialised                     */
+  OSC_ERR_TEAM = -3,        /* invalid team handle                         */
+  OSC_ERR_PTR = -4,         /* pointer is not inside the symmetric heap    */
+  OSC_ERR_NOMEM = -5,       /* symmetric heap or device memory exhausted   */
+  OSC_ERR_HIP = -6,         /* a HIP runtime call failed                   */
+  OSC_ERR_UNSUPPORTED = -7, /* type/op not available on gfx950             */
+  OSC_ERR_ALGO = -8,        /* unknown collective algorithm name           */
+  OSC_ERR_BUDGET = -9,      /* request exceeds the HBM admission budget    */
+  OSC_ERR_STATE = -10       /* call not valid in the current state         */
+};
+
+/* ---- reduction operators and element types ----------------------------- */
+typedef enum {
+  OSC_OP_AND = 0,
+  OSC_OP_OR,
+  OSC_OP_XOR,
+  OSC_OP_MIN,
+  OSC_OP_MAX,
+  OSC_OP_SUM,
+  OSC_OP_PROD,
+  OSC_OP_COUNT
+} osc_op_t;
+
+/* Element types as the kernels see them.  Signedness only matters for
+ * MIN/MAX; SUM/PROD on integers wrap modulo 2^bits as two's complement. */
+typedef enum {
+  OSC_T_U8 = 0,
+  OSC_T_U16,
+  OSC_T_U32,
+  OSC_T_U64,
+  OSC_T_I8,
+  OSC_T_I16,
+  OSC_T_I32,
+  OSC_T_I64,
+  OSC_T_F32,
+  OSC_T_F64,
+  OSC_T_C32, /* float _Complex  */
+  OSC_T_C64, /* double _Complex */
+  OSC_T_COUNT
+} osc_dtype_t;
+
+/* Float SUM/PROD accuracy mode.
+ *  ACCURATE : FP32 accumulates in FP64, FP64 in double-double; one final
+ *             rounding.  Order fixed (team index 0..P-1), so results are
+ *             reproducible run to run and call to call.
+ *  REFERENCE: the type's own arithmetic in exactly the combine order of the
+ *             recursive-doubling reduction (fold of non-power-of-two
+ *             members, then the XOR butterfly): bit-identical to it.
+ * MIN/MAX always reproduce the butterfly's per-PE selection semantics;
+ * integer ops are exact in both modes. */
+typedef enum { OSC_FP_ACCURATE = 0, OSC_FP_REFERENCE = 1 } osc_fpmode_t;
+
+/* Synthetic-input distributions for osc_fill().  Every value is a pure
+ * function of (seed, global PE, element index) through the SplitMix64
+ * finaliser, so inputs do not depend on team size or launch shape. */
+typedef enum {
+  OSC_DIST_BITS = 0,     /* raw 64-bit hash, truncated to the type          */
+  OSC_DIST_INT_RANGE,    /* integer uniform in [a, b] (a, b integers)       */
+  OSC_DIST_SET_M1_1_2,   /* integer uniform over {-1, 1, 2}                 */
+  OSC_DIST_NORMAL,       /* N(a, b^2) via Box-Muller, computed in FP64      */
+  OSC_DIST_REAL_RANGE,   /* real uniform in [a, b), computed in FP64        */
+  OSC_DIST_A2A_PATTERN,  /* (src_pe<<40 | dst_pe<<20 | idx), a = block elems */
+  OSC_DIST_FCOL_PATTERN, /* (uint32)(pe<<24 | idx)                          */
+  OSC_DIST_COUNT
+} osc_dist_t;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+/* Create npes emulated PEs with heap_bytes_per_pe of symmetric heap each on
+ * HIP device `device`.  shmem_init() calls this with OSC_NPES (default 8)
+ * and SHMEM_SYMMETRIC_SIZE (default 32M). */
+int osc_init(int npes, size_t heap_bytes_per_pe, int device);
+int osc_finalize(void);
+int osc_initialized(void);
+int osc_npes(void);
+size_t osc_slice_bytes(void); /* distance between consecutive PE slices */
+size_t osc_heap_bytes(void);  /* usable symmetric bytes per PE          */
+void *osc_heap_base(void);    /* PE 0's slice                            */
+void *osc_default_stream(void);
+int osc_set_fp_mode(int mode);
+int osc_get_fp_mode(void);
+const char *osc_strerror(int code);
+const char *osc_last_error(void);
+int osc_set_my_pe(int pe); /* which emulated PE the calling thread is */
+
+/* ---- symmetric addresses ------------------------------------------------ */
+/* Split a symmetric address into (pe, offset).  Returns OSC_ERR_PTR if the
+ * address is outside the heap. */
+int osc_sym_offset(const void *sym, int *pe, size_t *offset);
+void *osc_pe_ptr(const void *sym, int pe); /* same object on PE pe */
+int osc_put_host(void *sym_dest, const void *host_src, size_t bytes, int pe);
+int osc_get_host(void *host_dest, const void *sym_src, size_t bytes, int pe);
+
+/* ---- asynchronous team collectives (enqueue on `stream`; NULL = library
+ * stream).  dest/src are symmetric addresses; nbytes counts one block. --- */
+int osc_reduce(void *team, int op, int dtype, void *dest, const void *src,
+               size_t nreduce, void *stream);
+int osc_fcollect(void *team, void *dest, const void *src, size_t nbytes,
+                 void *stream);
+int osc_alltoall(void *team, void *dest, const void *src, size_t nbytes,
+                 void *stream);
+int osc_broadcast(void *team, void *dest, const void *src, size_t nbytes,
+                  int root, void *stream);
+int osc_stream_sync(void *stream);
+
+/* ---- synthetic inputs --------------------------------------------------- */
+int osc_fill(void *team, void *sym, size_t nelems, int dtype, int dist,
+             double a, double b, uint64_t seed, void *stream);
+
+/* ---- timing ------------------------------------------------------------- */
+int osc_event_create(void **ev);
+int osc_event_destroy(void *ev);
+int osc_event_record(void *ev, void *stream);
+int osc_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* ---- configuration helpers (pure host code, usable without a GPU) ------- */
+/* Parse a heap size such as "32M", "1G", "4096" (binary K/M/G/T). */
+int osc_parse_size(const char *s, size_t *out);
+/* Check a collective algorithm spec "algo" or "algo:type" for one of
+ * "reduce", "fcollect", "alltoall", "broadcast"; for reductions *fpmode
+ * (may be NULL) receives the float mode the name selects. */
+int osc_algo_resolve(const char *kind, const char *spec, int *fpmode);
+
+/* ---- raw device copy (the roofline calibration kernel) ------------------ */
+int osc_device_copy(void *dst, const void *src, size_t bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OSC_H */

@@ -1,0 +1,669 @@
+/*
+ * orc.c - CPU oracle for the collective hot path.  TEST INFRASTRUCTURE ONLY:
+ * used by tests/, the smoke check and the benchmark's CPU baseline, never
+ * by the product library.
+ *
+ * Each PE's memory is a separate host buffer; the algorithms below perform,
+ * for all PEs, the same puts/gets and local combines, in the same order, as
+ * the reference code cited at each function.  Built with -ffp-contract=off
+ * on x86-64 (SSE2: no excess precision), as the reference is.
+ */
+#define _GNU_SOURCE
+#include "orc.h"
+
+#include <complex.h>
+#include <math.h>
+#include <pthread.h>
+#include <sched.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+enum { AND, OR, XOR, MIN, MAX, SUM, PROD };
+enum { U8, U16, U32, U64, I8, I16, I32, I64, F32, F64, C32, C64, NTYPES };
+
+static size_t esize(int t) {
+  static const size_t sz[NTYPES] = {1, 2, 4, 8, 1, 2, 4, 8, 4, 8, 8, 16};
+  return (t < 0 || t >= NTYPES) ? 0 : sz[t];
+}
+
+/* ---- local_<T>_<op>_reduce: reduce.c:35-43 with the operators of
+ *      reduce.c:730-736.  Integer + and * wrap (the reference's C signed
+ *      arithmetic on x86-64 wraps in practice); WT is wide enough that the
+ *      product of two promoted values cannot overflow a signed int. ---- */
+#define DEF_LOCAL_INT(NAME, ST, UT, WT)                                        \
+  static int local_##NAME(int op, void *d_, const void *a_, const void *b_,    \
+                          size_t n) {                                          \
+    ST *d = (ST *)d_;                                                          \
+    const ST *a = (const ST *)a_, *b = (const ST *)b_;                         \
+    size_t i;                                                                  \
+    switch (op) {                                                              \
+      case AND: for (i = 0; i < n; i++) d[i] = (ST)((UT)a[i] & (UT)b[i]); break; \
+      case OR: for (i = 0; i < n; i++) d[i] = (ST)((UT)a[i] | (UT)b[i]); break;  \
+      case XOR: for (i = 0; i < n; i++) d[i] = (ST)((UT)a[i] ^ (UT)b[i]); break; \
+      case MIN: for (i = 0; i < n; i++) d[i] = (a[i] < b[i]) ? a[i] : b[i]; break; \
+      case MAX: for (i = 0; i < n; i++) d[i] = (a[i] > b[i]) ? a[i] : b[i]; break; \
+      case SUM:                                                                \
+        for (i = 0; i < n; i++) d[i] = (ST)(UT)((WT)(UT)a[i] + (WT)(UT)b[i]);  \
+        break;                                                                 \
+      case PROD:                                                               \
+        for (i = 0; i < n; i++) d[i] = (ST)(UT)((WT)(UT)a[i] * (WT)(UT)b[i]);  \
+        break;                                                                 \
+      default: return -1;                                                      \
+    }                                                                          \
+    return 0;                                                                  \
+  }
+DEF_LOCAL_INT(u8, uint8_t, uint8_t, unsigned)
+DEF_LOCAL_INT(u16, uint16_t, uint16_t, unsigned)
+DEF_LOCAL_INT(u32, uint32_t, uint32_t, uint32_t)
+DEF_LOCAL_INT(u64, uint64_t, uint64_t, uint64_t)
+DEF_LOCAL_INT(i8, int8_t, uint8_t, unsigned)
+DEF_LOCAL_INT(i16, int16_t, uint16_t, unsigned)
+DEF_LOCAL_INT(i32, int32_t, uint32_t, uint32_t)
+DEF_LOCAL_INT(i64, int64_t, uint64_t, uint64_t)
+
+#define DEF_LOCAL_REAL(NAME, T)                                                \
+  static int local_##NAME(int op, void *d_, const void *a_, const void *b_,    \
+                          size_t n) {                                          \
+    T *d = (T *)d_;                                                            \
+    const T *a = (const T *)a_, *b = (const T *)b_;                            \
+    size_t i;                                                                  \
+    switch (op) {                                                              \
+      case MIN: for (i = 0; i < n; i++) d[i] = (a[i] < b[i]) ? a[i] : b[i]; break; \
+      case MAX: for (i = 0; i < n; i++) d[i] = (a[i] > b[i]) ? a[i] : b[i]; break; \
+      case SUM: for (i = 0; i < n; i++) d[i] = a[i] + b[i]; break;             \
+      case PROD: for (i = 0; i < n; i++) d[i] = a[i] * b[i]; break;            \
+      default: return -1;                                                      \
+    }                                                                          \
+    return 0;                                                                  \
+  }
+DEF_LOCAL_REAL(f32, float)
+DEF_LOCAL_REAL(f64, double)
+
+/* C99 complex arithmetic exactly as the reference compiles it */
+#define DEF_LOCAL_CPLX(NAME, T)                                                \
+  static int local_##NAME(int op, void *d_, const void *a_, const void *b_,    \
+                          size_t n) {                                          \
+    T *d = (T *)d_;                                                            \
+    const T *a = (const T *)a_, *b = (const T *)b_;                            \
+    size_t i;                                                                  \
+    switch (op) {                                                              \
+      case SUM: for (i = 0; i < n; i++) d[i] = a[i] + b[i]; break;             \
+      case PROD: for (i = 0; i < n; i++) d[i] = a[i] * b[i]; break;            \
+      default: return -1;                                                      \
+    }                                                                          \
+    return 0;                                                                  \
+  }
+DEF_LOCAL_CPLX(c32, float _Complex)
+DEF_LOCAL_CPLX(c64, double _Complex)
+
+typedef int (*local_fn)(int, void *, const void *, const void *, size_t);
+
+static local_fn pick_local(int dtype) {
+  static const local_fn f[NTYPES] = {local_u8,  local_u16, local_u32, local_u64,
+                                     local_i8,  local_i16, local_i32, local_i64,
+                                     local_f32, local_f64, local_c32, local_c64};
+  return (dtype < 0 || dtype >= NTYPES) ? NULL : f[dtype];
+}
+
+/* ---- REDUCE_HELPER_REC_DBL, reduce.c:182-293, for all PEs of the active
+ *      set at once.  Members of the power-of-two set are me_as with
+ *      ceil(floor(me_as*p2s/P)*P/p2s) == me_as (:207-214). ---- */
+static int p2s_of(int P) {
+  int p2s;
+  for (p2s = 1; p2s * 2 <= P; p2s *= 2) {
+  }
+  return p2s;
+}
+
+static int member_p2s(int me, int P, int p2s) {
+  int m = me * p2s / P;
+  return ((m * P + p2s - 1) / p2s == me) ? m : -1;
+}
+
+int orc_reduce_rec_dbl(int op, int dtype, int P, size_t n,
+                       const void *const *src, void *const *dest) {
+  const local_fn f = pick_local(dtype);
+  const size_t nb = n * esize(dtype);
+  int p2s, me, mask, rc = 0;
+  int *mp;
+  char **tmp;
+  if (f == NULL || P < 1) return -1;
+  p2s = p2s_of(P);
+  mp = (int *)malloc(sizeof(int) * (size_t)P);
+  tmp = (char **)calloc((size_t)P, sizeof(char *));
+  if (mp == NULL || tmp == NULL) {
+    free(mp);
+    free(tmp);
+    return -1;
+  }
+  for (me = 0; me < P; ++me) mp[me] = member_p2s(me, P, p2s);
+  /* :218-244 tmp buffer; a member whose right neighbour is outside the
+   * power-of-two set pulls that neighbour's source into its dest and
+   * computes tmp = op(dest, source) = op(x[me+1], x[me]). */
+  for (me = 0; me < P; ++me) {
+    if (mp[me] < 0) continue;
+    tmp[me] = (char *)malloc(nb ? nb : 1);
+    if (tmp[me] == NULL) {
+      rc = -1;
+      goto out;
+    }
+    if ((me + 1) * p2s / P == mp[me]) {
+      memcpy(dest[me], src[me + 1], nb);
+      rc |= f(op, tmp[me], dest[me], src[me], n);
+    } else {
+      memcpy(tmp[me], src[me], nb);
+    }
+  }
+  /* :250-272 butterfly: in round `mask` every member receives its
+   * partner's tmp into dest (putmem) and then computes tmp = op(tmp, dest).
+   * All puts of a round land before any combine of that round. */
+  for (mask = 1; mask < p2s; mask <<= 1) {
+    for (me = 0; me < P; ++me) {
+      int peer_p2s, peer_as;
+      if (mp[me] < 0) continue;
+      peer_p2s = mp[me] ^ mask;
+      peer_as = (peer_p2s * P + p2s - 1) / p2s;
+      memcpy(dest[me], tmp[peer_as], nb);
+    }
+    for (me = 0; me < P; ++me)
+      if (mp[me] >= 0) rc |= f(op, tmp[me], tmp[me], dest[me], n);
+  }
+  /* :274-288 final copy; the folded neighbour receives the member's dest */
+  for (me = 0; me < P; ++me)
+    if (mp[me] >= 0) memcpy(dest[me], tmp[me], nb);
+  for (me = 0; me < P; ++me)
+    if (mp[me] >= 0 && (me + 1) * p2s / P == mp[me])
+      memcpy(dest[me + 1], dest[me], nb);
+out:
+  for (me = 0; me < P; ++me) free(tmp[me]);
+  free(tmp);
+  free(mp);
+  return rc;
+}
+
+/* ---- high-precision references ------------------------------------------ */
+
+/* Shewchuk's exact expansion sum with the correctly rounded final step, as
+ * Python's math.fsum (J. R. Shewchuk, "Adaptive Precision Floating-Point
+ * Arithmetic and Fast Robust Geometric Predicates", 1997). */
+static double fsum_partials(const double *x, size_t m, double *p, size_t *np_) {
+  size_t np = 0, i, j;
+  for (i = 0; i < m; ++i) {
+    double v = x[i];
+    size_t k = 0;
+    for (j = 0; j < np; ++j) {
+      double y = p[j], hi, lo;
+      if (fabs(v) < fabs(y)) {
+        double t = v;
+        v = y;
+        y = t;
+      }
+      hi = v + y;
+      lo = y - (hi - v);
+      if (lo != 0.0) p[k++] = lo;
+      v = hi;
+    }
+    p[k++] = v;
+    np = k;
+  }
+  *np_ = np;
+  {
+    double hi = 0.0;
+    if (np > 0) {
+      size_t k = np;
+      double lo = 0.0, yr;
+      hi = p[--k];
+      while (k > 0) {
+        double v = hi, y = p[--k];
+        hi = v + y;
+        yr = hi - v;
+        lo = y - yr;
+        if (lo != 0.0) break;
+      }
+      /* round-half-even correction across the remaining partials */
+      if (k > 0 && ((lo < 0 && p[k - 1] < 0) || (lo > 0 && p[k - 1] > 0))) {
+        double y = lo * 2.0, v = hi + y;
+        yr = v - hi;
+        if (y == yr) hi = v;
+      }
+    }
+    return hi;
+  }
+}
+
+double orc_fsum(const double *x, size_t m) {
+  double *p = (double *)malloc(sizeof(double) * (m + 1));
+  size_t np;
+  double r;
+  if (p == NULL) return NAN;
+  r = fsum_partials(x, m, p, &np);
+  free(p);
+  return r;
+}
+
+/* exact sum as hi (correctly rounded) + lo (rounded remainder) */
+static void fsum_dd(double *x, size_t m, double *hi, double *lo) {
+  double h = orc_fsum(x, m);
+  x[m] = -h;
+  *hi = h;
+  *lo = orc_fsum(x, m + 1);
+}
+
+typedef struct {
+  double hi, lo;
+} dd_t;
+static dd_t two_sum(double a, double b) {
+  double s = a + b, bb = s - a;
+  dd_t r = {s, (a - (s - bb)) + (b - bb)};
+  return r;
+}
+static dd_t fast_two_sum(double a, double b) {
+  double s = a + b;
+  dd_t r = {s, b - (s - a)};
+  return r;
+}
+static dd_t dd_mul_d(dd_t x, double y) {
+  double p = x.hi * y;
+  double e = fma(x.hi, y, -p);
+  e = fma(x.lo, y, e);
+  return fast_two_sum(p, e);
+}
+static dd_t dd_add(dd_t x, dd_t y) {
+  dd_t s = two_sum(x.hi, y.hi), t = two_sum(x.lo, y.lo);
+  s.lo += t.hi;
+  s = fast_two_sum(s.hi, s.lo);
+  s.lo += t.lo;
+  return fast_two_sum(s.hi, s.lo);
+}
+
+static double get_real(const void *p, int dtype, size_t i, int comp) {
+  switch (dtype) {
+    case F32: return (double)((const float *)p)[i];
+    case F64: return ((const double *)p)[i];
+    case C32: return (double)((const float *)p)[2 * i + comp];
+    case C64: return ((const double *)p)[2 * i + comp];
+  }
+  return NAN;
+}
+
+int orc_reduce_accurate(int op, int dtype, int P, size_t n,
+                        const void *const *src, double *hi, double *lo) {
+  const int cplx = (dtype == C32 || dtype == C64);
+  size_t i;
+  int k;
+  double *buf;
+  if (dtype != F32 && dtype != F64 && !cplx) return -1;
+  if (op != SUM && op != PROD) return -1;
+  buf = (double *)malloc(sizeof(double) * ((size_t)P + 2));
+  if (buf == NULL) return -1;
+  for (i = 0; i < n; ++i) {
+    if (op == SUM) {
+      int comp;
+      for (comp = 0; comp < (cplx ? 2 : 1); ++comp) {
+        size_t o = cplx ? 2 * i + comp : i;
+        for (k = 0; k < P; ++k) buf[k] = get_real(src[k], dtype, i, comp);
+        fsum_dd(buf, (size_t)P, &hi[o], &lo[o]);
+      }
+    } else if (!cplx) {
+      dd_t acc = {get_real(src[0], dtype, i, 0), 0.0};
+      for (k = 1; k < P; ++k) acc = dd_mul_d(acc, get_real(src[k], dtype, i, 0));
+      hi[i] = acc.hi;
+      lo[i] = acc.lo;
+    } else {
+      dd_t re = {get_real(src[0], dtype, i, 0), 0.0};
+      dd_t im = {get_real(src[0], dtype, i, 1), 0.0};
+      for (k = 1; k < P; ++k) {
+        const double xr = get_real(src[k], dtype, i, 0);
+        const double xi = get_real(src[k], dtype, i, 1);
+        dd_t a = dd_mul_d(re, xr), b = dd_mul_d(im, xi);
+        dd_t c = dd_mul_d(re, xi), d = dd_mul_d(im, xr);
+        b.hi = -b.hi;
+        b.lo = -b.lo;
+        re = dd_add(a, b);
+        im = dd_add(c, d);
+      }
+      hi[2 * i] = re.hi;
+      lo[2 * i] = re.lo;
+      hi[2 * i + 1] = im.hi;
+      lo[2 * i + 1] = im.lo;
+    }
+  }
+  free(buf);
+  return 0;
+}
+
+/* ---- data movement ------------------------------------------------------- */
+
+/* util/rotate.c:22-54: right-rotate arr by dist bytes along gcd cycles */
+void orc_rotate_inplace(char *arr, size_t size, size_t dist) {
+  size_t i, groups, a, b;
+  if (dist == 0) return;
+  a = dist;
+  b = size;
+  for (;;) {
+    size_t r = a % b;
+    if (r == 0) break;
+    a = b;
+    b = r;
+  }
+  groups = b;
+  for (i = 0; i < groups; i++) {
+    char temp = arr[i];
+    size_t j = i;
+    for (;;) {
+      size_t k = (j < dist) ? size - dist + j : j - dist;
+      if (k == i) break;
+      arr[j] = arr[k];
+      j = k;
+    }
+    arr[j] = temp;
+  }
+}
+
+/* fcollect.c:370-403 fcollect_helper_bruck_inplace for all PEs.  In round
+ * `distance` PE me puts dest[0, to_send) into peer (me - distance) at
+ * offset sent_bytes; the source and target ranges of one PE never overlap
+ * (to_send <= sent_bytes), so a round's puts commute. */
+int orc_fcollect_bruck(int P, size_t nbytes, const void *const *src,
+                       void *const *dest) {
+  const size_t total = (size_t)P * nbytes;
+  size_t distance, sent = nbytes;
+  int me;
+  for (me = 0; me < P; ++me) memcpy(dest[me], src[me], nbytes);
+  for (distance = 1; distance < (size_t)P; distance <<= 1) {
+    const size_t to_send = (2 * sent <= total) ? sent : total - sent;
+    for (me = 0; me < P; ++me) {
+      const int peer = (int)(((size_t)me - distance + (size_t)P) % (size_t)P);
+      memcpy((char *)dest[peer] + sent, dest[me], to_send);
+    }
+    sent += distance * nbytes;
+  }
+  for (me = 0; me < P; ++me)
+    orc_rotate_inplace((char *)dest[me], total, (size_t)me * nbytes);
+  return 0;
+}
+
+/* alltoall.c:75-110 with SHIFT_PEER (:206): own block by memcpy, then
+ * P-1 puts src_me[block peer] -> dest_peer[block me], peer = me + i. */
+int orc_alltoall_shift(int P, size_t nbytes, const void *const *src,
+                       void *const *dest) {
+  int me, i;
+  for (me = 0; me < P; ++me) {
+    memcpy((char *)dest[me] + (size_t)me * nbytes,
+           (const char *)src[me] + (size_t)me * nbytes, nbytes);
+    for (i = 1; i < P; ++i) {
+      const int peer = (me + i) % P;
+      memcpy((char *)dest[peer] + (size_t)me * nbytes,
+             (const char *)src[me] + (size_t)peer * nbytes, nbytes);
+    }
+  }
+  return 0;
+}
+
+/* broadcast.c:527-531 (root copies its source) + broadcast_helper_linear
+ * (:58-71: every other PE gets from the root) */
+int orc_broadcast_linear(int P, int root, size_t nbytes,
+                         const void *const *src, void *const *dest) {
+  int me;
+  if (root < 0 || root >= P) return -1;
+  memcpy(dest[root], src[root], nbytes);
+  for (me = 0; me < P; ++me)
+    if (me != root) memcpy(dest[me], src[root], nbytes);
+  return 0;
+}
+
+/* ---- generator (same function as the device fill kernel) ---------------- */
+static uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+uint64_t orc_hash3(uint64_t seed, uint64_t pe, uint64_t idx) {
+  const uint64_t G = 0x9E3779B97F4A7C15ull;
+  return mix64(mix64(mix64(seed) + G * (pe + 1)) + G * (idx + 1));
+}
+
+/* ---- CPU baseline: the reference algorithms with PE-parallel threads ---- */
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+typedef struct {
+  int kind; /* 0 reduce, 1 fcollect, 2 alltoall */
+  int op, dtype, P, p2s, T, iters;
+  size_t n, nb;
+  char **src, **dest, **tmp;
+  int *mp;
+  local_fn f;
+  pthread_barrier_t bar;
+  double *times;
+} bench_t;
+
+typedef struct {
+  bench_t *b;
+  int tid;
+} worker_t;
+
+static void reduce_phases(bench_t *b, int tid) {
+  const int P = b->P, p2s = b->p2s, T = b->T;
+  int me, mask;
+  /* per-call tmp allocation as the reference (:218-225) */
+  for (me = tid; me < P; me += T)
+    if (b->mp[me] >= 0) b->tmp[me] = (char *)malloc(b->nb);
+  for (me = tid; me < P; me += T) {
+    if (b->mp[me] < 0) continue;
+    if ((me + 1) * p2s / P == b->mp[me]) {
+      memcpy(b->dest[me], b->src[me + 1], b->nb);
+      b->f(b->op, b->tmp[me], b->dest[me], b->src[me], b->n);
+    } else {
+      memcpy(b->tmp[me], b->src[me], b->nb);
+    }
+  }
+  pthread_barrier_wait(&b->bar);
+  for (mask = 1; mask < p2s; mask <<= 1) {
+    for (me = tid; me < P; me += T) {
+      int pp, pa;
+      if (b->mp[me] < 0) continue;
+      pp = b->mp[me] ^ mask;
+      pa = (pp * P + p2s - 1) / p2s;
+      memcpy(b->dest[pa], b->tmp[me], b->nb); /* put my tmp to the peer */
+    }
+    pthread_barrier_wait(&b->bar);
+    for (me = tid; me < P; me += T)
+      if (b->mp[me] >= 0) b->f(b->op, b->tmp[me], b->tmp[me], b->dest[me], b->n);
+    pthread_barrier_wait(&b->bar);
+  }
+  for (me = tid; me < P; me += T)
+    if (b->mp[me] >= 0) memcpy(b->dest[me], b->tmp[me], b->nb);
+  pthread_barrier_wait(&b->bar);
+  for (me = tid; me < P; me += T)
+    if (b->mp[me] >= 0 && (me + 1) * p2s / P == b->mp[me])
+      memcpy(b->dest[me + 1], b->dest[me], b->nb);
+  for (me = tid; me < P; me += T) {
+    free(b->tmp[me]);
+    b->tmp[me] = NULL;
+  }
+}
+
+static void fcollect_phases(bench_t *b, int tid) {
+  const int P = b->P, T = b->T;
+  const size_t total = (size_t)P * b->nb;
+  size_t distance, sent = b->nb;
+  int me;
+  for (me = tid; me < P; me += T) memcpy(b->dest[me], b->src[me], b->nb);
+  pthread_barrier_wait(&b->bar);
+  for (distance = 1; distance < (size_t)P; distance <<= 1) {
+    const size_t to_send = (2 * sent <= total) ? sent : total - sent;
+    for (me = tid; me < P; me += T) {
+      const int peer = (int)(((size_t)me - distance + (size_t)P) % (size_t)P);
+      memcpy(b->dest[peer] + sent, b->dest[me], to_send);
+    }
+    sent += distance * b->nb;
+    pthread_barrier_wait(&b->bar);
+  }
+  for (me = tid; me < P; me += T)
+    orc_rotate_inplace(b->dest[me], total, (size_t)me * b->nb);
+}
+
+static void alltoall_phases(bench_t *b, int tid) {
+  const int P = b->P, T = b->T;
+  int me, i;
+  for (me = tid; me < P; me += T) {
+    memcpy(b->dest[me] + (size_t)me * b->nb, b->src[me] + (size_t)me * b->nb, b->nb);
+    for (i = 1; i < P; ++i) {
+      const int peer = (me + i) % P;
+      memcpy(b->dest[peer] + (size_t)me * b->nb, b->src[me] + (size_t)peer * b->nb,
+             b->nb);
+    }
+  }
+}
+
+static void *bench_worker(void *arg) {
+  worker_t *w = (worker_t *)arg;
+  bench_t *b = w->b;
+  int it;
+  for (it = 0; it < b->iters + 1; ++it) { /* first run warms up */
+    double t0;
+    pthread_barrier_wait(&b->bar);
+    t0 = now_s();
+    if (b->kind == 0) reduce_phases(b, w->tid);
+    else if (b->kind == 1) fcollect_phases(b, w->tid);
+    else alltoall_phases(b, w->tid);
+    pthread_barrier_wait(&b->bar);
+    if (w->tid == 0 && it > 0) b->times[it - 1] = now_s() - t0;
+  }
+  return NULL;
+}
+
+static int cmp_d(const void *a, const void *b) {
+  const double x = *(const double *)a, y = *(const double *)b;
+  return (x > y) - (x < y);
+}
+
+static void fill_src(bench_t *b, size_t bytes_per_pe) {
+  int me;
+  for (me = 0; me < b->P; ++me) {
+    size_t i;
+    if (b->dtype == F32 || b->dtype == F64) {
+      const size_t cnt = bytes_per_pe / esize(b->dtype);
+      for (i = 0; i < cnt; ++i) {
+        const uint64_t h = orc_hash3(0, (uint64_t)me, i);
+        const double u1 = (double)((h >> 11) + 1) * 0x1.0p-53;
+        const double u2 = (double)(mix64(h ^ 0xD1B54A32D192ED03ull) >> 11) * 0x1.0p-53;
+        const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+        if (b->dtype == F32)
+          ((float *)b->src[me])[i] = (float)z;
+        else
+          ((double *)b->src[me])[i] = z;
+      }
+    } else {
+      uint64_t *p = (uint64_t *)b->src[me];
+      for (i = 0; i < bytes_per_pe / 8; ++i) p[i] = orc_hash3(0, (uint64_t)me, i);
+    }
+  }
+}
+
+static int run_bench(bench_t *b, size_t src_bytes, size_t dest_bytes,
+                     double *sec_median) {
+  pthread_t *th;
+  worker_t *w;
+  int i, rc = 0;
+  b->src = (char **)calloc((size_t)b->P, sizeof(char *));
+  b->dest = (char **)calloc((size_t)b->P, sizeof(char *));
+  b->tmp = (char **)calloc((size_t)b->P, sizeof(char *));
+  b->times = (double *)calloc((size_t)b->iters, sizeof(double));
+  th = (pthread_t *)calloc((size_t)b->T, sizeof(pthread_t));
+  w = (worker_t *)calloc((size_t)b->T, sizeof(worker_t));
+  if (!b->src || !b->dest || !b->tmp || !b->times || !th || !w) {
+    rc = -1;
+    goto out;
+  }
+  for (i = 0; i < b->P; ++i) {
+    b->src[i] = (char *)aligned_alloc(64, (src_bytes + 63) / 64 * 64);
+    b->dest[i] = (char *)aligned_alloc(64, (dest_bytes + 63) / 64 * 64);
+    if (!b->src[i] || !b->dest[i]) {
+      rc = -1;
+      goto out;
+    }
+    memset(b->dest[i], 0, dest_bytes);
+  }
+  fill_src(b, src_bytes);
+  pthread_barrier_init(&b->bar, NULL, (unsigned)b->T);
+  for (i = 0; i < b->T; ++i) {
+    w[i].b = b;
+    w[i].tid = i;
+    pthread_create(&th[i], NULL, bench_worker, &w[i]);
+  }
+  for (i = 0; i < b->T; ++i) pthread_join(th[i], NULL);
+  pthread_barrier_destroy(&b->bar);
+  qsort(b->times, (size_t)b->iters, sizeof(double), cmp_d);
+  *sec_median = b->times[b->iters / 2];
+out:
+  if (b->src)
+    for (i = 0; i < b->P; ++i) {
+      free(b->src[i]);
+      free(b->dest[i]);
+    }
+  free(b->src);
+  free(b->dest);
+  free(b->tmp);
+  free(b->times);
+  free(th);
+  free(w);
+  free(b->mp);
+  return rc;
+}
+
+int orc_cpu_bench_reduce(int op, int dtype, int npes, size_t n, int iters,
+                         int nthreads, double *sec_median) {
+  bench_t b;
+  int me;
+  memset(&b, 0, sizeof(b));
+  if (npes < 1 || iters < 1 || nthreads < 1 || pick_local(dtype) == NULL) return -1;
+  b.kind = 0;
+  b.op = op;
+  b.dtype = dtype;
+  b.P = npes;
+  b.p2s = p2s_of(npes);
+  b.T = nthreads > npes ? npes : nthreads;
+  b.iters = iters;
+  b.n = n;
+  b.nb = n * esize(dtype);
+  b.f = pick_local(dtype);
+  b.mp = (int *)malloc(sizeof(int) * (size_t)npes);
+  if (b.mp == NULL) return -1;
+  for (me = 0; me < npes; ++me) b.mp[me] = member_p2s(me, npes, b.p2s);
+  return run_bench(&b, b.nb, b.nb, sec_median);
+}
+
+int orc_cpu_bench_fcollect(int npes, size_t nbytes, int iters, int nthreads,
+                           double *sec_median) {
+  bench_t b;
+  memset(&b, 0, sizeof(b));
+  if (npes < 1 || iters < 1 || nthreads < 1) return -1;
+  b.kind = 1;
+  b.P = npes;
+  b.T = nthreads > npes ? npes : nthreads;
+  b.iters = iters;
+  b.nb = nbytes;
+  b.dtype = U64;
+  return run_bench(&b, nbytes, nbytes * (size_t)npes, sec_median);
+}
+
+int orc_cpu_bench_alltoall(int npes, size_t nbytes, int iters, int nthreads,
+                           double *sec_median) {
+  bench_t b;
+  memset(&b, 0, sizeof(b));
+  if (npes < 1 || iters < 1 || nthreads < 1) return -1;
+  b.kind = 2;
+  b.P = npes;
+  b.T = nthreads > npes ? npes : nthreads;
+  b.iters = iters;
+  b.nb = nbytes;
+  b.dtype = U64;
+  return run_bench(&b, nbytes * (size_t)npes, nbytes * (size_t)npes, sec_median);
+}

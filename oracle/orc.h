@@ -1,0 +1,70 @@
+/*
+ * orc.h - CPU oracle for the collective hot path.  TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference's default SHCOLL algorithms, run
+ * for all PEs of a team inside one process (each PE = one host buffer),
+ * step by step as the reference executes them:
+ *   orc_reduce_rec_dbl   src/shcoll/src/reduce.c:182-293 (+ :35-43, :730-736)
+ *   orc_fcollect_bruck   src/shcoll/src/fcollect.c:370-403 + util/rotate.c:22-54
+ *   orc_alltoall_shift   src/shcoll/src/alltoall.c:75-110, SHIFT_PEER :206
+ *   orc_broadcast_linear src/shcoll/src/broadcast.c:58-71, :527-531
+ * plus high-precision references for float SUM/PROD:
+ *   orc_reduce_accurate  exact (Shewchuk expansion) sums, double-double
+ *                        products; result as an unevaluated hi + lo pair
+ * and a threaded CPU timing harness of the same reference algorithms
+ * (orc_cpu_bench_*), used as the CPU baseline.
+ *
+ * Only the tests, the smoke check and the benchmark's CPU-baseline leg use
+ * this library; the product (libosc) never links or calls it.
+ *
+ * Element types and ops use the numbering of include/osc.h.
+ */
+#ifndef ORC_H
+#define ORC_H 1
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Reference reduce: src[i], dest[i] are PE i's buffers of n elements. */
+int orc_reduce_rec_dbl(int op, int dtype, int npes, size_t n,
+                       const void *const *src, void *const *dest);
+
+/* High-precision SUM/PROD of the npes inputs per element.  For complex
+ * types the outputs hold 2n values (re, im interleaved). */
+int orc_reduce_accurate(int op, int dtype, int npes, size_t n,
+                        const void *const *src, double *hi, double *lo);
+
+/* Exactly rounded double sum of x[0..m) (Shewchuk 1997 / fsum). */
+double orc_fsum(const double *x, size_t m);
+
+int orc_fcollect_bruck(int npes, size_t nbytes, const void *const *src,
+                       void *const *dest);
+int orc_alltoall_shift(int npes, size_t nbytes, const void *const *src,
+                       void *const *dest);
+int orc_broadcast_linear(int npes, int root, size_t nbytes,
+                         const void *const *src, void *const *dest);
+/* util/rotate.c:22-54 byte juggling rotation, exposed for its own test */
+void orc_rotate_inplace(char *arr, size_t size, size_t dist);
+
+/* SplitMix64-based input generator shared with the device fill kernel:
+ * integer distributions are bit-identical to it. */
+uint64_t orc_hash3(uint64_t seed, uint64_t pe, uint64_t idx);
+
+/* CPU baseline: run the reference algorithm with npes PEs spread over
+ * nthreads threads (barrier between rounds), `iters` timed repetitions on
+ * seeded inputs; *sec_median receives the median seconds per call. */
+int orc_cpu_bench_reduce(int op, int dtype, int npes, size_t n, int iters,
+                         int nthreads, double *sec_median);
+int orc_cpu_bench_fcollect(int npes, size_t nbytes, int iters, int nthreads,
+                           double *sec_median);
+int orc_cpu_bench_alltoall(int npes, size_t nbytes, int iters, int nthreads,
+                           double *sec_median);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,163 @@
+// dev_runtime.hip - HIP runtime calls behind the C-ABI of osc_dev.h.
+//
+// The C host layer never includes a HIP header; every runtime call it needs
+// is one of these wrappers.  Errors are turned into OSC_ERR_HIP and the HIP
+// error string is kept for oscd_last_hip_error().
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/osc.h"
+#include "osc_dev.h"
+
+static thread_local char g_hip_err[256];
+
+static int fail(hipError_t e, const char *what) {
+  snprintf(g_hip_err, sizeof(g_hip_err), "%s: %s (%d)", what,
+           hipGetErrorString(e), (int)e);
+  return OSC_ERR_HIP;
+}
+
+#define HIPCALL(expr)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess) return fail(e_, #expr);                              \
+  } while (0)
+
+extern "C" {
+
+const char *oscd_last_hip_error(void) { return g_hip_err; }
+
+int oscd_set_device(int dev) {
+  HIPCALL(hipSetDevice(dev));
+  return OSC_OK;
+}
+
+int oscd_get_device_count(int *n) {
+  HIPCALL(hipGetDeviceCount(n));
+  return OSC_OK;
+}
+
+int oscd_malloc(void **p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    fail(e, "hipMalloc");
+    return OSC_ERR_NOMEM;
+  }
+  if (e != hipSuccess) return fail(e, "hipMalloc");
+  return OSC_OK;
+}
+
+int oscd_free(void *p) {
+  HIPCALL(hipFree(p));
+  return OSC_OK;
+}
+
+int oscd_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  HIPCALL(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return OSC_OK;
+}
+
+int oscd_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  HIPCALL(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return OSC_OK;
+}
+
+int oscd_memcpy_d2d_async(void *dst, const void *src, size_t bytes,
+                          void *stream) {
+  HIPCALL(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice,
+                         (hipStream_t)stream));
+  return OSC_OK;
+}
+
+int oscd_memset_async(void *dst, int value, size_t bytes, void *stream) {
+  HIPCALL(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream));
+  return OSC_OK;
+}
+
+int oscd_stream_create(void **s) {
+  hipStream_t st;
+  HIPCALL(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  *s = (void *)st;
+  return OSC_OK;
+}
+
+int oscd_stream_destroy(void *s) {
+  HIPCALL(hipStreamDestroy((hipStream_t)s));
+  return OSC_OK;
+}
+
+int oscd_stream_sync(void *s) {
+  HIPCALL(hipStreamSynchronize((hipStream_t)s));
+  return OSC_OK;
+}
+
+int oscd_stream_query(void *s) {
+  hipError_t e = hipStreamQuery((hipStream_t)s);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return fail(e, "hipStreamQuery");
+}
+
+int oscd_event_create(void **ev) {
+  hipEvent_t e;
+  HIPCALL(hipEventCreate(&e));
+  *ev = (void *)e;
+  return OSC_OK;
+}
+
+int oscd_event_destroy(void *e) {
+  HIPCALL(hipEventDestroy((hipEvent_t)e));
+  return OSC_OK;
+}
+
+int oscd_event_record(void *e, void *s) {
+  HIPCALL(hipEventRecord((hipEvent_t)e, (hipStream_t)s));
+  return OSC_OK;
+}
+
+int oscd_event_sync(void *e) {
+  HIPCALL(hipEventSynchronize((hipEvent_t)e));
+  return OSC_OK;
+}
+
+int oscd_event_query(void *e) {
+  hipError_t r = hipEventQuery((hipEvent_t)e);
+  if (r == hipSuccess) return 1;
+  if (r == hipErrorNotReady) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return fail(r, "hipEventQuery");
+}
+
+int oscd_event_elapsed_ms(void *a, void *b, float *ms) {
+  HIPCALL(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+  return OSC_OK;
+}
+
+int oscd_device_sync(void) {
+  HIPCALL(hipDeviceSynchronize());
+  return OSC_OK;
+}
+
+int oscd_mem_info(size_t *free_b, size_t *total_b) {
+  HIPCALL(hipMemGetInfo(free_b, total_b));
+  return OSC_OK;
+}
+
+int oscd_host_alloc(void **p, size_t bytes) {
+  HIPCALL(hipHostMalloc(p, bytes, hipHostMallocDefault));
+  return OSC_OK;
+}
+
+int oscd_host_free(void *p) {
+  HIPCALL(hipHostFree(p));
+  return OSC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,83 @@
+/*
+ * osc_internal.h - state shared by the C host files of libosc.
+ */
+#ifndef OSC_INTERNAL_H
+#define OSC_INTERNAL_H 1
+
+#include <pthread.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../../include/osc.h"
+#include "../osc_dev.h"
+
+#define OSC_TEAM_MAGIC 0x05C7EA11
+#define OSC_MAX_NPES 8191
+
+typedef struct osc_team {
+  int magic;
+  int start;  /* global PE of member 0 */
+  int stride; /* global PE distance between members (>= 1) */
+  int npes;
+  int predefined;
+  int num_contexts;
+  struct osc_team *parent;
+} osc_team;
+
+/* one block of the symmetric offset space */
+typedef struct osc_block {
+  size_t off;
+  size_t size;
+  int used;
+  struct osc_block *next;
+} osc_block;
+
+typedef struct osc_state {
+  int initialized;
+  int device;
+  int npes;
+  size_t heap_bytes; /* usable symmetric bytes per PE */
+  size_t slice;      /* distance between PE slices (>= heap_bytes) */
+  char *base;        /* device arena: npes * slice bytes */
+  void *stream;      /* library stream for blocking calls */
+  osc_block *blocks; /* sorted by offset, covers [0, heap_bytes) */
+  pthread_mutex_t lock;
+  int log_calls;
+  /* resolved algorithm choice: float mode per (op, dtype) */
+  unsigned char fpmode[OSC_OP_COUNT][OSC_T_COUNT];
+  int fp_default;
+} osc_state;
+
+extern osc_state osc_g;
+extern osc_team osc_team_world;
+extern osc_team osc_team_shared;
+
+/* errors */
+int osc_set_error(int code, const char *fmt, ...);
+int osc_hip_error(int code, const char *what);
+
+/* heap (osc_heap.c) */
+int osc_heap_init(size_t heap_bytes);
+void osc_heap_fini(void);
+int osc_heap_alloc(size_t size, size_t align, size_t *off);
+int osc_heap_free(size_t off);
+int osc_heap_block_size(size_t off, size_t *size);
+size_t osc_heap_used(void);
+
+/* teams (osc_team.c) */
+const osc_team *osc_team_get(void *team);
+void osc_team_geom(const osc_team *t, oscd_geom *g);
+int osc_my_pe_get(void);
+
+/* algorithm registry (osc_registry.c) */
+int osc_registry_init_from_env(void);
+int osc_registry_resolve(int kind, int op, const char *spec, int *fpmode);
+size_t osc_dtype_size(int dtype);
+
+enum { OSC_KIND_REDUCE = 0, OSC_KIND_FCOLLECT, OSC_KIND_ALLTOALL,
+       OSC_KIND_BROADCAST };
+
+/* symmetric-argument validation (osc_coll.c) */
+int osc_sym_range(const void *p, size_t bytes, size_t *off);
+
+#endif

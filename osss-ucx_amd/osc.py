@@ -1,0 +1,252 @@
+"""ctypes binding of libosc.so, the MI355X OpenSHMEM collective library.
+
+This is the Python-side binding of the C ABI declared in include/shmem.h
+and include/osc.h: the OpenSHMEM typed entry points (shmem_<T>_<op>_reduce,
+shmem_<T>_fcollect, shmem_<T>_alltoall, shmem_<T>_broadcast, the *mem
+forms, heap and team calls) plus the osc_* launch layer.  It loads the
+in-tree build (osss-ucx_amd/lib/libosc.so) and raises if it is missing: there
+is no fallback implementation.
+
+All device work happens inside the library; numpy arrays are only used to
+move data between the host and one PE's copy of a symmetric object.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libosc.so")
+
+# numbering of include/osc.h
+OPS = {"and": 0, "or": 1, "xor": 2, "min": 3, "max": 4, "sum": 5, "prod": 6}
+DTYPES = {"u8": 0, "u16": 1, "u32": 2, "u64": 3, "i8": 4, "i16": 5, "i32": 6,
+          "i64": 7, "f32": 8, "f64": 9, "c32": 10, "c64": 11}
+NP_DTYPE = {"u8": np.uint8, "u16": np.uint16, "u32": np.uint32,
+            "u64": np.uint64, "i8": np.int8, "i16": np.int16,
+            "i32": np.int32, "i64": np.int64, "f32": np.float32,
+            "f64": np.float64, "c32": np.complex64, "c64": np.complex128}
+DIST = {"bits": 0, "int_range": 1, "set_m1_1_2": 2, "normal": 3,
+        "real_range": 4, "a2a_pattern": 5, "fcol_pattern": 6}
+FP_ACCURATE, FP_REFERENCE = 0, 1
+
+# OpenSHMEM typename -> kernel dtype (include/shmem.h type tables, LP64)
+TYPENAME_DTYPE = {
+    "char": "i8", "schar": "i8", "short": "i16", "int": "i32", "long": "i64",
+    "longlong": "i64", "ptrdiff": "i64", "uchar": "u8", "ushort": "u16",
+    "uint": "u32", "ulong": "u64", "ulonglong": "u64", "int8": "i8",
+    "int16": "i16", "int32": "i32", "int64": "i64", "uint8": "u8",
+    "uint16": "u16", "uint32": "u32", "uint64": "u64", "size": "u64",
+    "float": "f32", "double": "f64", "complexf": "c32", "complexd": "c64",
+}
+
+ERRORS = {0: "OK", -1: "ARG", -2: "NOTINIT", -3: "TEAM", -4: "PTR",
+          -5: "NOMEM", -6: "HIP", -7: "UNSUPPORTED", -8: "ALGO",
+          -9: "BUDGET", -10: "STATE"}
+
+
+class OscError(RuntimeError):
+    def __init__(self, code: int, what: str, msg: str):
+        super().__init__(f"{what}: {ERRORS.get(code, code)} ({code}): {msg}")
+        self.code = code
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """Load libosc.so (once).  Raises if the build is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} not found: build it first (make -C osss-ucx_amd)")
+    L = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+    vp, sz, i, u64, d = C.c_void_p, C.c_size_t, C.c_int, C.c_uint64, C.c_double
+    sig = {
+        "osc_init": (i, [i, sz, i]),
+        "osc_finalize": (i, []),
+        "osc_initialized": (i, []),
+        "osc_npes": (i, []),
+        "osc_slice_bytes": (sz, []),
+        "osc_heap_bytes": (sz, []),
+        "osc_heap_base": (vp, []),
+        "osc_default_stream": (vp, []),
+        "osc_set_fp_mode": (i, [i]),
+        "osc_get_fp_mode": (i, []),
+        "osc_strerror": (C.c_char_p, [i]),
+        "osc_last_error": (C.c_char_p, []),
+        "osc_set_my_pe": (i, [i]),
+        "osc_sym_offset": (i, [vp, C.POINTER(i), C.POINTER(sz)]),
+        "osc_pe_ptr": (vp, [vp, i]),
+        "osc_put_host": (i, [vp, vp, sz, i]),
+        "osc_get_host": (i, [vp, vp, sz, i]),
+        "osc_reduce": (i, [vp, i, i, vp, vp, sz, vp]),
+        "osc_fcollect": (i, [vp, vp, vp, sz, vp]),
+        "osc_alltoall": (i, [vp, vp, vp, sz, vp]),
+        "osc_broadcast": (i, [vp, vp, vp, sz, i, vp]),
+        "osc_stream_sync": (i, [vp]),
+        "osc_fill": (i, [vp, vp, sz, i, i, d, d, u64, vp]),
+        "osc_event_create": (i, [C.POINTER(vp)]),
+        "osc_event_destroy": (i, [vp]),
+        "osc_event_record": (i, [vp, vp]),
+        "osc_event_elapsed_ms": (i, [vp, vp, C.POINTER(C.c_float)]),
+        "osc_parse_size": (i, [C.c_char_p, C.POINTER(sz)]),
+        "osc_algo_resolve": (i, [C.c_char_p, C.c_char_p, C.POINTER(i)]),
+        "osc_device_copy": (i, [vp, vp, sz, vp]),
+        "shmem_init": (None, []),
+        "shmem_finalize": (None, []),
+        "shmem_my_pe": (i, []),
+        "shmem_n_pes": (i, []),
+        "shmem_malloc": (vp, [sz]),
+        "shmem_calloc": (vp, [sz, sz]),
+        "shmem_align": (vp, [sz, sz]),
+        "shmem_realloc": (vp, [vp, sz]),
+        "shmem_free": (None, [vp]),
+        "shmem_ptr": (vp, [vp, i]),
+        "shmem_addr_accessible": (i, [vp, i]),
+        "shmem_barrier_all": (None, []),
+        "shmem_team_my_pe": (i, [vp]),
+        "shmem_team_n_pes": (i, [vp]),
+        "shmem_team_translate_pe": (i, [vp, i, vp]),
+        "shmem_team_split_strided": (i, [vp, i, i, i, vp, C.c_long,
+                                         C.POINTER(vp)]),
+        "shmem_team_split_2d": (i, [vp, i, vp, C.c_long, C.POINTER(vp), vp,
+                                    C.c_long, C.POINTER(vp)]),
+        "shmem_team_destroy": (None, [vp]),
+        "shmem_team_sync": (i, [vp]),
+        "shmem_putmem": (None, [vp, vp, sz, i]),
+        "shmem_getmem": (None, [vp, vp, sz, i]),
+        "shmem_putmem_nbi": (None, [vp, vp, sz, i]),
+        "shmem_getmem_nbi": (None, [vp, vp, sz, i]),
+        "shmem_fcollectmem": (i, [vp, vp, vp, sz]),
+        "shmem_alltoallmem": (i, [vp, vp, vp, sz]),
+        "shmem_broadcastmem": (i, [vp, vp, vp, sz, i]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def team_world() -> int:
+    return C.c_void_p.in_dll(lib(), "SHMEM_TEAM_WORLD").value
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise OscError(rc, what, lib().osc_last_error().decode())
+
+
+def typed(name: str) -> C._CFuncPtr:
+    """An OpenSHMEM typed collective, e.g. typed('shmem_int_sum_reduce')."""
+    fn = getattr(lib(), name)
+    fn.restype = C.c_int
+    if name.endswith("_broadcast"):
+        fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                       C.c_int]
+    else:
+        fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    return fn
+
+
+class Heap:
+    """Convenience wrapper: an initialised library with npes emulated PEs."""
+
+    def __init__(self, npes: int, heap_bytes: int, device: int = 0):
+        L = lib()
+        if L.osc_initialized():
+            L.osc_finalize()
+        check(L.osc_init(npes, heap_bytes, device), "osc_init")
+        self.L = L
+        self.npes = npes
+        self.world = team_world()
+
+    def close(self) -> None:
+        self.L.osc_finalize()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def malloc(self, nbytes: int) -> int:
+        p = self.L.shmem_malloc(nbytes)
+        if not p:
+            raise OscError(-5, "shmem_malloc",
+                           self.L.osc_last_error().decode())
+        return p
+
+    def free(self, p: int) -> None:
+        self.L.shmem_free(p)
+
+    def put(self, sym: int, arr: np.ndarray, pe: int) -> None:
+        a = np.ascontiguousarray(arr)
+        check(self.L.osc_put_host(sym, a.ctypes.data, a.nbytes, pe),
+              "osc_put_host")
+
+    def get(self, sym: int, count: int, dtype, pe: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        check(self.L.osc_get_host(out.ctypes.data, sym, out.nbytes, pe),
+              "osc_get_host")
+        return out
+
+    def put_all(self, sym: int, arrs, team=None, start=0, stride=1) -> None:
+        for k, a in enumerate(arrs):
+            self.put(sym, a, start + k * stride)
+
+    def get_all(self, sym: int, count: int, dtype, pes) -> np.ndarray:
+        return np.stack([self.get(sym, count, dtype, pe) for pe in pes])
+
+    def fill(self, team, sym, nelems, dtype: str, dist: str, a=0.0, b=0.0,
+             seed=0) -> None:
+        check(self.L.osc_fill(team or self.world, sym, nelems, DTYPES[dtype],
+                              DIST[dist], float(a), float(b), seed, None),
+              "osc_fill")
+
+    def sync(self) -> None:
+        check(self.L.osc_stream_sync(None), "osc_stream_sync")
+
+    def split(self, start: int, stride: int, size: int, parent=None) -> int:
+        out = C.c_void_p()
+        rc = self.L.shmem_team_split_strided(parent or self.world, start,
+                                             stride, size, None, 0,
+                                             C.byref(out))
+        check(rc, "shmem_team_split_strided")
+        return out.value
+
+
+class Timer:
+    """HIP-event timing on the library stream (or a given stream)."""
+
+    def __init__(self, stream=None):
+        L = lib()
+        self.L = L
+        self.stream = stream
+        self.a, self.b = C.c_void_p(), C.c_void_p()
+        check(L.osc_event_create(C.byref(self.a)), "event")
+        check(L.osc_event_create(C.byref(self.b)), "event")
+
+    def start(self):
+        check(self.L.osc_event_record(self.a, self.stream), "record")
+
+    def stop(self):
+        check(self.L.osc_event_record(self.b, self.stream), "record")
+
+    def ms(self) -> float:
+        self.L.osc_stream_sync(self.stream)
+        out = C.c_float()
+        check(self.L.osc_event_elapsed_ms(self.a, self.b, C.byref(out)),
+              "elapsed")
+        return out.value
+
+    def close(self):
+        self.L.osc_event_destroy(self.a)
+        self.L.osc_event_destroy(self.b)
