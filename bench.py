@@ -156,9 +156,9 @@ def main():
     dom_b = 2.0 * P * N * ESIZE[CELLS[dom][1]]
     achieved = dom_b / dom_avg / 1e9
 
-    # plain-copy ceiling on the same arena: 2 GiB read + 2 GiB write
+    # plain-copy ceiling on the same arena after the timed region (it
+    # overwrites slices 32.. of the heap): 1 GiB read + 1 GiB written
     cp_bytes = 1 << 31
-    a_ = h.malloc(N * 8)  # 32 MiB per PE -> 2 GiB contiguous over 64 PEs
     base = L.osc_heap_base()
     half = (P // 2) * L.osc_slice_bytes()
     t = osc.Timer(stream)
@@ -169,7 +169,6 @@ def main():
         L.osc_device_copy(base + half, base, cp_bytes // 2, stream)
     t.stop()
     copy_gbps = 5 * cp_bytes / (t.ms() / 1e3) / 1e9
-    h.free(a_)
 
     out = {
         "metric": "reduce HBM GB/s (% peak) at 64 PEs x 4M; p50/p99 latency "

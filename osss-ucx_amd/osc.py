@@ -159,6 +159,8 @@ def typed(name: str) -> C._CFuncPtr:
 class Heap:
     """Convenience wrapper: an initialised library with npes emulated PEs."""
 
+    _current = None  # the Heap whose osc_init is live
+
     def __init__(self, npes: int, heap_bytes: int, device: int = 0):
         L = lib()
         if L.osc_initialized():
@@ -166,10 +168,18 @@ class Heap:
         check(L.osc_init(npes, heap_bytes, device), "osc_init")
         self.L = L
         self.npes = npes
+        self.heap_bytes = heap_bytes
         self.world = team_world()
+        Heap._current = self
+
+    @property
+    def alive(self) -> bool:
+        return Heap._current is self and bool(self.L.osc_initialized())
 
     def close(self) -> None:
-        self.L.osc_finalize()
+        if Heap._current is self:
+            self.L.osc_finalize()
+            Heap._current = None
 
     def __enter__(self):
         return self

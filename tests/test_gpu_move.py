@@ -18,11 +18,16 @@ SIZES = [1, 3, 8, 16, 17, 24, 64, 65, 100, 256, 1000, 4096, 4104, 16392]
 TEAMS = [1, 2, 3, 5, 8, 13, 64]
 
 
-@pytest.fixture(scope="module")
+_HEAP = []
+
+
+@pytest.fixture
 def heap():
-    h = osc.Heap(npes=64, heap_bytes=80 << 20)
-    yield h
-    h.close()
+    """64 PEs x 80 MiB, re-created if another test re-initialised the
+    library with a different world."""
+    if not _HEAP or not _HEAP[0].alive:
+        _HEAP[:] = [osc.Heap(npes=64, heap_bytes=80 << 20)]
+    return _HEAP[0]
 
 
 def team_of(h, P):

@@ -44,11 +44,16 @@ OPS_FOR = {
 CASES = [(d, o) for d, ops in OPS_FOR.items() for o in ops]
 
 
-@pytest.fixture(scope="module")
+_HEAP = []
+
+
+@pytest.fixture
 def heap():
-    h = osc.Heap(npes=64, heap_bytes=80 << 20)
-    yield h
-    h.close()
+    """64 PEs x 80 MiB, re-created if another test re-initialised the
+    library with a different world."""
+    if not _HEAP or not _HEAP[0].alive:
+        _HEAP[:] = [osc.Heap(npes=64, heap_bytes=80 << 20)]
+    return _HEAP[0]
 
 
 def team_of(h, P):
