@@ -123,6 +123,12 @@ __device__ __forceinline__ void st_nt(void *p, typename Raw<BYTES>::type v) {
   __builtin_nontemporal_store(v, (typename Raw<BYTES>::type *)p);
 }
 
+// Streaming load: every source byte is read exactly once.
+template <int BYTES>
+__device__ __forceinline__ typename Raw<BYTES>::type ld_nt(const void *p) {
+  return __builtin_nontemporal_load((const typename Raw<BYTES>::type *)p);
+}
+
 }  // namespace osc
 
 #endif  // OSC_DEV_COMMON_H

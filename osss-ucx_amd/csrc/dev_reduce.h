@@ -1,0 +1,656 @@
+// dev_reduce.h - the team reduction kernel (all ops x all element types).
+//
+// Semantics follow the reference's default team reduction, recursive
+// doubling (src/shcoll/src/reduce.c:182-293, REDUCE_HELPER_REC_DBL), with
+// the element operators of reduce.c:730-736.  Its result has a closed form
+// that one pass over the P source slices computes directly:
+//
+//   p2s = largest power of two <= P, lg = log2(p2s)
+//   member k (k < p2s) is team index m_k = ceil(k*P/p2s)
+//   leaf k = op(x[m_k + 1], x[m_k]) if m_k + 1 is not a member, else x[m_k]
+//     (reduce.c:228-244: the member pulls its right neighbour's source and
+//      combines it as op(neighbour, own))
+//   member k's result = balanced pairwise tree over leaves k^0, k^1, ...,
+//     k^(p2s-1), each node op(left subtree, right subtree)
+//     (reduce.c:250-272: round r combines op(own, partner k^2^r))
+//   the non-member m_k + 1 receives member k's result (reduce.c:277-288).
+//
+// For + and * the two partners of a round compute op(a,b) and op(b,a),
+// which are identical in IEEE arithmetic, so every member ends with the
+// bits of member 0's tree; integer and bitwise ops are exact in any order.
+// Only float MIN/MAX differ between members (ternary a>b?a:b picks the
+// second operand on ties and NaN), and only where a NaN is present or the
+// extremum is a signed zero.  Such elements take a block-cooperative exact
+// path that evaluates every member's own tree.
+//
+// Layout: a 16-byte "unit" holds VEC elements.  A workgroup of 256 threads
+// owns 256*UPT consecutive units; thread t handles units t, t+256, ...,
+// t+256*(UPT-1), so each wave instruction touches 1 KiB contiguous per slice
+// and a workgroup streams 256*UPT*16 bytes contiguous per slice.  Sources
+// are read INFL slices at a time (UPT*INFL 16-byte loads in flight per
+// thread), combined in registers, and the result is written to all P
+// destination slices.  Loads and stores are non-temporal: every source
+// byte is read once and every destination byte written once, so HBM
+// traffic = 2*P*n*s.  (UPT, INFL) is a per-call "shape" chosen from a
+// tuning table.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/osc.h"
+#include "dev_common.h"
+#include "osc_dev.h"
+
+#ifndef OSC_DEV_REDUCE_H
+#define OSC_DEV_REDUCE_H
+
+namespace osc {
+
+// ---- element operators (reduce.c:730-736) --------------------------------
+__device__ __forceinline__ cf32 cadd(cf32 a, cf32 b) {
+  return {a.re + b.re, a.im + b.im};
+}
+__device__ __forceinline__ cf64 cadd(cf64 a, cf64 b) {
+  return {a.re + b.re, a.im + b.im};
+}
+// C99 complex product for finite operands: (ac - bd) + (ad + bc)i, each
+// product and sum rounded separately (no contraction).
+__device__ __forceinline__ cf32 cmul(cf32 a, cf32 b) {
+  float ac = a.re * b.re, bd = a.im * b.im, ad = a.re * b.im, bc = a.im * b.re;
+  return {ac - bd, ad + bc};
+}
+__device__ __forceinline__ cf64 cmul(cf64 a, cf64 b) {
+  double ac = a.re * b.re, bd = a.im * b.im, ad = a.re * b.im,
+         bc = a.im * b.re;
+  return {ac - bd, ad + bc};
+}
+
+template <int OP>
+struct Bin;
+template <>
+struct Bin<OSC_OP_AND> {
+  template <class T>
+  __device__ static T f(T a, T b) {
+    return (T)(a & b);
+  }
+};
+template <>
+struct Bin<OSC_OP_OR> {
+  template <class T>
+  __device__ static T f(T a, T b) {
+    return (T)(a | b);
+  }
+};
+template <>
+struct Bin<OSC_OP_XOR> {
+  template <class T>
+  __device__ static T f(T a, T b) {
+    return (T)(a ^ b);
+  }
+};
+template <>
+struct Bin<OSC_OP_MIN> {
+  template <class T>
+  __device__ static T f(T a, T b) {
+    return (a < b) ? a : b;
+  }
+};
+template <>
+struct Bin<OSC_OP_MAX> {
+  template <class T>
+  __device__ static T f(T a, T b) {
+    return (a > b) ? a : b;
+  }
+};
+template <>
+struct Bin<OSC_OP_SUM> {
+  template <class T>
+  __device__ static T f(T a, T b) {
+    return (T)(a + b);
+  }
+  __device__ static cf32 f(cf32 a, cf32 b) { return cadd(a, b); }
+  __device__ static cf64 f(cf64 a, cf64 b) { return cadd(a, b); }
+};
+template <>
+struct Bin<OSC_OP_PROD> {
+  template <class T>
+  __device__ static T f(T a, T b) {
+    return (T)(a * b);
+  }
+  __device__ static cf32 f(cf32 a, cf32 b) { return cmul(a, b); }
+  __device__ static cf64 f(cf64 a, cf64 b) { return cmul(a, b); }
+};
+
+// ---- accumulation policies -----------------------------------------------
+// TREE    : combine in the recursive-doubling tree order (reference mode).
+// SPECIAL : float MIN/MAX, whose per-member results can differ.
+// lift/seq/comb/out: value -> accumulator, acc (+) value, acc (+) acc,
+//                    accumulator -> value.
+
+// Exact in any order: integers (wrap-around), bitwise, float MIN/MAX.
+template <class T, int OP, bool SPEC>
+struct PolSeq {
+  typedef T Acc;
+  static constexpr bool TREE = false;
+  static constexpr bool SPECIAL = SPEC;
+  static constexpr int OPC = OP;
+  __device__ static Acc lift(T x) { return x; }
+  __device__ static Acc seq(Acc a, T x) { return Bin<OP>::f(a, x); }
+  __device__ static Acc comb(Acc a, Acc b) { return Bin<OP>::f(a, b); }
+  __device__ static T out(Acc a) { return a; }
+};
+
+// The type's own arithmetic in the reference's tree order.
+template <class T, int OP>
+struct PolTree {
+  typedef T Acc;
+  static constexpr bool TREE = true;
+  static constexpr bool SPECIAL = false;
+  static constexpr int OPC = OP;
+  __device__ static Acc lift(T x) { return x; }
+  __device__ static Acc seq(Acc a, T x) { return Bin<OP>::f(a, x); }
+  __device__ static Acc comb(Acc a, Acc b) { return Bin<OP>::f(a, b); }
+  __device__ static T out(Acc a) { return a; }
+};
+
+// FP32 SUM/PROD accumulated in FP64, rounded once.
+template <int OP>
+struct PolF32Acc {
+  typedef double Acc;
+  static constexpr bool TREE = false;
+  static constexpr bool SPECIAL = false;
+  static constexpr int OPC = OP;
+  __device__ static Acc lift(float x) { return (double)x; }
+  __device__ static Acc seq(Acc a, float x) {
+    return OP == OSC_OP_SUM ? a + (double)x : a * (double)x;
+  }
+  __device__ static Acc comb(Acc a, Acc b) {
+    return OP == OSC_OP_SUM ? a + b : a * b;
+  }
+  __device__ static float out(Acc a) { return (float)a; }
+};
+
+// FP64 SUM/PROD accumulated in double-double, rounded once.
+template <int OP>
+struct PolF64Acc {
+  typedef dd Acc;
+  static constexpr bool TREE = false;
+  static constexpr bool SPECIAL = false;
+  static constexpr int OPC = OP;
+  __device__ static Acc lift(double x) { return {x, 0.0}; }
+  __device__ static Acc seq(Acc a, double x) {
+    return OP == OSC_OP_SUM ? dd_add_d(a, x) : dd_mul_d(a, x);
+  }
+  __device__ static Acc comb(Acc a, Acc b) { return dd_add(a, b); }
+  __device__ static double out(Acc a) { return a.hi; }
+};
+
+// complex float SUM/PROD in FP64 components.
+struct c64acc {
+  double re, im;
+};
+template <int OP>
+struct PolC32Acc {
+  typedef c64acc Acc;
+  static constexpr bool TREE = false;
+  static constexpr bool SPECIAL = false;
+  static constexpr int OPC = OP;
+  __device__ static Acc lift(cf32 x) { return {(double)x.re, (double)x.im}; }
+  __device__ static Acc seq(Acc a, cf32 x) {
+    const double xr = x.re, xi = x.im;
+    if (OP == OSC_OP_SUM) return {a.re + xr, a.im + xi};
+    double ac = a.re * xr, bd = a.im * xi, ad = a.re * xi, bc = a.im * xr;
+    return {ac - bd, ad + bc};
+  }
+  __device__ static Acc comb(Acc a, Acc b) { return {a.re + b.re, a.im + b.im}; }
+  __device__ static cf32 out(Acc a) { return {(float)a.re, (float)a.im}; }
+};
+
+// complex double SUM/PROD in double-double components.
+struct cddacc {
+  dd re, im;
+};
+template <int OP>
+struct PolC64Acc {
+  typedef cddacc Acc;
+  static constexpr bool TREE = false;
+  static constexpr bool SPECIAL = false;
+  static constexpr int OPC = OP;
+  __device__ static Acc lift(cf64 x) { return {{x.re, 0.0}, {x.im, 0.0}}; }
+  __device__ static Acc seq(Acc a, cf64 x) {
+    if (OP == OSC_OP_SUM) return {dd_add_d(a.re, x.re), dd_add_d(a.im, x.im)};
+    dd re = dd_add(dd_mul_d(a.re, x.re), dd_neg(dd_mul_d(a.im, x.im)));
+    dd im = dd_add(dd_mul_d(a.re, x.im), dd_mul_d(a.im, x.re));
+    return {re, im};
+  }
+  __device__ static Acc comb(Acc a, Acc b) {
+    return {dd_add(a.re, b.re), dd_add(a.im, b.im)};
+  }
+  __device__ static cf64 out(Acc a) { return {a.re.hi, a.im.hi}; }
+};
+
+// ---- unit load / store ---------------------------------------------------
+template <class T, int VEC>
+__device__ __forceinline__ void load_unit(const char *p, T (&x)[VEC],
+                                          bool full, int cnt) {
+  constexpr int B = VEC * (int)sizeof(T);
+  if (B <= 16 && (B & (B - 1)) == 0 && full) {
+    typename Raw<(B <= 16 ? B : 16)>::type r = ld_nt<(B <= 16 ? B : 16)>(p);
+    __builtin_memcpy(&x[0], &r, B);
+  } else {
+#pragma unroll
+    for (int v = 0; v < VEC; ++v)
+      if (v < cnt) x[v] = ((const T *)p)[v];
+  }
+}
+
+template <class T, int VEC>
+__device__ __forceinline__ void store_unit(char *p, const T (&x)[VEC],
+                                           bool full, int cnt,
+                                           uint32_t skip) {
+  constexpr int B = VEC * (int)sizeof(T);
+  if (B <= 16 && (B & (B - 1)) == 0 && full && skip == 0) {
+    typename Raw<(B <= 16 ? B : 16)>::type r;
+    __builtin_memcpy(&r, &x[0], B);
+    st_nt<(B <= 16 ? B : 16)>(p, r);
+  } else {
+#pragma unroll
+    for (int v = 0; v < VEC; ++v)
+      if (v < cnt && !((skip >> v) & 1u)) ((T *)p)[v] = x[v];
+  }
+}
+
+template <class T>
+__device__ __forceinline__ bool is_nan_or_zero(T) {
+  return false;
+}
+__device__ __forceinline__ bool is_nan_or_zero(float x) {
+  return x != x || x == 0.0f;
+}
+__device__ __forceinline__ bool is_nan_or_zero(double x) {
+  return x != x || x == 0.0;
+}
+template <class T>
+__device__ __forceinline__ bool is_nan(T) {
+  return false;
+}
+__device__ __forceinline__ bool is_nan(float x) { return x != x; }
+__device__ __forceinline__ bool is_nan(double x) { return x != x; }
+
+// Team index of leaf k's member: ceil(k*P / 2^lg).
+__device__ __forceinline__ int member_of(int k, int P, int lg) {
+  return (int)(((int64_t)k * P + ((1 << lg) - 1)) >> lg);
+}
+
+// ---- exact per-member MIN/MAX for special elements ----------------------
+// Cooperative over the workgroup: each special element is handled in turn.
+// Its P inputs are copied into LDS before any result is written (so
+// dest == source works), then thread t evaluates members t, t+256, ...
+// over the LDS copy and writes their results (and the folded neighbour's).
+// `mine` has bit k*VEC+v set for element v of the thread's k-th unit
+// (unit index ubase + k*256).
+template <class T, int OP, int VEC>
+__device__ void special_block(const oscd_geom &g, uint64_t src_off,
+                              uint64_t dest_off, uint64_t ubase,
+                              uint32_t mine, int lg, T *snap) {
+  __shared__ int s_owner;
+  __shared__ unsigned long long s_e;
+  const int P = g.npes;
+  const int p2s = 1 << lg;
+  for (;;) {
+    if (threadIdx.x == 0) s_owner = 0x7fffffff;
+    __syncthreads();
+    if (mine) atomicMin(&s_owner, (int)threadIdx.x);
+    __syncthreads();
+    const int owner = s_owner;
+    if (owner == 0x7fffffff) break;
+    if ((int)threadIdx.x == owner) {
+      const int b = __ffs(mine) - 1;
+      mine &= mine - 1;
+      s_e = (ubase + (uint64_t)(b / VEC) * blockDim.x) * VEC + (uint64_t)(b % VEC);
+    }
+    __syncthreads();
+    const uint64_t e = s_e;
+    for (int i = threadIdx.x; i < P; i += blockDim.x)
+      snap[i] = *(const T *)(member_ptr(g, i, src_off) + e * sizeof(T));
+    __syncthreads();
+    for (int k = threadIdx.x; k < p2s; k += blockDim.x) {
+      T stk[16];
+      for (int j = 0; j < p2s; ++j) {
+        const int kk = k ^ j;
+        const int m = member_of(kk, P, lg);
+        const int m1 = member_of(kk + 1, P, lg);
+        T leaf = (m1 - m == 2) ? Bin<OP>::f(snap[m + 1], snap[m]) : snap[m];
+        int L = 0;
+        while ((j >> L) & 1) {
+          leaf = Bin<OP>::f(stk[L], leaf);
+          ++L;
+        }
+        stk[L] = leaf;
+      }
+      const T r = stk[lg];
+      const int mk = member_of(k, P, lg);
+      const int mk1 = member_of(k + 1, P, lg);
+      *(T *)(member_ptr(g, mk, dest_off) + e * sizeof(T)) = r;
+      if (mk1 - mk == 2)
+        *(T *)(member_ptr(g, mk + 1, dest_off) + e * sizeof(T)) = r;
+    }
+    __syncthreads();
+  }
+}
+
+
+// ---- the kernel ------------------------------------------------------------
+constexpr int kBlock = 256;
+
+// Shapes (units per thread, source slices in flight) a launch can use:
+// 0: 8x2, 1: 4x4, 2: 2x4, 3: 1x8.
+constexpr int kNumShapes = 4;
+__host__ __device__ constexpr int shape_upt(int s) {
+  return s == 0 ? 8 : s == 1 ? 4 : s == 2 ? 2 : 1;
+}
+__host__ __device__ constexpr int shape_infl(int s) {
+  return s == 0 ? 2 : s == 1 ? 4 : s == 2 ? 4 : 8;
+}
+
+// One block-iteration: the workgroup reduces units [ub, ub + 256*UPT).
+// Addresses are a wave-uniform base (slice + block start, kept in SGPRs)
+// plus a 32-bit lane offset (< 32 KiB), so the loads and stores use the
+// scalar-base addressing form.  FAST: every unit of the block is complete
+// and in range (all blocks but the last), so no per-unit guards.
+template <bool FAST, class T, class Pol, int VEC, int UPT, int INFL, int MAXLV>
+__device__ __forceinline__ void reduce_block(const oscd_geom &g,
+                                             const uint64_t dest_off,
+                                             const uint64_t src_off,
+                                             const uint64_t n, const int lg,
+                                             const uint64_t ub, T *lds) {
+  typedef typename Pol::Acc Acc;
+  constexpr uint32_t UB = VEC * (uint32_t)sizeof(T);
+  const int P = g.npes;
+  const int64_t step = (int64_t)g.stride * (int64_t)g.slice;
+  const char *s0 = member_ptr(g, 0, src_off) + ub * UB;
+  char *d0 = member_ptr(g, 0, dest_off) + ub * UB;
+  const uint64_t nunits = (n + VEC - 1) / VEC;
+  const uint64_t ubase = ub + threadIdx.x;
+  uint32_t loff[UPT];
+  bool full[UPT];
+  int cnt[UPT];
+  bool any = FAST;
+#pragma unroll
+  for (int k = 0; k < UPT; ++k) {
+    loff[k] = (threadIdx.x + (uint32_t)k * kBlock) * UB;
+    if (FAST) {
+      full[k] = true;
+      cnt[k] = VEC;
+    } else {
+      const uint64_t u = ubase + (uint64_t)k * kBlock;
+      const uint64_t e0 = u * VEC;
+      const bool act = u < nunits;
+      full[k] = act && (e0 + VEC <= n);
+      cnt[k] = !act ? 0 : (full[k] ? VEC : (int)(n - e0));
+      any |= act;
+    }
+  }
+  T r[UPT][VEC];
+  uint32_t spec = 0;
+
+  if (any) {
+    Acc acc[UPT][VEC];
+    if constexpr (!Pol::TREE) {
+      uint32_t nanm = 0;
+      for (int i0 = 0; i0 < P; i0 += INFL) {
+        T buf[INFL][UPT][VEC];
+        const int m = (P - i0) < INFL ? (P - i0) : INFL;
+#pragma unroll
+        for (int j = 0; j < INFL; ++j)
+          if (j < m) {
+            const char *sp = s0 + (i0 + j) * step;
+#pragma unroll
+            for (int k = 0; k < UPT; ++k)
+              if (FAST || cnt[k] > 0)
+                load_unit<T, VEC>(sp + loff[k], buf[j][k], FAST || full[k],
+                                  cnt[k]);
+          }
+#pragma unroll
+        for (int j = 0; j < INFL; ++j) {
+          if (j < m) {
+#pragma unroll
+            for (int k = 0; k < UPT; ++k)
+#pragma unroll
+              for (int v = 0; v < VEC; ++v) {
+                if (Pol::SPECIAL && is_nan(buf[j][k][v]))
+                  nanm |= 1u << (k * VEC + v);
+                acc[k][v] = (i0 + j == 0) ? Pol::lift(buf[j][k][v])
+                                          : Pol::seq(acc[k][v], buf[j][k][v]);
+              }
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < UPT; ++k)
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          r[k][v] = Pol::out(acc[k][v]);
+          const int b = k * VEC + v;
+          if (Pol::SPECIAL && v < cnt[k] &&
+              (((nanm >> b) & 1u) || is_nan_or_zero(r[k][v])))
+            spec |= 1u << b;
+        }
+    } else {
+      // Reference tree order: leaves in team-index order, folded pairs
+      // op(x[m+1], x[m]), merged by a binary counter: stk[L] holds the
+      // pending left subtree of 2^L leaves.
+      Acc stk[MAXLV][UPT][VEC];
+      T pend[UPT][VEC];
+      int kl = 0;
+      int mk = 0;
+      int mn = member_of(1, P, lg);
+      for (int i0 = 0; i0 < P; i0 += INFL) {
+        T buf[INFL][UPT][VEC];
+        const int m = (P - i0) < INFL ? (P - i0) : INFL;
+#pragma unroll
+        for (int j = 0; j < INFL; ++j)
+          if (j < m) {
+            const char *sp = s0 + (i0 + j) * step;
+#pragma unroll
+            for (int k = 0; k < UPT; ++k)
+              if (FAST || cnt[k] > 0)
+                load_unit<T, VEC>(sp + loff[k], buf[j][k], FAST || full[k],
+                                  cnt[k]);
+          }
+#pragma unroll
+        for (int j = 0; j < INFL; ++j) {
+          if (j < m) {
+            const int i = i0 + j;
+            Acc leaf[UPT][VEC];
+            bool ready = true;
+            if (i == mk) {
+              if (mn - mk == 2) {
+#pragma unroll
+                for (int k = 0; k < UPT; ++k)
+#pragma unroll
+                  for (int v = 0; v < VEC; ++v) pend[k][v] = buf[j][k][v];
+                ready = false;
+              } else {
+#pragma unroll
+                for (int k = 0; k < UPT; ++k)
+#pragma unroll
+                  for (int v = 0; v < VEC; ++v)
+                    leaf[k][v] = Pol::lift(buf[j][k][v]);
+              }
+            } else {
+#pragma unroll
+              for (int k = 0; k < UPT; ++k)
+#pragma unroll
+                for (int v = 0; v < VEC; ++v)
+                  leaf[k][v] = Pol::comb(Pol::lift(buf[j][k][v]),
+                                         Pol::lift(pend[k][v]));
+            }
+            if (ready) {
+              // carry the new subtree up while level L is occupied; every
+              // stk index is a constant after unrolling (stays in VGPRs)
+              bool carry = true;
+#pragma unroll
+              for (int L = 0; L < MAXLV; ++L) {
+                if (carry) {
+                  if ((kl >> L) & 1) {
+#pragma unroll
+                    for (int k = 0; k < UPT; ++k)
+#pragma unroll
+                      for (int v = 0; v < VEC; ++v)
+                        leaf[k][v] = Pol::comb(stk[L][k][v], leaf[k][v]);
+                  } else {
+#pragma unroll
+                    for (int k = 0; k < UPT; ++k)
+#pragma unroll
+                      for (int v = 0; v < VEC; ++v) stk[L][k][v] = leaf[k][v];
+                    carry = false;
+                  }
+                }
+              }
+              ++kl;
+              mk = mn;
+              mn = member_of(kl + 1, P, lg);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int L = 0; L < MAXLV; ++L)
+        if (L == lg) {
+#pragma unroll
+          for (int k = 0; k < UPT; ++k)
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) acc[k][v] = stk[L][k][v];
+        }
+#pragma unroll
+      for (int k = 0; k < UPT; ++k)
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) r[k][v] = Pol::out(acc[k][v]);
+    }
+  }
+
+  if constexpr (Pol::SPECIAL) {
+    if (__syncthreads_or(spec != 0))
+      special_block<T, Pol::OPC, VEC>(g, src_off, dest_off, ubase, spec, lg,
+                                      lds);
+  }
+
+  if (any) {
+    for (int j = 0; j < P; ++j) {
+      char *dp = d0 + j * step;
+#pragma unroll
+      for (int k = 0; k < UPT; ++k)
+        if (FAST || cnt[k] > 0)
+          store_unit<T, VEC>(dp + loff[k], r[k], FAST || full[k], cnt[k],
+                             Pol::SPECIAL ? (spec >> (k * VEC)) &
+                                                ((1u << VEC) - 1u)
+                                          : 0u);
+    }
+  }
+}
+
+template <class T, class Pol, int VEC, int UPT, int INFL, int MAXLV>
+__global__ __launch_bounds__(kBlock) void k_reduce(const oscd_geom g,
+                                                   const uint64_t dest_off,
+                                                   const uint64_t src_off,
+                                                   const uint64_t n,
+                                                   const int lg) {
+  static_assert(!Pol::SPECIAL || UPT * VEC <= 32,
+                "special-element mask is 32 bits");
+  extern __shared__ unsigned char k_reduce_lds[];
+  constexpr uint64_t kSpan = (uint64_t)kBlock * UPT;
+  const uint64_t nunits = (n + VEC - 1) / VEC;
+  const uint64_t nfull = n / VEC;
+  for (uint64_t ub = (uint64_t)blockIdx.x * kSpan; ub < nunits;
+       ub += (uint64_t)gridDim.x * kSpan) {
+    if (ub + kSpan <= nfull)
+      reduce_block<true, T, Pol, VEC, UPT, INFL, MAXLV>(
+          g, dest_off, src_off, n, lg, ub, (T *)k_reduce_lds);
+    else
+      reduce_block<false, T, Pol, VEC, UPT, INFL, MAXLV>(
+          g, dest_off, src_off, n, lg, ub, (T *)k_reduce_lds);
+  }
+}
+
+// ---- host-side launch ------------------------------------------------------
+template <class T>
+constexpr int vec_of() {
+  return (16 / (int)sizeof(T)) > 8 ? 8 : (16 / (int)sizeof(T));
+}
+
+template <class T, class Pol, int VEC, int UPT, int INFL>
+static void launch_shape(const oscd_geom *g, int lg, uint64_t doff,
+                         uint64_t soff, uint64_t n, size_t lds,
+                         hipStream_t s) {
+  const uint64_t units = (n + VEC - 1) / VEC;
+  const uint64_t span = (uint64_t)kBlock * UPT;
+  uint64_t grid = (units + span - 1) / span;
+  if (grid > (1u << 20)) grid = 1u << 20;
+  if constexpr (Pol::TREE) {
+    if (lg >= 8) {
+      hipLaunchKernelGGL((k_reduce<T, Pol, VEC, UPT, INFL, 13>),
+                         dim3((unsigned)grid), dim3(kBlock), lds, s, *g, doff,
+                         soff, n, lg);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_reduce<T, Pol, VEC, UPT, INFL, Pol::TREE ? 8 : 1>),
+                     dim3((unsigned)grid), dim3(kBlock), lds, s, *g, doff,
+                     soff, n, lg);
+}
+
+// Default shape per policy: wide for the register-light order-free kernels,
+// narrower where the accumulator (double-double, reference tree stack)
+// takes registers.
+template <class Pol>
+constexpr int default_shape() {
+  return Pol::TREE ? 2 : 0;
+}
+
+template <class T, class Pol>
+static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
+                  uint64_t n, int shape, hipStream_t s) {
+  constexpr int VB = vec_of<T>();
+  constexpr uint64_t AL = VB * sizeof(T);
+  const bool vec_ok = (doff % AL == 0) && (soff % AL == 0) &&
+                      (g->slice % AL == 0) && ((uintptr_t)g->base % AL == 0);
+  const size_t lds = Pol::SPECIAL ? (size_t)g->npes * sizeof(T) : 0;
+  if (n == 0) return OSC_OK;
+  if (shape < 0 || shape >= kNumShapes) shape = default_shape<Pol>();
+  if (!vec_ok) {
+    launch_shape<T, Pol, 1, 1, 8>(g, lg, doff, soff, n, lds, s);
+  } else if constexpr (Pol::TREE) {
+    // the tree's subtree stack takes MAXLV*UPT*VEC registers: narrow shapes
+    if (shape == 3)
+      launch_shape<T, Pol, VB, shape_upt(3), shape_infl(3)>(g, lg, doff, soff, n, lds, s);
+    else
+      launch_shape<T, Pol, VB, shape_upt(2), shape_infl(2)>(g, lg, doff, soff, n, lds, s);
+  } else {
+    switch (shape) {
+      case 0: launch_shape<T, Pol, VB, shape_upt(0), shape_infl(0)>(g, lg, doff, soff, n, lds, s); break;
+      case 1: launch_shape<T, Pol, VB, shape_upt(1), shape_infl(1)>(g, lg, doff, soff, n, lds, s); break;
+      case 2: launch_shape<T, Pol, VB, shape_upt(2), shape_infl(2)>(g, lg, doff, soff, n, lds, s); break;
+      default: launch_shape<T, Pol, VB, shape_upt(3), shape_infl(3)>(g, lg, doff, soff, n, lds, s); break;
+    }
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? OSC_OK : OSC_ERR_HIP;
+}
+
+// Family launchers, one translation unit each (compiled in parallel).
+int launch_reduce_int(const oscd_geom *g, int op, int dtype, int lg,
+                      uint64_t doff, uint64_t soff, uint64_t n, int shape,
+                      hipStream_t s);
+int launch_reduce_f32(const oscd_geom *g, int op, int mode, int lg,
+                      uint64_t doff, uint64_t soff, uint64_t n, int shape,
+                      hipStream_t s);
+int launch_reduce_f64(const oscd_geom *g, int op, int mode, int lg,
+                      uint64_t doff, uint64_t soff, uint64_t n, int shape,
+                      hipStream_t s);
+int launch_reduce_cplx(const oscd_geom *g, int op, int dtype, int mode,
+                       int lg, uint64_t doff, uint64_t soff, uint64_t n,
+                       int shape, hipStream_t s);
+
+}  // namespace osc
+
+#endif  // OSC_DEV_REDUCE_H

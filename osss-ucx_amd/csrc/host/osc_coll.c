@@ -52,8 +52,8 @@ int osc_reduce(void *team, int op, int dtype, void *dest, const void *src,
   log_call("reduce", t, op, dtype, nreduce);
   if (nreduce == 0) return OSC_OK;
   osc_team_geom(t, &g);
-  rc = oscd_reduce(&g, op, dtype, osc_g.fpmode[op][dtype], doff, soff,
-                   nreduce, stream ? stream : osc_g.stream);
+  rc = oscd_reduce(&g, op, dtype, osc_g.fpmode[op][dtype], osc_g.reduce_shape,
+                   doff, soff, nreduce, stream ? stream : osc_g.stream);
   return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "reduce launch");
 }
 

@@ -73,7 +73,9 @@ int osc_init(int npes, size_t heap_bytes_per_pe, int device) {
   void *base = NULL;
   const char *lg = getenv("OSC_LOG");
   if (osc_g.initialized) return osc_set_error(OSC_ERR_STATE, "already initialised");
+  const char *shp = getenv("OSC_REDUCE_SHAPE");
   osc_g.log_calls = (lg != NULL && lg[0] != '\0' && lg[0] != '0');
+  osc_g.reduce_shape = (shp != NULL && shp[0] != '\0') ? atoi(shp) : -1;
   if (npes < 1 || npes > OSC_MAX_NPES)
     return osc_set_error(OSC_ERR_ARG, "npes %d outside [1, %d]", npes,
                          OSC_MAX_NPES);

@@ -46,6 +46,7 @@ typedef struct osc_state {
   /* resolved algorithm choice: float mode per (op, dtype) */
   unsigned char fpmode[OSC_OP_COUNT][OSC_T_COUNT];
   int fp_default;
+  int reduce_shape; /* -1 = per-policy default; OSC_REDUCE_SHAPE overrides */
 } osc_state;
 
 extern osc_state osc_g;
