@@ -49,12 +49,35 @@ CELLS = [  # (op, dtype, dist, a, b)
 ESIZE = {"u64": 8, "i64": 8, "f32": 4, "f64": 8}
 
 
-def kernel_name(op, dtype, mode):
-    pol = {"f32": "PolF32Acc", "f64": "PolF64Acc"}
-    if dtype in pol and op in ("sum", "prod"):
-        return "k_reduce<%s,%s>" % (dtype, pol[dtype] if mode == 0
-                                    else "PolTree")
-    return "k_reduce<%s,PolSeq>" % dtype
+def kernel_key(op, dtype, mode):
+    """The kernel instance the library launches for one cell at the default
+    shape (8 units x 2 slices in flight; reference-order tree: 2 x 4), in
+    the short form the profiles/ summaries use."""
+    o = osc.OPS[op]
+    if dtype in ("u64", "i64"):
+        t = "long" if (dtype == "i64" and op in ("min", "max")) else "u64"
+        return "k_reduce<%s, PolSeq<%s, %d, false>, 2, 8, 2, 1>" % (t, t, o)
+    t = "float" if dtype == "f32" else "double"
+    vec = 4 if dtype == "f32" else 2
+    if op in ("min", "max"):
+        return "k_reduce<%s, PolSeq<%s, %d, true>, %d, 8, 2, 1>" % (t, t, o,
+                                                                     vec)
+    if mode == 0:
+        acc = "PolF32Acc" if dtype == "f32" else "PolF64Acc"
+        return "k_reduce<%s, %s<%d>, %d, 8, 2, 1>" % (t, acc, o, vec)
+    return "k_reduce<%s, PolTree<%s, %d>, %d, 2, 4, 8>" % (t, t, o, vec)
+
+
+def measured_traffic(key):
+    """HBM bytes per launch of `key` from the committed PMC summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated on the plain copy kernel)."""
+    path = os.path.join(ROOT, "profiles", "reduce_matrix_64pe_4M_traffic.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"].get(key)
+        return None if k is None else k["traffic_bytes"]
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def cpu_baseline(seconds_budget=20.0):
@@ -191,12 +214,14 @@ def main():
                                "elements per PE",
                    "npes": P, "nreduce": N, "fp_mode": args.fp_mode,
                    "parallelism": "1 GPU, 64 emulated PEs"},
-        "roofline": {"bound": "hbm", "kernel": kernel_name(
-            CELLS[dom][0], CELLS[dom][1], mode) + " %s" % CELLS[dom][0],
-            "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS,
-            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
-            "traffic": None,
-            "algorithmic_bytes_per_launch": dom_b},
+        "roofline": {"bound": "hbm",
+                     "kernel": kernel_key(CELLS[dom][0], CELLS[dom][1], mode),
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
+                     "traffic": measured_traffic(
+                         kernel_key(CELLS[dom][0], CELLS[dom][1], mode)),
+                     "algorithmic_bytes_per_launch": dom_b,
+                     "avg_launch_us": round(dom_avg * 1e6, 1)},
         "copy_ceiling_GBps": round(copy_gbps, 1),
         "cells": cells,
     }
