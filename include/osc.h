@@ -146,6 +146,66 @@ int osc_event_destroy(void *ev);
 int osc_event_record(void *ev, void *stream);
 int osc_event_elapsed_ms(void *start, void *stop, float *ms);
 
+/* ---- batched execution ---------------------------------------------------
+ * Many independent collective calls (uneven sizes, teams, types, ops) run
+ * as one captured hipGraph: calls are grouped in waves whose destinations
+ * do not overlap; each wave runs one persistent tile-loop kernel per kernel
+ * family over all of the wave's calls.  Results equal the same calls
+ * issued one by one. */
+typedef enum {
+  OSC_CALL_REDUCE = 0,
+  OSC_CALL_FCOLLECT,
+  OSC_CALL_ALLTOALL,
+  OSC_CALL_BROADCAST
+} osc_call_kind_t;
+
+typedef struct osc_call {
+  int32_t kind;       /* OSC_CALL_* */
+  int32_t op, dtype;  /* reduce only */
+  int32_t root;       /* broadcast: team rank of the root */
+  int32_t team_start, team_stride, team_npes; /* global PEs of the team */
+  int32_t wave;       /* set by osc_batch_plan (or by the caller) */
+  uint64_t count;     /* reduce: elements; others: bytes per block */
+  uint64_t src_off;   /* symmetric offsets, 16-byte aligned */
+  uint64_t dest_off;
+} osc_call;
+
+typedef struct osc_batch osc_batch;
+
+typedef struct osc_batch_info {
+  int ncalls, nwaves, nlaunches;
+  uint64_t ntiles;
+  double algorithmic_bytes; /* sum over calls of the bytes moved */
+} osc_batch_info;
+
+/* Place every call's destination inside [dest_base, dest_base+dest_bytes)
+ * of each PE of its team span and assign waves: within a wave, no two
+ * calls write the same byte of the same PE (skyline packing in call
+ * order).  *nwaves receives the wave count. */
+int osc_batch_plan(osc_call *calls, int ncalls, uint64_t dest_base,
+                   uint64_t dest_bytes, int *nwaves);
+/* The same for a world of world_npes PEs; pure host code, no GPU needed. */
+int osc_batch_plan_world(osc_call *calls, int ncalls, int world_npes,
+                         uint64_t dest_base, uint64_t dest_bytes,
+                         int *nwaves);
+/* Validate, tile and upload; tiles follow the tuning table. */
+int osc_batch_create(const osc_call *calls, int ncalls, osc_batch **out);
+/* Run all waves.  The first call captures the launches into a hipGraph
+ * on `stream`; later calls replay it. */
+int osc_batch_launch(osc_batch *b, void *stream);
+/* The same calls, one per-call kernel each, in call order (baseline). */
+int osc_batch_launch_serial(osc_batch *b, void *stream);
+/* Per-call digest of all members' destination bytes (host array of
+ * ncalls); equal digests mean bit-identical results. */
+int osc_batch_digests(osc_batch *b, uint64_t *digests, void *stream);
+int osc_batch_get_info(const osc_batch *b, osc_batch_info *info);
+int osc_batch_destroy(osc_batch *b);
+/* Tile parameters: per (class, size bucket); see osc_tune.c. */
+int osc_tune_load(const char *path);
+int osc_tune_save(const char *path);
+int osc_tune_set(int cls, int bucket, int shape, int tile_iters, int grid);
+int osc_tune_get(int cls, int bucket, int *shape, int *tile_iters, int *grid);
+
 /* ---- configuration helpers (pure host code, usable without a GPU) ------- */
 /* Parse a heap size such as "32M", "1G", "4096" (binary K/M/G/T). */
 int osc_parse_size(const char *s, size_t *out);

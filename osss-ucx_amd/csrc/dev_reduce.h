@@ -606,6 +606,14 @@ template <class Pol>
 constexpr int default_shape() {
   return Pol::TREE ? 2 : 0;
 }
+// The reference tree only runs the narrow shapes (its subtree stack takes
+// MAXLV*UPT*VEC registers).
+template <class Pol>
+constexpr int clamp_shape(int shape) {
+  return (shape < 0 || shape >= kNumShapes)
+             ? default_shape<Pol>()
+             : (Pol::TREE ? (shape == 3 ? 3 : 2) : shape);
+}
 
 template <class T, class Pol>
 static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
@@ -616,11 +624,10 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
                       (g->slice % AL == 0) && ((uintptr_t)g->base % AL == 0);
   const size_t lds = Pol::SPECIAL ? (size_t)g->npes * sizeof(T) : 0;
   if (n == 0) return OSC_OK;
-  if (shape < 0 || shape >= kNumShapes) shape = default_shape<Pol>();
+  shape = clamp_shape<Pol>(shape);
   if (!vec_ok) {
     launch_shape<T, Pol, 1, 1, 8>(g, lg, doff, soff, n, lds, s);
   } else if constexpr (Pol::TREE) {
-    // the tree's subtree stack takes MAXLV*UPT*VEC registers: narrow shapes
     if (shape == 3)
       launch_shape<T, Pol, VB, shape_upt(3), shape_infl(3)>(g, lg, doff, soff, n, lds, s);
     else
@@ -637,19 +644,197 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
   return e == hipSuccess ? OSC_OK : OSC_ERR_HIP;
 }
 
-// Family launchers, one translation unit each (compiled in parallel).
-int launch_reduce_int(const oscd_geom *g, int op, int dtype, int lg,
-                      uint64_t doff, uint64_t soff, uint64_t n, int shape,
-                      hipStream_t s);
-int launch_reduce_f32(const oscd_geom *g, int op, int mode, int lg,
-                      uint64_t doff, uint64_t soff, uint64_t n, int shape,
-                      hipStream_t s);
-int launch_reduce_f64(const oscd_geom *g, int op, int mode, int lg,
-                      uint64_t doff, uint64_t soff, uint64_t n, int shape,
-                      hipStream_t s);
-int launch_reduce_cplx(const oscd_geom *g, int op, int dtype, int mode,
-                       int lg, uint64_t doff, uint64_t soff, uint64_t n,
-                       int shape, hipStream_t s);
+// ---- batched tile loop -------------------------------------------------------
+// Workgroup b processes tiles b, b+grid, ...; each tile is whole block
+// iterations of one call (the planner aligns tile bounds to 256*UPT units),
+// so the per-call kernel body runs unchanged.
+template <class T, class Pol, int VEC, int UPT, int INFL, int MAXLV>
+__global__ __launch_bounds__(kBlock) void k_batch_reduce(
+    const oscd_bcall *__restrict__ calls, const oscd_tile *__restrict__ tiles,
+    const uint32_t ntiles, char *const base, const uint64_t slice) {
+  extern __shared__ unsigned char k_batch_lds[];
+  constexpr uint64_t kSpan = (uint64_t)kBlock * UPT;
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const oscd_tile tl = tiles[t];
+    const oscd_bcall c = calls[tl.call];
+    oscd_geom g;
+    g.base = base;
+    g.slice = slice;
+    g.start = c.start;
+    g.stride = c.stride;
+    g.npes = c.npes;
+    g.pad_ = 0;
+    const uint64_t n = c.count;
+    const uint64_t nfull = n / VEC;
+    for (uint64_t ub = tl.begin; ub < tl.end; ub += kSpan) {
+      if (ub + kSpan <= nfull)
+        reduce_block<true, T, Pol, VEC, UPT, INFL, MAXLV>(
+            g, c.dest_off, c.src_off, n, c.lg, ub, (T *)k_batch_lds);
+      else
+        reduce_block<false, T, Pol, VEC, UPT, INFL, MAXLV>(
+            g, c.dest_off, c.src_off, n, c.lg, ub, (T *)k_batch_lds);
+    }
+  }
+}
+
+struct BatchArgs {
+  const oscd_bcall *calls;
+  const oscd_tile *tiles;
+  uint32_t ntiles, grid;
+  size_t lds;
+  char *base;
+  uint64_t slice;
+  int maxlg;
+  hipStream_t s;
+};
+
+template <class T, class Pol, int UPT, int INFL>
+static void batch_shape(const BatchArgs &a) {
+  constexpr int VB = vec_of<T>();
+  if constexpr (Pol::TREE) {
+    if (a.maxlg >= 8) {
+      hipLaunchKernelGGL((k_batch_reduce<T, Pol, VB, UPT, INFL, 13>),
+                         dim3(a.grid), dim3(kBlock), a.lds, a.s, a.calls,
+                         a.tiles, a.ntiles, a.base, a.slice);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_batch_reduce<T, Pol, VB, UPT, INFL, Pol::TREE ? 8 : 1>),
+                     dim3(a.grid), dim3(kBlock), a.lds, a.s, a.calls, a.tiles,
+                     a.ntiles, a.base, a.slice);
+}
+
+// Batched families run the widest and the narrowest shape only (large
+// calls vs. calls of a few thousand elements).
+template <class T, class Pol>
+static int batch_launch(const BatchArgs &a, int shape) {
+  if (a.ntiles == 0) return OSC_OK;
+  shape = clamp_shape<Pol>(shape);
+  if constexpr (Pol::TREE) {
+    if (shape == 3)
+      batch_shape<T, Pol, shape_upt(3), shape_infl(3)>(a);
+    else
+      batch_shape<T, Pol, shape_upt(2), shape_infl(2)>(a);
+  } else {
+    if (shape == 3)
+      batch_shape<T, Pol, shape_upt(3), shape_infl(3)>(a);
+    else
+      batch_shape<T, Pol, shape_upt(0), shape_infl(0)>(a);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? OSC_OK : OSC_ERR_HIP;
+}
+
+// Units a workgroup covers per block iteration at a shape, as the
+// launchers above will run it.
+template <class Pol>
+static int span_units(int shape, bool batched) {
+  shape = clamp_shape<Pol>(shape);
+  if (batched && !Pol::TREE && shape != 3) shape = 0;
+  return kBlock * shape_upt(shape);
+}
+
+// ---- policy selection ----------------------------------------------------------
+// v.template run<T, Pol>() for the kernel policy of (dtype, op, mode).
+// Integer bitwise/sum/prod use the unsigned type of the same width
+// (two's-complement wrap, as the reference's C arithmetic on x86-64);
+// MIN/MAX compare with the element type's own signedness.
+template <class S, class U, class V>
+static int visit_int_t(int op, V &v) {
+  switch (op) {
+    case OSC_OP_AND: return v.template run<U, PolSeq<U, OSC_OP_AND, false>>();
+    case OSC_OP_OR: return v.template run<U, PolSeq<U, OSC_OP_OR, false>>();
+    case OSC_OP_XOR: return v.template run<U, PolSeq<U, OSC_OP_XOR, false>>();
+    case OSC_OP_SUM: return v.template run<U, PolSeq<U, OSC_OP_SUM, false>>();
+    case OSC_OP_PROD: return v.template run<U, PolSeq<U, OSC_OP_PROD, false>>();
+    case OSC_OP_MIN: return v.template run<S, PolSeq<S, OSC_OP_MIN, false>>();
+    case OSC_OP_MAX: return v.template run<S, PolSeq<S, OSC_OP_MAX, false>>();
+  }
+  return OSC_ERR_ARG;
+}
+
+// MIN/MAX: exact selection, per-member results on NaN / signed-zero ties.
+// SUM/PROD: reference-order tree (mode 1) or accurate accumulation.
+template <class T, template <int> class AccPol, class V>
+static int visit_real_t(int op, int mode, V &v) {
+  switch (op) {
+    case OSC_OP_MIN: return v.template run<T, PolSeq<T, OSC_OP_MIN, true>>();
+    case OSC_OP_MAX: return v.template run<T, PolSeq<T, OSC_OP_MAX, true>>();
+    case OSC_OP_SUM:
+      return mode == OSC_FP_REFERENCE
+                 ? v.template run<T, PolTree<T, OSC_OP_SUM>>()
+                 : v.template run<T, AccPol<OSC_OP_SUM>>();
+    case OSC_OP_PROD:
+      return mode == OSC_FP_REFERENCE
+                 ? v.template run<T, PolTree<T, OSC_OP_PROD>>()
+                 : v.template run<T, AccPol<OSC_OP_PROD>>();
+  }
+  return OSC_ERR_UNSUPPORTED;
+}
+
+template <class T, template <int> class AccPol, class V>
+static int visit_cplx_t(int op, int mode, V &v) {
+  switch (op) {
+    case OSC_OP_SUM:
+      return mode == OSC_FP_REFERENCE
+                 ? v.template run<T, PolTree<T, OSC_OP_SUM>>()
+                 : v.template run<T, AccPol<OSC_OP_SUM>>();
+    case OSC_OP_PROD:
+      return mode == OSC_FP_REFERENCE
+                 ? v.template run<T, PolTree<T, OSC_OP_PROD>>()
+                 : v.template run<T, AccPol<OSC_OP_PROD>>();
+  }
+  return OSC_ERR_UNSUPPORTED;
+}
+
+// Visitors: per-call launch, batched launch, tile-span query.
+struct PerCallV {
+  const oscd_geom *g;
+  int lg;
+  uint64_t doff, soff, n;
+  int shape;
+  hipStream_t s;
+  template <class T, class Pol>
+  int run() {
+    return launch<T, Pol>(g, lg, doff, soff, n, shape, s);
+  }
+};
+struct BatchV {
+  BatchArgs a;
+  int shape;
+  template <class T, class Pol>
+  int run() {
+    return batch_launch<T, Pol>(a, shape);
+  }
+};
+struct SpanV {
+  int shape;
+  template <class T, class Pol>
+  int run() {
+    return span_units<Pol>(shape, true);
+  }
+};
+
+// Family entry points, one translation unit each (compiled in parallel).
+int visit_reduce_int_wide(int dtype, int op, int mode, PerCallV *pc, BatchV *bv, SpanV *sv);
+int visit_reduce_int_narrow(int dtype, int op, int mode, PerCallV *pc, BatchV *bv, SpanV *sv);
+int visit_reduce_f32(int dtype, int op, int mode, PerCallV *pc, BatchV *bv, SpanV *sv);
+int visit_reduce_f64(int dtype, int op, int mode, PerCallV *pc, BatchV *bv, SpanV *sv);
+int visit_reduce_cplx(int dtype, int op, int mode, PerCallV *pc, BatchV *bv, SpanV *sv);
+
+// Exactly one of pc / bv / sv is non-null.
+#define OSC_VISIT_FAMILY(NAME, ...)                                           \
+  int NAME(int dtype, int op, int mode, PerCallV *pc, BatchV *bv,             \
+           SpanV *sv) {                                                       \
+    (void)mode;                                                               \
+    if (pc) { PerCallV &v = *pc; __VA_ARGS__ }                                \
+    if (bv) { BatchV &v = *bv; __VA_ARGS__ }                                  \
+    SpanV &v = *sv;                                                           \
+    __VA_ARGS__                                                               \
+  }
+
+int visit_reduce(int dtype, int op, int mode, PerCallV *pc, BatchV *bv,
+                 SpanV *sv);
 
 }  // namespace osc
 

@@ -84,6 +84,7 @@ int osc_init(int npes, size_t heap_bytes_per_pe, int device) {
   heap_bytes_per_pe = (heap_bytes_per_pe + 255) / 256 * 256;
   rc = osc_registry_init_from_env();
   if (rc != OSC_OK) return rc;
+  osc_tune_init_from_env();
   rc = oscd_get_device_count(&ndev);
   if (rc != OSC_OK) return osc_hip_error(rc, "hipGetDeviceCount");
   if (device < 0 || device >= ndev)

@@ -78,6 +78,10 @@ size_t osc_dtype_size(int dtype);
 enum { OSC_KIND_REDUCE = 0, OSC_KIND_FCOLLECT, OSC_KIND_ALLTOALL,
        OSC_KIND_BROADCAST };
 
+/* tuning table (osc_tune.c) */
+void osc_tune_init_from_env(void);
+int osc_tune_bucket(uint64_t bytes_per_pe);
+
 /* symmetric-argument validation (osc_coll.c) */
 int osc_sym_range(const void *p, size_t bytes, size_t *off);
 

@@ -72,6 +72,58 @@ int oscd_alltoall(const oscd_geom *g, uint64_t dest_off, uint64_t src_off,
 int oscd_broadcast(const oscd_geom *g, int root, uint64_t dest_off,
                    uint64_t src_off, uint64_t nbytes, void *stream);
 
+/* ---- batched execution ------------------------------------------------- */
+/* One collective call of a batch as the device sees it (64 bytes). */
+typedef struct oscd_bcall {
+  uint64_t src_off, dest_off;
+  uint64_t count; /* reduce: elements; fcollect/alltoall/broadcast: bytes */
+  int32_t start, stride, npes, lg;
+  int32_t kind;   /* OSC_CALL_* */
+  int32_t op, dtype;
+  int32_t aux;    /* reduce: float mode; broadcast: root (team rank) */
+  uint64_t pad_;
+} oscd_bcall;
+
+/* One tile: part of one call's work for one workgroup iteration.
+ *  reduce    : units [begin, end) of VEC elements (16 bytes)
+ *  fcollect  : source member `aux`, units [begin, end) of its block
+ *  broadcast : units [begin, end) of the root's block
+ *  alltoall  : if the block has >= 256 units: pair `aux`, units
+ *              [begin, end); else pairs [begin, end), all units. */
+typedef struct oscd_tile {
+  uint32_t call;
+  uint32_t aux;
+  uint64_t begin, end;
+} oscd_tile;
+
+/* Persistent tile loop for one reduce family (dtype, op, mode, shape);
+ * grid workgroups stride over ntiles tiles; lds_bytes covers the largest
+ * team's special-element snapshot. */
+int oscd_batch_reduce(int dtype, int op, int fpmode, int shape, int maxlg,
+                      const oscd_bcall *calls, const oscd_tile *tiles,
+                      uint32_t ntiles, uint32_t grid, size_t lds_bytes,
+                      char *base, uint64_t slice, void *stream);
+/* Persistent tile loop for data movement of unit width W (1..16). */
+int oscd_batch_move(int unit_width, const oscd_bcall *calls,
+                    const oscd_tile *tiles, uint32_t ntiles, uint32_t grid,
+                    char *base, uint64_t slice, void *stream);
+/* Per-call digest of every member's destination bytes (XOR of hashed
+ * words, order-independent); digests[] must be zeroed first. */
+int oscd_batch_digest(const oscd_bcall *calls, uint32_t ncalls,
+                      uint64_t *digests, char *base, uint64_t slice,
+                      void *stream);
+int oscd_fill_u64(uint64_t *p, uint64_t value, uint64_t count, void *stream);
+/* reduce kernel shape/unit geometry, for the host tile planner */
+int oscd_reduce_span_units(int dtype, int op, int fpmode, int shape);
+int oscd_reduce_vec(int dtype);
+int oscd_cu_count(int *n);
+
+/* graph capture of everything enqueued on `stream` between begin and end */
+int oscd_graph_begin(void *stream);
+int oscd_graph_end(void *stream, void **exec);
+int oscd_graph_launch(void *exec, void *stream);
+int oscd_graph_destroy(void *exec);
+
 /* plain device copy (put/get between slices, roofline calibration) */
 int oscd_copy(void *dst, const void *src, uint64_t bytes, void *stream);
 

@@ -54,6 +54,24 @@ class OscError(RuntimeError):
         self.code = code
 
 
+CALL_REDUCE, CALL_FCOLLECT, CALL_ALLTOALL, CALL_BROADCAST = 0, 1, 2, 3
+
+
+class Call(C.Structure):
+    """osc_call of include/osc.h."""
+    _fields_ = [("kind", C.c_int32), ("op", C.c_int32), ("dtype", C.c_int32),
+                ("root", C.c_int32), ("team_start", C.c_int32),
+                ("team_stride", C.c_int32), ("team_npes", C.c_int32),
+                ("wave", C.c_int32), ("count", C.c_uint64),
+                ("src_off", C.c_uint64), ("dest_off", C.c_uint64)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("ncalls", C.c_int), ("nwaves", C.c_int),
+                ("nlaunches", C.c_int), ("ntiles", C.c_uint64),
+                ("algorithmic_bytes", C.c_double)]
+
+
 _lib: Optional[C.CDLL] = None
 
 
@@ -98,6 +116,19 @@ def lib() -> C.CDLL:
         "osc_parse_size": (i, [C.c_char_p, C.POINTER(sz)]),
         "osc_algo_resolve": (i, [C.c_char_p, C.c_char_p, C.POINTER(i)]),
         "osc_device_copy": (i, [vp, vp, sz, vp]),
+        "osc_batch_plan": (i, [C.POINTER(Call), i, u64, u64, C.POINTER(i)]),
+        "osc_batch_plan_world": (i, [C.POINTER(Call), i, i, u64, u64,
+                                     C.POINTER(i)]),
+        "osc_batch_create": (i, [C.POINTER(Call), i, C.POINTER(vp)]),
+        "osc_batch_launch": (i, [vp, vp]),
+        "osc_batch_launch_serial": (i, [vp, vp]),
+        "osc_batch_digests": (i, [vp, C.POINTER(C.c_uint64), vp]),
+        "osc_batch_get_info": (i, [vp, C.POINTER(BatchInfo)]),
+        "osc_batch_destroy": (i, [vp]),
+        "osc_tune_load": (i, [C.c_char_p]),
+        "osc_tune_save": (i, [C.c_char_p]),
+        "osc_tune_set": (i, [i, i, i, i, i]),
+        "osc_tune_get": (i, [i, i, C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
         "shmem_init": (None, []),
         "shmem_finalize": (None, []),
         "shmem_my_pe": (i, []),
@@ -260,3 +291,56 @@ class Timer:
     def close(self):
         self.L.osc_event_destroy(self.a)
         self.L.osc_event_destroy(self.b)
+
+
+class Batch:
+    """A created osc_batch over a list of Call records."""
+
+    def __init__(self, calls):
+        L = lib()
+        self.L = L
+        self.n = len(calls)
+        self.calls = (Call * self.n)(*calls)
+        h = C.c_void_p()
+        check(L.osc_batch_create(self.calls, self.n, C.byref(h)),
+              "osc_batch_create")
+        self.h = h
+
+    def launch(self, stream=None):
+        check(self.L.osc_batch_launch(self.h, stream), "osc_batch_launch")
+
+    def launch_serial(self, stream=None):
+        check(self.L.osc_batch_launch_serial(self.h, stream),
+              "osc_batch_launch_serial")
+
+    def digests(self, stream=None) -> np.ndarray:
+        out = np.zeros(self.n, np.uint64)
+        check(self.L.osc_batch_digests(
+            self.h, out.ctypes.data_as(C.POINTER(C.c_uint64)), stream),
+            "osc_batch_digests")
+        return out
+
+    def info(self) -> BatchInfo:
+        inf = BatchInfo()
+        check(self.L.osc_batch_get_info(self.h, C.byref(inf)), "info")
+        return inf
+
+    def close(self):
+        if self.h:
+            self.L.osc_batch_destroy(self.h)
+            self.h = None
+
+
+def plan(calls, dest_base, dest_bytes, world_npes=None):
+    """osc_batch_plan (or _world): fills dest_off / wave in place."""
+    arr = (Call * len(calls))(*calls)
+    nw = C.c_int()
+    L = lib()
+    if world_npes is None:
+        rc = L.osc_batch_plan(arr, len(calls), dest_base, dest_bytes,
+                              C.byref(nw))
+    else:
+        rc = L.osc_batch_plan_world(arr, len(calls), world_npes, dest_base,
+                                    dest_bytes, C.byref(nw))
+    check(rc, "osc_batch_plan")
+    return list(arr), nw.value
