@@ -107,6 +107,64 @@ def cpu_baseline(seconds_budget=20.0):
                       "matrix cells, median of 3" % (threads, n)}
 
 
+def bench_batched(seed=4, ncalls=4096):
+    """Config 4: ncalls independent calls (osc_calls.batched_mix) in one
+    captured graph vs. the same calls issued one per kernel, on a 128-PE
+    world; every call's result is verified batched == one-by-one by
+    per-call digests, wave by wave."""
+    import numpy as np
+    import osc_calls
+    world = osc_calls.CallWorld(npes=128, max_reduce=1048576,
+                                move_bytes=32 << 20, dest_bytes=1 << 30)
+    try:
+        rng = np.random.default_rng(seed)
+        calls = osc_calls.batched_mix(rng, world, ncalls)
+        calls, nwaves = osc.plan(calls, world.dest_off, world.dest_bytes)
+        L = world.h.L
+        stream = L.osc_default_stream()
+        # correctness, wave by wave (destinations of one wave are disjoint)
+        match = True
+        for w in range(nwaves):
+            bw = osc.Batch([c for c in calls if c.wave == w])
+            bw.launch_serial()
+            d1 = bw.digests()
+            bw.launch()
+            d2 = bw.digests()
+            match = match and bool(np.array_equal(d1, d2))
+            bw.close()
+        b = osc.Batch(calls)
+        info = b.info()
+        t = osc.Timer(stream)
+        b.launch()  # capture + warm
+        b.launch_serial()
+        world.h.sync()
+        serial, batched = [], []
+        for _ in range(3):
+            t.start()
+            b.launch_serial()
+            t.stop()
+            serial.append(t.ms())
+            t.start()
+            b.launch()
+            t.stop()
+            batched.append(t.ms())
+        t.close()
+        b.close()
+        ms_b, ms_s = sorted(batched)[1], sorted(serial)[1]
+        gb = info.algorithmic_bytes
+        return {"calls": ncalls, "world_pes": 128, "waves": nwaves,
+                "launches_in_graph": info.nlaunches,
+                "tiles": int(info.ntiles),
+                "algorithmic_GB": round(gb / 1e9, 3),
+                "batched_ms": round(ms_b, 3), "serial_ms": round(ms_s, 3),
+                "batched_GBps": round(gb / (ms_b / 1e3) / 1e9, 1),
+                "serial_GBps": round(gb / (ms_s / 1e3) / 1e9, 1),
+                "speedup": round(ms_s / ms_b, 3),
+                "digests_match": match}
+    finally:
+        world.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +173,7 @@ def main():
     ap.add_argument("--fp-mode", choices=["accurate", "reference"],
                     default="accurate")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the emulated PEs all live in one "
@@ -226,6 +285,7 @@ def main():
         "cells": cells,
     }
     h.close()
+    out["batched"] = None if args.no_batched else bench_batched()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     else:
