@@ -165,6 +165,97 @@ def bench_batched(seed=4, ncalls=4096):
         world.close()
 
 
+def bench_served(seed=5, nreq=2000, rate=20000.0, verify=96):
+    """Config 5: 16 teams of 8..128 PEs in a 128-PE world send Poisson
+    requests at `rate` (60% f32/f64/i64 sum|max reduce with nreduce
+    log-uniform 256..4M, 20% fcollect, 20% alltoall, blocks 1 KiB..1 MiB).
+    Heap (inputs + destination arena) is sized to a 200 GB budget; the
+    admission budget is what the inputs leave of it.  Runs continuous
+    batching and the per-call baseline on the same request stream, then a
+    verification run with digests: sampled requests must give identical
+    results alone, and the f32 sums' error vs an FP64 oracle is measured
+    on sampled columns."""
+    import numpy as np
+    import osc_calls
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    budget_total = 200e9
+    npes = 128
+    inputs = (4 + 8 + 8) * 4194304 + (128 << 20)   # f32, f64, i64 + moves
+    arena = int((budget_total / npes - inputs) // (2 << 20)) * (2 << 20) \
+        - (64 << 20)
+    scratch = 128 << 20
+    world = osc_calls.CallWorld(npes=npes, max_reduce=4194304,
+                                move_bytes=128 << 20,
+                                dest_bytes=arena + scratch,
+                                keys=["f32", "f64", "i64"], seed=seed)
+    try:
+        rng = np.random.default_rng(seed)
+        teams = osc_calls.served_teams(rng, world, 16, 8, 128)
+        reqs = osc_calls.served_mix(rng, world, teams, nreq, rate)
+
+        def cfg(mode, digests):
+            c = osc.ServeCfg()
+            c.mode, c.max_inflight, c.nstreams, c.max_group = mode, 256, 4, 64
+            c.dest_base, c.dest_bytes = world.dest_off, arena
+            c.page_bytes = 256 << 10
+            c.budget_bytes = budget_total - npes * inputs
+            c.digests = digests
+            return c
+
+        def summary(st):
+            return {"p50_ms": round(st.p50_ms, 3), "p90_ms": round(st.p90_ms, 3),
+                    "p99_ms": round(st.p99_ms, 3), "max_ms": round(st.max_ms, 3),
+                    "mean_ms": round(st.mean_ms, 3),
+                    "wall_s": round(st.wall_s, 4), "GBps": round(st.gbps, 1),
+                    "groups": st.groups,
+                    "max_inflight": st.max_inflight_seen,
+                    "peak_inflight_GB": round(st.peak_inflight_bytes / 1e9, 2),
+                    "completed": st.completed, "rejected": st.rejected}
+
+        osc.serve(reqs[:64], cfg(osc.SERVE_BATCHED, 0))  # warm-up
+        _, st_b = osc.serve(reqs, cfg(osc.SERVE_BATCHED, 0))
+        _, st_p = osc.serve(reqs, cfg(osc.SERVE_PER_CALL, 0))
+        # verification: same stream with digests, then sampled requests alone
+        out, _ = osc.serve(reqs, cfg(osc.SERVE_BATCHED, 1))
+        pick = rng.choice(len(out), size=min(verify, len(out)), replace=False)
+        same = all(osc.request_alone(out[i].call, world.dest_off + arena) ==
+                   out[i].digest for i in pick)
+        # FP32 sum error vs the FP64 oracle on sampled columns
+        import oracle as O
+        h, base0 = world.h, world.h.L.osc_heap_base()
+        max_err = 0.0
+        f32_sums = [r for r in out if r.call.kind == osc.CALL_REDUCE and
+                    r.call.dtype == osc.DTYPES["f32"] and
+                    r.call.op == osc.OPS["sum"] and
+                    r.call.count <= 262144][:8]
+        for r in f32_sums:
+            c = r.call
+            osc.request_alone(c, world.dest_off + arena)
+            cols = rng.choice(c.count, size=min(512, c.count), replace=False)
+            pes = [c.team_start + k for k in range(c.team_npes)]
+            src = np.stack([h.get(base0 + c.src_off, c.count, np.float32,
+                                  p)[cols] for p in pes])
+            got = h.get(base0 + world.dest_off + arena, c.count, np.float32,
+                        pes[0])[cols]
+            hi, lo = O.accurate("sum", "f32", src)
+            err = np.abs((got.astype(np.float64) - hi) - lo) / np.abs(hi)
+            max_err = max(max_err, float(err.max()))
+        return {"requests": nreq, "rate_per_s": rate, "teams": 16,
+                "world_pes": npes,
+                "hbm_budget_GB": budget_total / 1e9,
+                "admission_budget_GB": round((budget_total - npes * inputs)
+                                             / 1e9, 1),
+                "offered_GB": round(st_b.algorithmic_bytes / 1e9, 1),
+                "served": summary(st_b), "per_call": summary(st_p),
+                "p99_speedup": round(st_p.p99_ms / st_b.p99_ms, 3),
+                "GBps_speedup": round(st_b.gbps / st_p.gbps, 3),
+                "same_alone_and_in_mix": bool(same),
+                "verified_requests": int(len(pick)),
+                "max_fp32_sum_rel_err_vs_fp64": max_err}
+    finally:
+        world.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +265,7 @@ def main():
                     default="accurate")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the emulated PEs all live in one "
@@ -286,6 +378,7 @@ def main():
     }
     h.close()
     out["batched"] = None if args.no_batched else bench_batched()
+    out["served"] = None if args.no_served else bench_served()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     else:

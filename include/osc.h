@@ -206,6 +206,56 @@ int osc_tune_save(const char *path);
 int osc_tune_set(int cls, int bucket, int shape, int tile_iters, int grid);
 int osc_tune_get(int cls, int bucket, int *shape, int *tile_iters, int *grid);
 
+/* ---- served mode -----------------------------------------------------------
+ * A request loop for many concurrent teams: requests arrive over time
+ * (arrival_s), are admitted in arrival order when a slot and destination
+ * memory inside the HBM budget are free, and run on the device as soon as
+ * admitted.  OSC_SERVE_BATCHED is continuous batching: every admitted
+ * request that is not yet running joins the next launch group (one tiled
+ * persistent launch per kernel family), groups run concurrently on several
+ * streams, and memory is released as each group completes.
+ * OSC_SERVE_PER_CALL is the baseline: each request is its own per-call
+ * kernel launch, in arrival order on one stream. */
+typedef enum { OSC_SERVE_BATCHED = 0, OSC_SERVE_PER_CALL = 1 } osc_serve_mode_t;
+
+typedef struct osc_request {
+  osc_call call;     /* dest_off is assigned by the server */
+  double arrival_s;  /* seconds after the start of the run (non-decreasing) */
+  double admit_s;    /* filled: memory and slot granted */
+  double launch_s;   /* filled: enqueued on a device stream */
+  double done_s;     /* filled: device work observed complete */
+  uint64_t digest;   /* filled when cfg->digests */
+  int32_t group;     /* filled: launch group id */
+  int32_t status;    /* filled: 0, or an OSC_ERR_* code if rejected */
+} osc_request;
+
+typedef struct osc_serve_cfg {
+  int32_t mode;         /* OSC_SERVE_* */
+  int32_t max_inflight; /* admitted, unfinished requests */
+  int32_t nstreams;     /* device streams for launch groups (batched) */
+  int32_t max_group;    /* requests per launch group (batched) */
+  uint64_t dest_base;   /* per-PE destination arena [base, base + bytes) */
+  uint64_t dest_bytes;
+  uint64_t page_bytes;  /* allocation granule inside the arena */
+  double budget_bytes;  /* admission budget: in-flight destination bytes
+                           summed over every team PE */
+  int32_t digests;      /* digest each result before its memory is reused */
+  int32_t pad_;
+} osc_serve_cfg;
+
+typedef struct osc_serve_stats {
+  int32_t completed, rejected, groups, max_inflight_seen;
+  double wall_s, mean_ms, p50_ms, p90_ms, p99_ms, max_ms;
+  double algorithmic_bytes, gbps, peak_inflight_bytes;
+} osc_serve_stats;
+
+int osc_serve_run(osc_request *reqs, int nreq, const osc_serve_cfg *cfg,
+                  osc_serve_stats *stats);
+/* One call alone through the per-call kernels, writing at dest_off; its
+ * digest (same function as the served and batched digests). */
+int osc_request_alone(const osc_call *call, uint64_t dest_off,
+                      uint64_t *digest);
+
 /* ---- configuration helpers (pure host code, usable without a GPU) ------- */
 /* Parse a heap size such as "32M", "1G", "4096" (binary K/M/G/T). */
 int osc_parse_size(const char *s, size_t *out);

@@ -71,6 +71,13 @@ int oscd_memcpy_d2d_async(void *dst, const void *src, size_t bytes,
   return OSC_OK;
 }
 
+int oscd_memcpy_h2d_async(void *dst, const void *src, size_t bytes,
+                          void *stream) {
+  HIPCALL(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice,
+                         (hipStream_t)stream));
+  return OSC_OK;
+}
+
 int oscd_memset_async(void *dst, int value, size_t bytes, void *stream) {
   HIPCALL(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream));
   return OSC_OK;

@@ -130,6 +130,7 @@ int osc_finalize(void) {
   if (!osc_g.initialized) return OSC_OK;
   oscd_stream_sync(osc_g.stream);
   oscd_stream_destroy(osc_g.stream);
+  if (osc_g.aux_dev) oscd_free(osc_g.aux_dev);
   oscd_free(osc_g.base);
   osc_heap_fini();
   pthread_mutex_destroy(&osc_g.lock);

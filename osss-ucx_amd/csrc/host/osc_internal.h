@@ -47,6 +47,7 @@ typedef struct osc_state {
   unsigned char fpmode[OSC_OP_COUNT][OSC_T_COUNT];
   int fp_default;
   int reduce_shape; /* -1 = per-policy default; OSC_REDUCE_SHAPE overrides */
+  void *aux_dev;    /* 256 device bytes for single-call descriptors */
 } osc_state;
 
 extern osc_state osc_g;

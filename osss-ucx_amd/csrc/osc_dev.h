@@ -35,6 +35,8 @@ int oscd_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int oscd_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int oscd_memcpy_d2d_async(void *dst, const void *src, size_t bytes,
                           void *stream);
+int oscd_memcpy_h2d_async(void *dst, const void *src, size_t bytes,
+                          void *stream);
 int oscd_memset_async(void *dst, int value, size_t bytes, void *stream);
 int oscd_stream_create(void **s);
 int oscd_stream_destroy(void *s);

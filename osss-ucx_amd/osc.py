@@ -344,3 +344,56 @@ def plan(calls, dest_base, dest_bytes, world_npes=None):
                                     dest_bytes, C.byref(nw))
     check(rc, "osc_batch_plan")
     return list(arr), nw.value
+
+
+SERVE_BATCHED, SERVE_PER_CALL = 0, 1
+
+
+class Request(C.Structure):
+    """osc_request of include/osc.h."""
+    _fields_ = [("call", Call), ("arrival_s", C.c_double),
+                ("admit_s", C.c_double), ("launch_s", C.c_double),
+                ("done_s", C.c_double), ("digest", C.c_uint64),
+                ("group", C.c_int32), ("status", C.c_int32)]
+
+
+class ServeCfg(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("max_inflight", C.c_int32),
+                ("nstreams", C.c_int32), ("max_group", C.c_int32),
+                ("dest_base", C.c_uint64), ("dest_bytes", C.c_uint64),
+                ("page_bytes", C.c_uint64), ("budget_bytes", C.c_double),
+                ("digests", C.c_int32), ("pad_", C.c_int32)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("completed", C.c_int32), ("rejected", C.c_int32),
+                ("groups", C.c_int32), ("max_inflight_seen", C.c_int32),
+                ("wall_s", C.c_double), ("mean_ms", C.c_double),
+                ("p50_ms", C.c_double), ("p90_ms", C.c_double),
+                ("p99_ms", C.c_double), ("max_ms", C.c_double),
+                ("algorithmic_bytes", C.c_double), ("gbps", C.c_double),
+                ("peak_inflight_bytes", C.c_double)]
+
+
+def serve(reqs, cfg):
+    """osc_serve_run over a list of Request; returns (requests, stats)."""
+    L = lib()
+    fn = L.osc_serve_run
+    fn.restype = C.c_int
+    fn.argtypes = [C.POINTER(Request), C.c_int, C.POINTER(ServeCfg),
+                   C.POINTER(ServeStats)]
+    arr = (Request * len(reqs))(*reqs)
+    st = ServeStats()
+    check(fn(arr, len(reqs), C.byref(cfg), C.byref(st)), "osc_serve_run")
+    return list(arr), st
+
+
+def request_alone(call, dest_off):
+    """osc_request_alone: run one call by itself at dest_off; its digest."""
+    L = lib()
+    fn = L.osc_request_alone
+    fn.restype = C.c_int
+    fn.argtypes = [C.POINTER(Call), C.c_uint64, C.POINTER(C.c_uint64)]
+    d = C.c_uint64()
+    check(fn(C.byref(call), dest_off, C.byref(d)), "osc_request_alone")
+    return d.value

@@ -37,10 +37,11 @@ class CallWorld:
     max_reduce: largest nreduce; move_bytes: per-PE bytes of the
     data-movement input area; dest_bytes: per-PE destination area."""
 
-    def __init__(self, npes, max_reduce, move_bytes, dest_bytes, seed=1):
+    def __init__(self, npes, max_reduce, move_bytes, dest_bytes, seed=1,
+                 keys=None):
         self.npes = npes
         self.max_reduce = max_reduce
-        keys = sorted(DIST)
+        keys = sorted(DIST) if keys is None else sorted(keys)
         self.in_bytes = {k: max_reduce * ESZ[k[:3]] for k in keys}
         heap = sum(self.in_bytes.values()) + move_bytes + dest_bytes + (1 << 20)
         self.h = osc.Heap(npes=npes, heap_bytes=heap)
@@ -109,3 +110,47 @@ def batched_mix(rng, world, ncalls, pmin=2, pmax=128, nmin=64,
             n = min(log_uniform_int(rng, nmin, nmax), world.max_reduce)
             calls.append(reduce_call(world, op, dt, P, start, n))
     return calls
+
+
+# ---- the served configuration ---------------------------------------------
+SERVED_REDUCE = [("sum", "f32"), ("max", "f32"), ("sum", "f64"),
+                 ("max", "f64"), ("sum", "i64"), ("max", "i64")]
+
+
+def served_teams(rng, world, nteams=16, pmin=8, pmax=128):
+    """nteams contiguous teams of uniform size in [pmin, pmax] at random
+    positions of the world (teams may overlap, as split teams can)."""
+    teams = []
+    for _ in range(nteams):
+        P = int(rng.integers(pmin, min(pmax, world.npes) + 1))
+        teams.append((int(rng.integers(0, world.npes - P + 1)), P))
+    return teams
+
+
+def served_mix(rng, world, teams, nreq, rate_hz, nmin=256, nmax=4194304,
+               bmin=1024, bmax=1048576):
+    """Poisson arrivals at rate_hz from uniformly chosen teams: 60% reduce
+    (f32/f64/i64 sum or max, nreduce log-uniform [nmin, nmax]), 20%
+    fcollect, 20% alltoall (block log-uniform [bmin, bmax] bytes, a
+    multiple of 16)."""
+    reqs = []
+    t = 0.0
+    for _ in range(nreq):
+        t += rng.exponential(1.0 / rate_hz)
+        start, P = teams[int(rng.integers(len(teams)))]
+        u = rng.random()
+        if u < 0.6:
+            op, dt = SERVED_REDUCE[int(rng.integers(len(SERVED_REDUCE)))]
+            n = min(log_uniform_int(rng, nmin, nmax), world.max_reduce)
+            c = reduce_call(world, op, dt, P, start, n)
+        else:
+            kind = osc.CALL_FCOLLECT if u < 0.8 else osc.CALL_ALLTOALL
+            block = max(16, log_uniform_int(rng, bmin, bmax) // 16 * 16)
+            if kind == osc.CALL_ALLTOALL:
+                block = min(block, world.move_bytes // P // 16 * 16)
+            c = move_call(world, kind, P, start, block)
+        r = osc.Request()
+        r.call = c
+        r.arrival_s = t
+        reqs.append(r)
+    return reqs
