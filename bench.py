@@ -107,6 +107,73 @@ def cpu_baseline(seconds_budget=20.0):
                       "matrix cells, median of 3" % (threads, n)}
 
 
+def _median_event_ms(L, stream, fn, reps=10):
+    t = osc.Timer(stream)
+    fn()
+    L.osc_stream_sync(stream)
+    ms = []
+    for _ in range(reps):
+        t.start()
+        fn()
+        t.stop()
+        ms.append(t.ms())
+    t.close()
+    return sorted(ms)[len(ms) // 2]
+
+
+def bench_movement():
+    """Configs 3a/3b at full size: shmem_int64_alltoall with 65,536-byte
+    blocks on 64 PEs (2*P^2*b bytes) and shmem_int32_fcollect of 1 MiB per
+    PE on 256 PEs (P*m + P^2*m bytes), inputs resident, HIP-event medians.
+    Also config 1's latency: shmem_int_sum_reduce, 8 PEs x 65,536 int32,
+    kernel time and the blocking call's wall time."""
+    import numpy as np
+    L = osc.lib()
+    out = {}
+    P, blk = 64, 8192
+    with osc.Heap(npes=P, heap_bytes=2 * P * blk * 8 + (1 << 20)) as h:
+        s, d = h.malloc(P * blk * 8), h.malloc(P * blk * 8)
+        h.fill(None, s, P * blk, "i64", "a2a_pattern", blk, 0)
+        st = L.osc_default_stream()
+        ms = _median_event_ms(L, st, lambda: L.osc_alltoall(
+            h.world, d, s, blk * 8, st))
+        b = 2.0 * P * P * blk * 8
+        out["alltoall_64pe_64KiB"] = {"ms": round(ms, 4),
+                                      "GBps": round(b / ms / 1e6, 1),
+                                      "pct_peak": round(b / ms / 1e6 /
+                                                        PEAK_HBM_GBPS * 100, 1)}
+    P, n = 256, 262144
+    with osc.Heap(npes=P, heap_bytes=(P + 1) * n * 4 + (1 << 20)) as h:
+        s, d = h.malloc(n * 4), h.malloc(P * n * 4)
+        h.fill(None, s, n, "i32", "fcol_pattern")
+        st = L.osc_default_stream()
+        ms = _median_event_ms(L, st, lambda: L.osc_fcollect(
+            h.world, d, s, n * 4, st), reps=5)
+        b = P * n * 4.0 + P * P * n * 4.0
+        out["fcollect_256pe_1MiB"] = {"ms": round(ms, 3),
+                                      "GBps": round(b / ms / 1e6, 1),
+                                      "pct_peak": round(b / ms / 1e6 /
+                                                        PEAK_HBM_GBPS * 100, 1)}
+    P, n = 8, 65536
+    with osc.Heap(npes=P, heap_bytes=4 << 20) as h:
+        s, d = h.malloc(n * 4), h.malloc(n * 4)
+        h.fill(None, s, n, "i32", "int_range", -1000, 1000, seed=0)
+        st = L.osc_default_stream()
+        kern = _median_event_ms(L, st, lambda: L.osc_reduce(
+            h.world, osc.OPS["sum"], osc.DTYPES["i32"], d, s, n, st), reps=50)
+        fn = osc.typed("shmem_int_sum_reduce")
+        walls = []
+        for _ in range(200):
+            t0 = time.perf_counter()
+            fn(h.world, d, s, n)
+            walls.append((time.perf_counter() - t0) * 1e6)
+        walls.sort()
+        out["c1_int_sum_8pe_64K"] = {"kernel_us": round(kern * 1e3, 2),
+                                     "call_p50_us": round(walls[100], 2),
+                                     "call_p99_us": round(walls[198], 2)}
+    return out
+
+
 def bench_batched(seed=4, ncalls=4096):
     """Config 4: ncalls independent calls (osc_calls.batched_mix) in one
     captured graph vs. the same calls issued one per kernel, on a 128-PE
@@ -266,6 +333,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-movement", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the emulated PEs all live in one "
@@ -377,6 +445,7 @@ def main():
         "cells": cells,
     }
     h.close()
+    out["movement"] = None if args.no_movement else bench_movement()
     out["batched"] = None if args.no_batched else bench_batched()
     out["served"] = None if args.no_served else bench_served()
     if not args.no_cpu:
