@@ -114,6 +114,8 @@ int osc_set_fp_mode(int mode);
 int osc_get_fp_mode(void);
 const char *osc_strerror(int code);
 const char *osc_last_error(void);
+int osc_last_error_code(void); /* code of the calling thread's last error */
+void osc_clear_error(void);
 int osc_set_my_pe(int pe); /* which emulated PE the calling thread is */
 
 /* ---- symmetric addresses ------------------------------------------------ */
@@ -134,6 +136,16 @@ int osc_alltoall(void *team, void *dest, const void *src, size_t nbytes,
                  void *stream);
 int osc_broadcast(void *team, void *dest, const void *src, size_t nbytes,
                   int root, void *stream);
+/* collect with one size per member: nbytes[i] for team member i (npes
+ * entries); every member's dest receives the sources concatenated in team
+ * order (reference src/shcoll/src/collect.c:406-492 closed form). */
+int osc_collect(void *team, void *dest, const void *src, const size_t *nbytes,
+                void *stream);
+/* strided alltoall of nelems elements of elem_bytes per member pair;
+ * dst/sst >= 1 in elements (reference src/shcoll/src/alltoalls.c:54-90). */
+int osc_alltoalls(void *team, void *dest, const void *src, ptrdiff_t dst,
+                  ptrdiff_t sst, size_t elem_bytes, size_t nelems,
+                  void *stream);
 int osc_stream_sync(void *stream);
 
 /* ---- synthetic inputs --------------------------------------------------- */

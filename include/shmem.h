@@ -77,11 +77,94 @@ int shmem_team_split_2d(shmem_team_t parent_team, int xrange,
 void shmem_team_destroy(shmem_team_t team);
 int shmem_team_sync(shmem_team_t team);
 
-/* ---- RMA between emulated PEs (reference src/api/rma/putget.h:96-556) --- */
+/* ---- constants (reference include/shmem/defs.h:61-103) ------------------ */
+enum shmem_cmp_constants {
+  SHMEM_CMP_EQ = 0,
+  SHMEM_CMP_NE,
+  SHMEM_CMP_GT,
+  SHMEM_CMP_LE,
+  SHMEM_CMP_LT,
+  SHMEM_CMP_GE
+};
+enum shmem_signal_ops { SHMEM_SIGNAL_SET = 0, SHMEM_SIGNAL_ADD };
+
+/* pSync / pWrk sizes the active-set forms take.  The arrays are accepted
+ * and never touched: stream order replaces the pSync protocol. */
+#define SHMEM_BCAST_SYNC_SIZE 64
+#define SHMEM_BARRIER_SYNC_SIZE 2
+#define SHMEM_REDUCE_SYNC_SIZE 128
+#define SHMEM_REDUCE_MIN_WRKDATA_SIZE 64
+#define SHMEM_COLLECT_SYNC_SIZE 128
+#define SHMEM_ALLTOALL_SYNC_SIZE 64
+#define SHMEM_ALLTOALLS_SYNC_SIZE 64
+#define SHMEM_SYNC_SIZE SHMEM_REDUCE_SYNC_SIZE
+#define SHMEM_SYNC_VALUE 0L
+
+/* ---- RMA between emulated PEs (reference src/api/rma/putget.h:96-600,
+ * put_signal.h:28-290) ------------------------------------------------------
+ * The local side of a put (source) or get (dest) may be host memory or a
+ * symmetric address; the remote side is the symmetric object on PE pe.
+ * Blocking forms return when the data has landed; _nbi forms are complete
+ * at the next shmem_quiet / shmem_barrier_all.  Strides (tst, sst) are in
+ * elements. */
 void shmem_putmem(void *dest, const void *source, size_t nbytes, int pe);
 void shmem_getmem(void *dest, const void *source, size_t nbytes, int pe);
 void shmem_putmem_nbi(void *dest, const void *source, size_t nbytes, int pe);
 void shmem_getmem_nbi(void *dest, const void *source, size_t nbytes, int pe);
+
+#define OSC_DECL_RMA(_type, _typename)                                         \
+  void shmem_##_typename##_put(_type *dest, const _type *src, size_t nelems,   \
+                               int pe);                                        \
+  void shmem_##_typename##_put_nbi(_type *dest, const _type *src,              \
+                                   size_t nelems, int pe);                     \
+  void shmem_##_typename##_get(_type *dest, const _type *src, size_t nelems,   \
+                               int pe);                                        \
+  void shmem_##_typename##_get_nbi(_type *dest, const _type *src,              \
+                                   size_t nelems, int pe);                     \
+  void shmem_##_typename##_p(_type *dest, _type value, int pe);                \
+  _type shmem_##_typename##_g(const _type *src, int pe);                       \
+  void shmem_##_typename##_iput(_type *dest, const _type *src, ptrdiff_t tst,  \
+                                ptrdiff_t sst, size_t nelems, int pe);         \
+  void shmem_##_typename##_iget(_type *dest, const _type *src, ptrdiff_t tst,  \
+                                ptrdiff_t sst, size_t nelems, int pe);         \
+  void shmem_##_typename##_put_signal(_type *dest, const _type *src,           \
+                                      size_t nelems, uint64_t *sig_addr,       \
+                                      uint64_t signal, int sig_op, int pe);    \
+  void shmem_##_typename##_put_signal_nbi(_type *dest, const _type *src,       \
+                                          size_t nelems, uint64_t *sig_addr,   \
+                                          uint64_t signal, int sig_op,         \
+                                          int pe);
+#define OSC_DECL_SIZED_RMA(_size)                                              \
+  void shmem_put##_size(void *dest, const void *src, size_t nelems, int pe);   \
+  void shmem_put##_size##_nbi(void *dest, const void *src, size_t nelems,      \
+                              int pe);                                         \
+  void shmem_get##_size(void *dest, const void *src, size_t nelems, int pe);   \
+  void shmem_get##_size##_nbi(void *dest, const void *src, size_t nelems,      \
+                              int pe);                                         \
+  void shmem_iput##_size(void *dest, const void *src, ptrdiff_t tst,           \
+                         ptrdiff_t sst, size_t nelems, int pe);                \
+  void shmem_iget##_size(void *dest, const void *src, ptrdiff_t tst,           \
+                         ptrdiff_t sst, size_t nelems, int pe);                \
+  void shmem_put##_size##_signal(void *dest, const void *src, size_t nelems,   \
+                                 uint64_t *sig_addr, uint64_t signal,          \
+                                 int sig_op, int pe);                          \
+  void shmem_put##_size##_signal_nbi(void *dest, const void *src,              \
+                                     size_t nelems, uint64_t *sig_addr,        \
+                                     uint64_t signal, int sig_op, int pe);
+#define OSC_SIZED_RMA(X) X(8) X(16) X(32) X(64) X(128)
+
+void shmem_putmem_signal(void *dest, const void *source, size_t nbytes,
+                         uint64_t *sig_addr, uint64_t signal, int sig_op,
+                         int pe);
+void shmem_putmem_signal_nbi(void *dest, const void *source, size_t nbytes,
+                             uint64_t *sig_addr, uint64_t signal, int sig_op,
+                             int pe);
+/* the calling PE's signal word (reference src/api/rma/put_signal.c:351) */
+uint64_t shmem_signal_fetch(const uint64_t *sig_addr);
+/* waits until the calling PE's signal word satisfies cmp (reference
+ * src/api/pt2pt_synch/waituntil.c:121-127) */
+uint64_t shmem_signal_wait_until(uint64_t *sig_addr, int cmp,
+                                 uint64_t cmp_value);
 
 /* ---- type tables -------------------------------------------------------
  * The sets of <T> each collective is defined for.  The lists are those of
@@ -167,6 +250,9 @@ void shmem_getmem_nbi(void *dest, const void *source, size_t nbytes, int pe);
   X(double _Complex, complexd)                                                 \
   X(float _Complex, complexf)
 
+OSC_RMA_TYPES(OSC_DECL_RMA)
+OSC_SIZED_RMA(OSC_DECL_SIZED_RMA)
+
 /* ---- team reductions (reference include/shmem/api.h:3079-3260) ---------
  * long double is declared for link compatibility; on gfx950 (no 80-bit
  * type) it returns OSC_ERR_UNSUPPORTED. */
@@ -205,6 +291,92 @@ int shmem_alltoallmem(shmem_team_t team, void *dest, const void *source,
                       size_t nbytes);
 int shmem_broadcastmem(shmem_team_t team, void *dest, const void *source,
                        size_t nbytes, int PE_root);
+
+/* collect (reference include/shmem/api.h:3395-3415): every member's dest
+ * receives the members' sources concatenated in team order.  In team-call
+ * mode one call stands for all members, so nelems applies to each of them;
+ * osc_collect() in <osc.h> takes one size per member (the variable-size
+ * case).  alltoalls (api.h:3565-3587): strided alltoall, dst/sst in
+ * elements; alltoallsmem moves one element of nelems bytes per pair, as
+ * the reference's front end does (src/shcoll/src/alltoalls.c:391-428). */
+#define OSC_DECL_MOVE2(_type, _typename)                                       \
+  int shmem_##_typename##_collect(shmem_team_t team, _type *dest,              \
+                                  const _type *source, size_t nelems);         \
+  int shmem_##_typename##_alltoalls(shmem_team_t team, _type *dest,            \
+                                    const _type *source, ptrdiff_t dst,        \
+                                    ptrdiff_t sst, size_t nelems);
+OSC_RMA_TYPES(OSC_DECL_MOVE2)
+int shmem_collectmem(shmem_team_t team, void *dest, const void *source,
+                     size_t nelems);
+int shmem_alltoallsmem(shmem_team_t team, void *dest, const void *source,
+                       ptrdiff_t dst, ptrdiff_t sst, size_t nelems);
+
+/* ---- active-set forms, deprecated in OpenSHMEM 1.5 ---------------------
+ * (reference include/shmem/api.h:3262-3607, :1040-1140).  The active set
+ * is global PEs PE_start + k * 2^logPE_stride, k < PE_size; PE_root of
+ * broadcast is an index into it and the root's target is not written.
+ * These return void, as declared; a failure is reported by
+ * osc_last_error() and leaves target untouched.  pWrk and pSync are not
+ * used. */
+#define OSC_DECL_TO_ALL(_type, _typename, _op)                                 \
+  void shmem_##_typename##_##_op##_to_all(                                     \
+      _type *target, const _type *source, int nreduce, int PE_start,           \
+      int logPE_stride, int PE_size, _type *pWrk, long *pSync);
+/* type tables of the to_all forms (reference api_types.h:164-205) */
+#define OSC_TO_ALL_BITWISE_TYPES(X)                                            \
+  X(short, short) X(int, int) X(long, long) X(long long, longlong)
+#define OSC_TO_ALL_MINMAX_TYPES(X)                                             \
+  OSC_TO_ALL_BITWISE_TYPES(X)                                                  \
+  X(float, float) X(double, double) X(long double, longdouble)
+#define OSC_TO_ALL_ARITH_TYPES(X)                                              \
+  OSC_TO_ALL_MINMAX_TYPES(X)                                                   \
+  X(double _Complex, complexd) X(float _Complex, complexf)
+#define OSC_DECL_AND_TO_ALL(_t, _n) OSC_DECL_TO_ALL(_t, _n, and)
+#define OSC_DECL_OR_TO_ALL(_t, _n) OSC_DECL_TO_ALL(_t, _n, or)
+#define OSC_DECL_XOR_TO_ALL(_t, _n) OSC_DECL_TO_ALL(_t, _n, xor)
+#define OSC_DECL_MIN_TO_ALL(_t, _n) OSC_DECL_TO_ALL(_t, _n, min)
+#define OSC_DECL_MAX_TO_ALL(_t, _n) OSC_DECL_TO_ALL(_t, _n, max)
+#define OSC_DECL_SUM_TO_ALL(_t, _n) OSC_DECL_TO_ALL(_t, _n, sum)
+#define OSC_DECL_PROD_TO_ALL(_t, _n) OSC_DECL_TO_ALL(_t, _n, prod)
+OSC_TO_ALL_BITWISE_TYPES(OSC_DECL_AND_TO_ALL)
+OSC_TO_ALL_BITWISE_TYPES(OSC_DECL_OR_TO_ALL)
+OSC_TO_ALL_BITWISE_TYPES(OSC_DECL_XOR_TO_ALL)
+OSC_TO_ALL_MINMAX_TYPES(OSC_DECL_MIN_TO_ALL)
+OSC_TO_ALL_MINMAX_TYPES(OSC_DECL_MAX_TO_ALL)
+OSC_TO_ALL_ARITH_TYPES(OSC_DECL_SUM_TO_ALL)
+OSC_TO_ALL_ARITH_TYPES(OSC_DECL_PROD_TO_ALL)
+
+#define OSC_DECL_SIZED_COLL(_size)                                             \
+  void shmem_fcollect##_size(void *target, const void *source, size_t nelems, \
+                             int PE_start, int logPE_stride, int PE_size,     \
+                             long *pSync);                                     \
+  void shmem_collect##_size(void *target, const void *source, size_t nelems,  \
+                            int PE_start, int logPE_stride, int PE_size,      \
+                            long *pSync);                                      \
+  void shmem_alltoall##_size(void *target, const void *source, size_t nelems, \
+                             int PE_start, int logPE_stride, int PE_size,     \
+                             long *pSync);                                     \
+  void shmem_alltoalls##_size(void *target, const void *source,               \
+                              ptrdiff_t dst, ptrdiff_t sst, size_t nelems,    \
+                              int PE_start, int logPE_stride, int PE_size,    \
+                              long *pSync);                                    \
+  void shmem_broadcast##_size(void *target, const void *source,               \
+                              size_t nelems, int PE_root, int PE_start,       \
+                              int logPE_stride, int PE_size, long *pSync);
+OSC_DECL_SIZED_COLL(32)
+OSC_DECL_SIZED_COLL(64)
+
+void shmem_barrier(int PE_start, int logPE_stride, int PE_size, long *pSync);
+void shmem_sync_deprecated(int PE_start, int logPE_stride, int PE_size,
+                           long *pSync);
+void shmem_sync_all(void);
+/* shmem_sync(team) or shmem_sync(PE_start, logPE_stride, PE_size, pSync),
+ * chosen by argument count as in the reference (api.h:3711-3729) */
+#define OSC_SYNC_PICK(a, b, c, d, e, ...) e
+#define shmem_sync(...)                                                        \
+  OSC_SYNC_PICK(__VA_ARGS__, shmem_sync_deprecated, shmem_sync_deprecated,     \
+                shmem_sync_deprecated, shmem_team_sync)                        \
+  (__VA_ARGS__)
 
 #ifdef __cplusplus
 }

@@ -10,7 +10,8 @@ and adds independent numpy formulations used to pin it:
   pairwise tree over the folded leaves k^0, k^1, ..., k^(p2s-1) -- written
   without simulating any message, per member;
 * hash3 / gen: the SplitMix64-keyed input generator in numpy;
-* fcollect / alltoall / broadcast closed forms as numpy concatenations.
+* fcollect / alltoall / broadcast / collect closed forms as numpy
+  concatenations, alltoalls as one fancy-index gather.
 """
 from __future__ import annotations
 
@@ -52,6 +53,17 @@ def lib() -> C.CDLL:
         L.orc_broadcast_linear.argtypes = [C.c_int, C.c_int, C.c_size_t, pp,
                                            pp]
         L.orc_rotate_inplace.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t]
+        L.orc_rotate.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t]
+        L.orc_collect_bruck.argtypes = [C.c_int, C.POINTER(C.c_size_t), pp,
+                                        pp]
+        L.orc_alltoalls_shift.argtypes = [C.c_int, C.c_size_t, C.c_size_t,
+                                          C.c_size_t, C.c_size_t, pp, pp]
+        L.orc_broadcast_active_set.argtypes = [C.c_int, C.c_int, C.c_size_t,
+                                               pp, pp]
+        L.orc_strided_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t,
+                                       C.c_ssize_t, C.c_ssize_t, C.c_size_t]
+        L.orc_strided_copy.restype = None
+        L.orc_rotate.restype = None
         L.orc_hash3.restype = C.c_uint64
         L.orc_hash3.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_cpu_bench_reduce.argtypes = [C.c_int, C.c_int, C.c_int,
@@ -139,6 +151,63 @@ def rotate_inplace(buf: np.ndarray, dist: int) -> np.ndarray:
     b = np.ascontiguousarray(buf, dtype=np.uint8).copy()
     lib().orc_rotate_inplace(b.ctypes.data, b.size, dist)
     return b
+
+
+def rotate(buf: np.ndarray, dist: int) -> np.ndarray:
+    b = np.ascontiguousarray(buf, dtype=np.uint8).copy()
+    lib().orc_rotate(b.ctypes.data, b.size, dist)
+    return b
+
+
+def collect(rows) -> np.ndarray:
+    """Bruck collect of per-PE byte rows of any lengths (list of uint8
+    arrays); returns dest[P, total]."""
+    rows = [np.ascontiguousarray(r, dtype=np.uint8) for r in rows]
+    P = len(rows)
+    sizes = (C.c_size_t * P)(*[r.size for r in rows])
+    total = sum(r.size for r in rows)
+    dest = np.zeros((P, max(total, 1)), np.uint8)
+    srcs = [r if r.size else np.zeros(1, np.uint8) for r in rows]
+    lib().orc_collect_bruck(P, sizes, _ptrs(srcs),
+                            _ptrs([dest[i] for i in range(P)]))
+    return dest[:, :total]
+
+
+def alltoalls(src: np.ndarray, es: int, dst: int, sst: int, n: int,
+              dest_bytes: int, dest_init=None) -> np.ndarray:
+    """Shift-exchange strided alltoall of src[P, bytes] (uint8); returns
+    dest[P, dest_bytes] starting from dest_init (zeros if None)."""
+    src = np.ascontiguousarray(src, dtype=np.uint8)
+    P = src.shape[0]
+    dest = (np.zeros((P, dest_bytes), np.uint8) if dest_init is None
+            else np.ascontiguousarray(dest_init, dtype=np.uint8).copy())
+    lib().orc_alltoalls_shift(P, es, dst, sst, n,
+                              _ptrs([src[i] for i in range(P)]),
+                              _ptrs([dest[i] for i in range(P)]))
+    return dest
+
+
+def broadcast_active_set(src: np.ndarray, root: int,
+                         dest_init: np.ndarray) -> np.ndarray:
+    src = np.ascontiguousarray(src, dtype=np.uint8)
+    P, nb = src.shape
+    dest = np.ascontiguousarray(dest_init, dtype=np.uint8).copy()
+    lib().orc_broadcast_active_set(P, root, nb,
+                                   _ptrs([src[i] for i in range(P)]),
+                                   _ptrs([dest[i] for i in range(P)]))
+    return dest
+
+
+def strided_copy(dst: np.ndarray, src: np.ndarray, es: int, tst: int,
+                 sst: int, n: int, dst_base: int = 0,
+                 src_base: int = 0) -> np.ndarray:
+    """Element-wise strided copy between byte buffers; element 0 sits at
+    byte dst_base / src_base (so negative strides stay inside)."""
+    d = np.ascontiguousarray(dst, dtype=np.uint8).copy()
+    s = np.ascontiguousarray(src, dtype=np.uint8)
+    lib().orc_strided_copy(d.ctypes.data + dst_base, s.ctypes.data + src_base,
+                           es, tst, sst, n)
+    return d
 
 
 def cpu_bench_reduce(op, dtype, npes, n, iters, threads) -> float:
@@ -262,3 +331,26 @@ def alltoall_closed_form(src: np.ndarray) -> np.ndarray:
     nb = tot // P
     blocks = src.reshape(P, P, nb)          # [i, j] = src_i block j
     return blocks.transpose(1, 0, 2).reshape(P, tot)  # dest_j block i
+
+
+def collect_closed_form(rows) -> np.ndarray:
+    """dest_j = concatenation of every PE's row, in PE order."""
+    cat = np.concatenate([np.asarray(r, np.uint8) for r in rows])
+    return np.tile(cat, (len(rows), 1))
+
+
+def alltoalls_closed_form(src: np.ndarray, es: int, dst: int, sst: int,
+                          n: int, dest_bytes: int) -> np.ndarray:
+    """dest_j[(i*n + t)*dst] = src_i[(j*n + t)*sst], written as one fancy-
+    index gather over (i, t, byte) per destination j."""
+    P = src.shape[0]
+    dest = np.zeros((P, dest_bytes), np.uint8)
+    i = np.arange(P)[:, None, None]
+    t = np.arange(n)[None, :, None]
+    b = np.arange(es)[None, None, :]
+    for j in range(P):
+        d_idx = ((i * n + t) * dst) * es + b                  # [P, n, es]
+        s_idx = np.broadcast_to(((j * n + t) * sst) * es + b, d_idx.shape)
+        rows = np.broadcast_to(i, d_idx.shape)
+        dest[j, d_idx.ravel()] = src[rows.ravel(), s_idx.ravel()]
+    return dest

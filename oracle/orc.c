@@ -413,6 +413,110 @@ int orc_broadcast_linear(int P, int root, size_t nbytes,
   return 0;
 }
 
+/* util/rotate.c:56-64: right-rotate arr by dist bytes through a copy */
+void orc_rotate(char *arr, size_t size, size_t dist) {
+  char *tmp;
+  if (dist == 0 || size == 0) return;
+  tmp = (char *)malloc(dist);
+  memcpy(tmp, arr + (size - dist), dist);
+  memmove(arr + dist, arr, size - dist);
+  memcpy(arr, tmp, dist);
+  free(tmp);
+}
+
+/* collect.c:406-492 collect_helper_bruck for all PEs, with PE me
+ * contributing nbytes[me].  The exclusive prefix sum (util/scan.c) gives
+ * block_offset; the last PE's offset + size is the total every PE learns
+ * (broadcast_size).  In the round with distance d, PE me learns the
+ * received byte count of recv_from = me + d (block_sizes[round]), clips it
+ * to the total, and gets that prefix of recv_from's dest into its own dest
+ * at its received count.  A round's gets read [0, recv(peer)) of the peer
+ * and write [recv(me), ...) of their own dest, so with the counts taken
+ * before the round they commute.  Finally dest is rotated right by
+ * block_offset (rotate(), not rotate_inplace). */
+int orc_collect_bruck(int P, const size_t *nbytes, const void *const *src,
+                      void *const *dest) {
+  size_t *off = (size_t *)malloc(sizeof(size_t) * (size_t)P);
+  size_t *recv = (size_t *)malloc(sizeof(size_t) * (size_t)P);
+  size_t *take = (size_t *)malloc(sizeof(size_t) * (size_t)P);
+  size_t total = 0, d;
+  int me;
+  if (off == NULL || recv == NULL || take == NULL) {
+    free(off);
+    free(recv);
+    free(take);
+    return -1;
+  }
+  for (me = 0; me < P; ++me) {
+    off[me] = total;
+    total += nbytes[me];
+  }
+  for (me = 0; me < P; ++me) {
+    memcpy(dest[me], src[me], nbytes[me]);
+    recv[me] = nbytes[me];
+  }
+  for (d = 1; d < (size_t)P; d <<= 1) {
+    for (me = 0; me < P; ++me) {
+      const int from = (int)(((size_t)me + d) % (size_t)P);
+      size_t r = recv[from];
+      take[me] = recv[me] + r < total ? r : total - recv[me];
+    }
+    for (me = 0; me < P; ++me) {
+      const int from = (int)(((size_t)me + d) % (size_t)P);
+      memcpy((char *)dest[me] + recv[me], dest[from], take[me]);
+    }
+    for (me = 0; me < P; ++me) recv[me] += take[me];
+  }
+  for (me = 0; me < P; ++me) orc_rotate((char *)dest[me], total, off[me]);
+  free(off);
+  free(recv);
+  free(take);
+  return 0;
+}
+
+/* alltoalls.c:54-90 alltoalls_helper_shift_exchange_barrier for all PEs:
+ * the self block, then for i = 1..P-1 peer = me + i: element t of
+ * src_me at (peer*n + t)*sst goes to dest_peer at (me*n + t)*dst. */
+int orc_alltoalls_shift(int P, size_t es, size_t dst, size_t sst, size_t n,
+                        const void *const *src, void *const *dest) {
+  int me, i;
+  size_t t;
+  for (me = 0; me < P; ++me) {
+    for (t = 0; t < n; ++t)
+      memcpy((char *)dest[me] + (((size_t)me * n + t) * dst) * es,
+             (const char *)src[me] + (((size_t)me * n + t) * sst) * es, es);
+    for (i = 1; i < P; ++i) {
+      const int peer = (me + i) % P;
+      for (t = 0; t < n; ++t)
+        memcpy((char *)dest[peer] + (((size_t)me * n + t) * dst) * es,
+               (const char *)src[me] + (((size_t)peer * n + t) * sst) * es,
+               es);
+    }
+  }
+  return 0;
+}
+
+/* broadcast.c:451-470 sized (active-set) form: no root copy, and the
+ * helpers only write non-roots (binomial :150-190, `me_as != PE_root`). */
+int orc_broadcast_active_set(int P, int root, size_t nbytes,
+                             const void *const *src, void *const *dest) {
+  int me;
+  if (root < 0 || root >= P) return -1;
+  for (me = 0; me < P; ++me)
+    if (me != root) memcpy(dest[me], src[root], nbytes);
+  return 0;
+}
+
+/* putget.h:130-186: strided put/get as one element copy per index, in
+ * index order (later elements win on a repeated target) */
+void orc_strided_copy(char *dst, const char *src, size_t es, ptrdiff_t tst,
+                      ptrdiff_t sst, size_t n) {
+  size_t t;
+  for (t = 0; t < n; ++t)
+    memcpy(dst + (ptrdiff_t)t * tst * (ptrdiff_t)es,
+           src + (ptrdiff_t)t * sst * (ptrdiff_t)es, es);
+}
+
 /* ---- generator (same function as the device fill kernel) ---------------- */
 static uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;

@@ -8,6 +8,11 @@
  *   orc_fcollect_bruck   src/shcoll/src/fcollect.c:370-403 + util/rotate.c:22-54
  *   orc_alltoall_shift   src/shcoll/src/alltoall.c:75-110, SHIFT_PEER :206
  *   orc_broadcast_linear src/shcoll/src/broadcast.c:58-71, :527-531
+ *   orc_broadcast_active_set  broadcast.c:451-470 (+ binomial :150-190)
+ *   orc_collect_bruck    src/shcoll/src/collect.c:406-492 + util/scan.c:10-60
+ *                        + util/rotate.c:56-64
+ *   orc_alltoalls_shift  src/shcoll/src/alltoalls.c:54-90
+ *   orc_strided_copy     src/api/rma/putget.h:130-186
  * plus high-precision references for float SUM/PROD:
  *   orc_reduce_accurate  exact (Shewchuk expansion) sums, double-double
  *                        products; result as an unevaluated hi + lo pair
@@ -49,6 +54,21 @@ int orc_broadcast_linear(int npes, int root, size_t nbytes,
                          const void *const *src, void *const *dest);
 /* util/rotate.c:22-54 byte juggling rotation, exposed for its own test */
 void orc_rotate_inplace(char *arr, size_t size, size_t dist);
+/* util/rotate.c:56-64 rotation through a copy (used by collect) */
+void orc_rotate(char *arr, size_t size, size_t dist);
+/* collect with per-PE sizes nbytes[i] (Bruck + prefix sum + rotate) */
+int orc_collect_bruck(int npes, const size_t *nbytes, const void *const *src,
+                      void *const *dest);
+/* strided alltoall, shift exchange; dst/sst in elements of es bytes */
+int orc_alltoalls_shift(int npes, size_t es, size_t dst, size_t sst,
+                        size_t nelems, const void *const *src,
+                        void *const *dest);
+/* active-set broadcast: every PE but the root receives the root's source */
+int orc_broadcast_active_set(int npes, int root, size_t nbytes,
+                             const void *const *src, void *const *dest);
+/* strided copy of n elements of es bytes, strides in elements */
+void orc_strided_copy(char *dst, const char *src, size_t es, ptrdiff_t tst,
+                      ptrdiff_t sst, size_t n);
 
 /* SplitMix64-based input generator shared with the device fill kernel:
  * integer distributions are bit-identical to it. */

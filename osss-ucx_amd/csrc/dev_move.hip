@@ -9,10 +9,19 @@
 //  * alltoall, default shift_exchange_barrier (src/shcoll/src/alltoall.c:
 //    75-110): dest_j[i*nbytes ..) = src_i[j*nbytes ..);
 //  * broadcast (src/shcoll/src/broadcast.c:511-553): dest_j = src_root for
-//    every member j, the root included.
-// Each source byte is loaded once into registers and stored to every
-// destination it belongs to; unit width W (16 bytes when alignment allows)
-// is picked per call from the alignment of sizes, offsets and slice.
+//    every member j, the root included; the active-set form
+//    (broadcast.c:451-470, binomial helper :150-190) skips the root;
+//  * collect, default bruck (src/shcoll/src/collect.c:406-492, with the
+//    prefix sum of src/shcoll/src/util/scan.c:10-60 and the final rotate):
+//    dest_j[off_i .. off_i + n_i) = src_i, off_i = sum of n_k for k < i;
+//  * alltoalls, default shift_exchange_barrier (src/shcoll/src/
+//    alltoalls.c:54-90): dest_j[(i*n + t)*dst] = src_i[(j*n + t)*sst];
+//  * strided put/get (src/api/rma/putget.h:130-186): element-wise copy.
+// fcollect, broadcast and collect run destination-major, so the
+// workgroups in flight read the same source chunk (served by the caches
+// after its first read) and each writes one contiguous run.  Unit width W
+// (16 bytes when alignment allows) is picked per call from the alignment
+// of sizes, offsets and slice.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -24,26 +33,132 @@ namespace osc {
 
 constexpr int kMoveBlock = 256;
 
-// Fan-out: one source member's [soff, soff+nbytes) to every member's
-// [doff_base + blockIdx.y*doff_step, ...).  blockIdx.y selects the source
-// member when src_member < 0 (fcollect), otherwise src_member is used
-// (broadcast, gridDim.y == 1).
-template <int W>
-__global__ __launch_bounds__(kMoveBlock) void k_fanout(
+// Gather, destination-major: member j's [doff + i*doff_step, +nbytes) =
+// source member i's [soff, +nbytes), for every source i (fcollect,
+// src_member < 0) or for i = src_member only (broadcast, doff_step = 0).
+// Virtual block vb -> destination j = vb % P (fastest), then source i and
+// chunk: the workgroups in flight all read the same few source chunks,
+// which the caches serve after the first read, and each workgroup writes
+// one contiguous run of 256*UPT units.  Member `skip` is left unwritten
+// (the legacy broadcast leaves the root's destination alone); -1 for none.
+template <int W, int UPT>
+__global__ __launch_bounds__(kMoveBlock) void k_gather(
     const oscd_geom g, const int src_member, const uint64_t soff,
-    const uint64_t doff_base, const uint64_t doff_step, const uint64_t nbytes) {
-  const int i = src_member >= 0 ? src_member : (int)blockIdx.y;
+    const uint64_t doff, const uint64_t doff_step, const uint64_t nbytes,
+    const uint64_t cps, const int skip, const uint64_t total) {
   const uint64_t nunits = nbytes / W;
-  const char *src = member_ptr(g, i, soff);
-  char *d0 = member_ptr(g, 0, doff_base + (uint64_t)blockIdx.y * doff_step);
-  const int64_t step = (int64_t)g.stride * (int64_t)g.slice;
-  const int P = g.npes;
-  for (uint64_t u = (uint64_t)blockIdx.x * kMoveBlock + threadIdx.x;
-       u < nunits; u += (uint64_t)gridDim.x * kMoveBlock) {
-    const typename Raw<W>::type v = ld<W>(src + u * W);
-    char *d = d0 + u * W;
-    for (int j = 0; j < P; ++j) st_nt<W>(d + j * step, v);
+  const uint64_t P = (uint64_t)g.npes;
+  for (uint64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const int j = (int)(vb % P);
+    if (j == skip) continue;
+    const uint64_t r = vb / P;
+    const int i = src_member >= 0 ? src_member : (int)(r / cps);
+    const uint64_t cc = src_member >= 0 ? r : r - (uint64_t)i * cps;
+    const char *src = member_ptr(g, i, soff);
+    char *dst = member_ptr(g, j, doff + (uint64_t)i * doff_step);
+    const uint64_t u0 = cc * (uint64_t)(kMoveBlock * UPT) + threadIdx.x;
+    typename Raw<W>::type v[UPT];
+#pragma unroll
+    for (int k = 0; k < UPT; ++k) {
+      const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
+      if (u < nunits) v[k] = ld<W>(src + u * W);
+    }
+#pragma unroll
+    for (int k = 0; k < UPT; ++k) {
+      const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
+      if (u < nunits) st_nt<W>(dst + u * W, v[k]);
+    }
   }
+}
+
+// Collect with per-member sizes: member j's [doff + off_i, + n_i) =
+// member i's [soff, + n_i); tab[2i] = off_i, tab[2i+1] = n_i (bytes).
+// Same virtual-block order as k_gather, with cps chunks per source sized
+// for the largest member; chunks past a smaller member's end exit at once.
+template <int W, int UPT>
+__global__ __launch_bounds__(kMoveBlock) void k_collect(
+    const oscd_geom g, const uint64_t soff, const uint64_t doff,
+    const uint64_t *__restrict__ tab, const uint64_t cps,
+    const uint64_t total) {
+  const uint64_t P = (uint64_t)g.npes;
+  for (uint64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const int j = (int)(vb % P);
+    const uint64_t r = vb / P;
+    const int i = (int)(r / cps);
+    const uint64_t cc = r - (uint64_t)i * cps;
+    const uint64_t nunits = tab[2 * i + 1] / W;
+    const uint64_t c0 = cc * (uint64_t)(kMoveBlock * UPT);
+    if (c0 >= nunits) continue;
+    const char *src = member_ptr(g, i, soff);
+    char *dst = member_ptr(g, j, doff + tab[2 * i]);
+    const uint64_t u0 = c0 + threadIdx.x;
+    typename Raw<W>::type v[UPT];
+#pragma unroll
+    for (int k = 0; k < UPT; ++k) {
+      const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
+      if (u < nunits) v[k] = ld<W>(src + u * W);
+    }
+#pragma unroll
+    for (int k = 0; k < UPT; ++k) {
+      const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
+      if (u < nunits) st_nt<W>(dst + u * W, v[k]);
+    }
+  }
+}
+
+// Strided alltoall: for member pair (i source, j destination) and element
+// t < n, es bytes (upe units of W) move from src_i at element
+// (j*n + t)*sst to dest_j at element (i*n + t)*dst.  Virtual block vb ->
+// pair vb / cpp (pair = i*P + j), chunk vb % cpp of the pair's n*upe units.
+template <int W, bool ONE>
+__global__ __launch_bounds__(kMoveBlock) void k_a2as(
+    const oscd_geom g, const uint64_t doff, const uint64_t soff,
+    const uint64_t es, const uint64_t dst, const uint64_t sst,
+    const uint64_t n, const uint64_t upe, const uint64_t cpp,
+    const uint64_t total) {
+  const uint64_t P = (uint64_t)g.npes;
+  const uint64_t units = n * upe;
+  for (uint64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const uint64_t pair = vb / cpp;
+    const uint64_t chunk = vb - pair * cpp;
+    const int i = (int)(pair / P);
+    const int j = (int)(pair - (uint64_t)i * P);
+    const uint64_t u = chunk * kMoveBlock + threadIdx.x;
+    if (u >= units) continue;
+    const uint64_t t = ONE ? u : u / upe;
+    const uint64_t w = ONE ? 0 : u - t * upe;
+    const char *s = member_ptr(g, i, soff) +
+                    (((uint64_t)j * n + t) * sst) * es + w * W;
+    char *d = member_ptr(g, j, doff) + (((uint64_t)i * n + t) * dst) * es +
+              w * W;
+    st<W>(d, ld<W>(s));
+  }
+}
+
+// Strided copy of n elements of upe units: dst + t*dstep <- src + t*sstep.
+template <int W>
+__global__ __launch_bounds__(kMoveBlock) void k_icopy(
+    char *dst, const char *src, const uint64_t dstep, const uint64_t sstep,
+    const uint64_t n, const uint64_t upe) {
+  const uint64_t units = n * upe;
+  const uint64_t gs = (uint64_t)gridDim.x * kMoveBlock;
+  for (uint64_t u = (uint64_t)blockIdx.x * kMoveBlock + threadIdx.x;
+       u < units; u += gs) {
+    const uint64_t t = u / upe;
+    const uint64_t w = u - t * upe;
+    st<W>(dst + t * dstep + w * W, ld<W>(src + t * sstep + w * W));
+  }
+}
+
+// One 64-bit atomic signal update (set or add) by a single lane.
+__global__ void k_signal(uint64_t *sig, const uint64_t value, const int op) {
+  if (threadIdx.x != 0) return;
+  if (op == 0)
+    __hip_atomic_store(sig, value, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+  else
+    __hip_atomic_fetch_add(sig, value, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Alltoall, large blocks: block b covers chunk (b % cpb) of the pair
@@ -195,16 +310,29 @@ static int check_launch() {
     default: { constexpr int WW = 1; CALL; } break;                            \
   }
 
-static int fanout(const oscd_geom *g, int src_member, uint64_t soff,
-                  uint64_t doff_base, uint64_t doff_step, uint64_t nbytes,
-                  unsigned ny, hipStream_t s) {
-  const int W = alignment_of(g, doff_base | doff_step, soff, nbytes);
+// nsrc = number of source members gathered (P for fcollect, 1 for
+// broadcast); src_member < 0 selects fcollect.
+static int gather(const oscd_geom *g, int src_member, uint64_t soff,
+                  uint64_t doff, uint64_t doff_step, uint64_t nbytes,
+                  uint64_t nsrc, int skip, hipStream_t s) {
+  const int W = alignment_of(g, doff | doff_step, soff, nbytes);
   const uint64_t nunits = nbytes / W;
-  // enough workgroups in x to fill the chip when ny is small
-  const unsigned gx = grid_for(nunits, ny >= 64 ? 4096 : 65535);
-  OSC_W_DISPATCH(W, hipLaunchKernelGGL((k_fanout<WW>), dim3(gx, ny),
-                                       dim3(kMoveBlock), 0, s, *g, src_member,
-                                       soff, doff_base, doff_step, nbytes));
+  const bool wide = nunits >= 4 * kMoveBlock;
+  const uint64_t per = wide ? 4 * kMoveBlock : kMoveBlock;
+  const uint64_t cps = (nunits + per - 1) / per;
+  const uint64_t total = (uint64_t)g->npes * nsrc * cps;
+  const unsigned grid = (unsigned)(total < (1u << 22) ? total : (1u << 22));
+  if (wide) {
+    OSC_W_DISPATCH(W, hipLaunchKernelGGL((k_gather<WW, 4>), dim3(grid),
+                                         dim3(kMoveBlock), 0, s, *g,
+                                         src_member, soff, doff, doff_step,
+                                         nbytes, cps, skip, total));
+  } else {
+    OSC_W_DISPATCH(W, hipLaunchKernelGGL((k_gather<WW, 1>), dim3(grid),
+                                         dim3(kMoveBlock), 0, s, *g,
+                                         src_member, soff, doff, doff_step,
+                                         nbytes, cps, skip, total));
+  }
   return check_launch();
 }
 
@@ -215,18 +343,108 @@ extern "C" int oscd_fcollect(const oscd_geom *g, uint64_t dest_off,
   using namespace osc;
   if (g == nullptr || g->npes <= 0 || g->npes > 65535) return OSC_ERR_ARG;
   if (nbytes == 0) return OSC_OK;
-  return fanout(g, -1, src_off, dest_off, nbytes, nbytes, (unsigned)g->npes,
-                (hipStream_t)stream);
+  return gather(g, -1, src_off, dest_off, nbytes, nbytes, (uint64_t)g->npes,
+                -1, (hipStream_t)stream);
 }
 
 extern "C" int oscd_broadcast(const oscd_geom *g, int root, uint64_t dest_off,
-                              uint64_t src_off, uint64_t nbytes, void *stream) {
+                              uint64_t src_off, uint64_t nbytes, int skip_root,
+                              void *stream) {
   using namespace osc;
   if (g == nullptr || g->npes <= 0 || root < 0 || root >= g->npes)
     return OSC_ERR_ARG;
   if (nbytes == 0) return OSC_OK;
-  return fanout(g, root, src_off, dest_off, 0, nbytes, 1u,
-                (hipStream_t)stream);
+  return gather(g, root, src_off, dest_off, 0, nbytes, 1u,
+                skip_root ? root : -1, (hipStream_t)stream);
+}
+
+extern "C" int oscd_collect(const oscd_geom *g, uint64_t dest_off,
+                            uint64_t src_off, const uint64_t *d_tab,
+                            uint64_t max_n, int unit_width, void *stream) {
+  using namespace osc;
+  if (g == nullptr || g->npes <= 0 || g->npes > 65535 || d_tab == nullptr)
+    return OSC_ERR_ARG;
+  if (unit_width != 1 && unit_width != 2 && unit_width != 4 &&
+      unit_width != 8 && unit_width != 16)
+    return OSC_ERR_ARG;
+  if (max_n == 0) return OSC_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int W = unit_width;
+  const uint64_t nunits = max_n / W;
+  const bool wide = nunits >= 4 * kMoveBlock;
+  const uint64_t per = wide ? 4 * kMoveBlock : kMoveBlock;
+  const uint64_t cps = (nunits + per - 1) / per;
+  const uint64_t P = (uint64_t)g->npes;
+  const uint64_t total = P * P * cps;
+  const unsigned grid = (unsigned)(total < (1u << 22) ? total : (1u << 22));
+  if (wide) {
+    OSC_W_DISPATCH(W, hipLaunchKernelGGL((k_collect<WW, 4>), dim3(grid),
+                                         dim3(kMoveBlock), 0, s, *g, src_off,
+                                         dest_off, d_tab, cps, total));
+  } else {
+    OSC_W_DISPATCH(W, hipLaunchKernelGGL((k_collect<WW, 1>), dim3(grid),
+                                         dim3(kMoveBlock), 0, s, *g, src_off,
+                                         dest_off, d_tab, cps, total));
+  }
+  return check_launch();
+}
+
+extern "C" int oscd_alltoalls(const oscd_geom *g, uint64_t dest_off,
+                              uint64_t src_off, uint64_t es, uint64_t dst,
+                              uint64_t sst, uint64_t nelems, void *stream) {
+  using namespace osc;
+  if (g == nullptr || g->npes <= 0 || g->npes > 65535 || es == 0 ||
+      dst == 0 || sst == 0)
+    return OSC_ERR_ARG;
+  if (nelems == 0) return OSC_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int W = alignment_of(g, dest_off, src_off, es);
+  const uint64_t upe = es / W;
+  const uint64_t P = (uint64_t)g->npes;
+  const uint64_t cpp = (nelems * upe + kMoveBlock - 1) / kMoveBlock;
+  const uint64_t total = P * P * cpp;
+  const unsigned grid = (unsigned)(total < (1u << 22) ? total : (1u << 22));
+  if (upe == 1) {
+    OSC_W_DISPATCH(W, hipLaunchKernelGGL((k_a2as<WW, true>), dim3(grid),
+                                         dim3(kMoveBlock), 0, s, *g, dest_off,
+                                         src_off, es, dst, sst, nelems, upe,
+                                         cpp, total));
+  } else {
+    OSC_W_DISPATCH(W, hipLaunchKernelGGL((k_a2as<WW, false>), dim3(grid),
+                                         dim3(kMoveBlock), 0, s, *g, dest_off,
+                                         src_off, es, dst, sst, nelems, upe,
+                                         cpp, total));
+  }
+  return check_launch();
+}
+
+extern "C" int oscd_copy_strided(void *dst, const void *src, uint64_t es,
+                                 uint64_t dst_step, uint64_t src_step,
+                                 uint64_t n, void *stream) {
+  using namespace osc;
+  if (es == 0) return OSC_ERR_ARG;
+  if (n == 0) return OSC_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int W = unit_width((uint64_t)(uintptr_t)dst |
+                           (uint64_t)(uintptr_t)src | es | dst_step |
+                           src_step);
+  const uint64_t upe = es / W;
+  const unsigned grid = grid_for(n * upe, 1u << 16);
+  OSC_W_DISPATCH(W, hipLaunchKernelGGL((k_icopy<WW>), dim3(grid),
+                                       dim3(kMoveBlock), 0, s, (char *)dst,
+                                       (const char *)src, dst_step, src_step,
+                                       n, upe));
+  return check_launch();
+}
+
+extern "C" int oscd_signal(uint64_t *sig, uint64_t value, int op,
+                           void *stream) {
+  using namespace osc;
+  if (sig == nullptr || ((uintptr_t)sig & 7u) != 0 || (op != 0 && op != 1))
+    return OSC_ERR_ARG;
+  hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, (hipStream_t)stream, sig,
+                     value, op);
+  return check_launch();
 }
 
 extern "C" int oscd_alltoall(const oscd_geom *g, uint64_t dest_off,

@@ -102,8 +102,8 @@ int osc_alltoall(void *team, void *dest, const void *src, size_t nbytes,
   return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "alltoall launch");
 }
 
-int osc_broadcast(void *team, void *dest, const void *src, size_t nbytes,
-                  int root, void *stream) {
+int osc_broadcast_impl(void *team, void *dest, const void *src, size_t nbytes,
+                       int root, int skip_root, void *stream) {
   const osc_team *t = osc_team_get(team);
   size_t doff, soff;
   oscd_geom g;
@@ -120,6 +120,136 @@ int osc_broadcast(void *team, void *dest, const void *src, size_t nbytes,
   log_call("broadcast", t, -1, -1, nbytes);
   if (nbytes == 0) return OSC_OK;
   osc_team_geom(t, &g);
-  rc = oscd_broadcast(&g, root, doff, soff, nbytes, stream ? stream : osc_g.stream);
+  rc = oscd_broadcast(&g, root, doff, soff, nbytes, skip_root,
+                      stream ? stream : osc_g.stream);
   return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "broadcast launch");
+}
+
+int osc_broadcast(void *team, void *dest, const void *src, size_t nbytes,
+                  int root, void *stream) {
+  return osc_broadcast_impl(team, dest, src, nbytes, root, 0, stream);
+}
+
+/* Variable-size collect (collect.c:406-492 closed form): the per-member
+ * byte counts are known to the one host caller, so the exclusive prefix
+ * sum the reference computes with a scan over the PEs (util/scan.c:10-60)
+ * is a host loop here, and the (offset, size) table goes to the device
+ * through a pinned staging buffer on the call's stream.  The staging is
+ * reused only after the previous collect's kernel has consumed it. */
+int osc_collect(void *team, void *dest, const void *src, const size_t *nbytes,
+                void *stream) {
+  const osc_team *t = osc_team_get(team);
+  size_t doff, soff, total = 0, maxn = 0, m;
+  uint64_t align;
+  oscd_geom g;
+  void *s = stream ? stream : osc_g.stream;
+  int i, rc, W;
+  if (!osc_g.initialized) return osc_set_error(OSC_ERR_NOTINIT, "not initialised");
+  if (t == NULL) return osc_set_error(OSC_ERR_TEAM, "invalid team");
+  if (nbytes == NULL) return osc_set_error(OSC_ERR_ARG, "NULL size table");
+  for (i = 0; i < t->npes; ++i) {
+    if (nbytes[i] > ((size_t)-1) - total)
+      return osc_set_error(OSC_ERR_ARG, "size overflow");
+    total += nbytes[i];
+    if (nbytes[i] > maxn) maxn = nbytes[i];
+  }
+  rc = osc_sym_range(dest, total, &doff);
+  if (rc != OSC_OK) return rc;
+  rc = osc_sym_range(src, maxn, &soff);
+  if (rc != OSC_OK) return rc;
+  if (total > 0 && maxn > 0 && overlap(doff, total, soff, maxn))
+    return osc_set_error(OSC_ERR_ARG, "dest and source overlap");
+  log_call("collect", t, -1, -1, total);
+  if (maxn == 0) return OSC_OK;
+  if (osc_g.tab_host == NULL) {
+    const size_t tb = (size_t)2 * OSC_MAX_NPES * sizeof(uint64_t);
+    rc = oscd_host_alloc((void **)&osc_g.tab_host, tb);
+    if (rc == OSC_OK) rc = oscd_malloc((void **)&osc_g.tab_dev, tb);
+    if (rc == OSC_OK) rc = oscd_event_create(&osc_g.tab_event);
+    if (rc != OSC_OK) return osc_hip_error(rc, "collect table");
+  }
+  if (osc_g.tab_used) {
+    rc = oscd_event_sync(osc_g.tab_event);
+    if (rc != OSC_OK) return osc_hip_error(rc, "collect table wait");
+  }
+  align = (uint64_t)doff | (uint64_t)soff | (uint64_t)osc_g.slice |
+          (uint64_t)(uintptr_t)osc_g.base;
+  for (i = 0, m = 0; i < t->npes; ++i) {
+    osc_g.tab_host[2 * i] = m;
+    osc_g.tab_host[2 * i + 1] = nbytes[i];
+    align |= (uint64_t)m | (uint64_t)nbytes[i];
+    m += nbytes[i];
+  }
+  for (W = 16; W > 1 && (align % (uint64_t)W) != 0; W >>= 1) {
+  }
+  rc = oscd_memcpy_h2d_async(osc_g.tab_dev, osc_g.tab_host,
+                             (size_t)2 * t->npes * sizeof(uint64_t), s);
+  if (rc != OSC_OK) return osc_hip_error(rc, "collect table copy");
+  osc_team_geom(t, &g);
+  rc = oscd_collect(&g, doff, soff, osc_g.tab_dev, maxn, W, s);
+  if (rc != OSC_OK) return osc_hip_error(rc, "collect launch");
+  rc = oscd_event_record(osc_g.tab_event, s);
+  if (rc != OSC_OK) return osc_hip_error(rc, "collect table event");
+  osc_g.tab_used = 1;
+  return OSC_OK;
+}
+
+/* Strided alltoall (alltoalls.c:54-90 closed form); dst/sst in elements
+ * of elem_bytes, both >= 1 as the reference's checks require
+ * (alltoalls.c:316-318). */
+int osc_alltoalls(void *team, void *dest, const void *src, ptrdiff_t dst,
+                  ptrdiff_t sst, size_t elem_bytes, size_t nelems,
+                  void *stream) {
+  const osc_team *t = osc_team_get(team);
+  size_t doff, soff, need_d, need_s, span;
+  oscd_geom g;
+  int rc;
+  if (!osc_g.initialized) return osc_set_error(OSC_ERR_NOTINIT, "not initialised");
+  if (t == NULL) return osc_set_error(OSC_ERR_TEAM, "invalid team");
+  if (dst < 1 || sst < 1 || elem_bytes == 0)
+    return osc_set_error(OSC_ERR_ARG, "strides %td/%td, element %zu bytes",
+                         dst, sst, elem_bytes);
+  if (nelems == 0) return OSC_OK;
+  /* the last element touched is ((P*n - 1)*stride)*es + es */
+  span = (size_t)t->npes * nelems;
+  if (nelems > ((size_t)-1) / (size_t)t->npes ||
+      span - 1 > (((size_t)-1) - elem_bytes) / elem_bytes / (size_t)dst ||
+      span - 1 > (((size_t)-1) - elem_bytes) / elem_bytes / (size_t)sst)
+    return osc_set_error(OSC_ERR_ARG, "size overflow");
+  need_d = (span - 1) * (size_t)dst * elem_bytes + elem_bytes;
+  need_s = (span - 1) * (size_t)sst * elem_bytes + elem_bytes;
+  rc = osc_sym_range(dest, need_d, &doff);
+  if (rc != OSC_OK) return rc;
+  rc = osc_sym_range(src, need_s, &soff);
+  if (rc != OSC_OK) return rc;
+  if (overlap(doff, need_d, soff, need_s))
+    return osc_set_error(OSC_ERR_ARG, "dest and source overlap");
+  log_call("alltoalls", t, -1, -1, nelems);
+  osc_team_geom(t, &g);
+  rc = oscd_alltoalls(&g, doff, soff, elem_bytes, (uint64_t)dst,
+                      (uint64_t)sst, nelems, stream ? stream : osc_g.stream);
+  return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "alltoalls launch");
+}
+
+/* Active set checks of the reference's sized wrappers
+ * (SHMEMU_CHECK_ACTIVE_SET_RANGE, e.g. reduce.c:936-940). */
+int osc_active_set(int PE_start, int logPE_stride, int PE_size, osc_team *t) {
+  long stride, last;
+  if (!osc_g.initialized) return osc_set_error(OSC_ERR_NOTINIT, "not initialised");
+  if (PE_size < 1 || PE_start < 0 || logPE_stride < 0 || logPE_stride > 30)
+    return osc_set_error(OSC_ERR_TEAM, "active set (%d, %d, %d)", PE_start,
+                         logPE_stride, PE_size);
+  stride = 1L << logPE_stride;
+  last = (long)PE_start + (long)(PE_size - 1) * stride;
+  if (last >= osc_g.npes)
+    return osc_set_error(OSC_ERR_TEAM, "active set (%d, %d, %d) past PE %d",
+                         PE_start, logPE_stride, PE_size, osc_g.npes - 1);
+  t->magic = OSC_TEAM_MAGIC;
+  t->start = PE_start;
+  t->stride = (int)stride;
+  t->npes = PE_size;
+  t->predefined = 0;
+  t->num_contexts = 0;
+  t->parent = NULL;
+  return OSC_OK;
 }

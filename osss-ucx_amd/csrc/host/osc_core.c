@@ -17,10 +17,12 @@
 
 osc_state osc_g = {0};
 static __thread char g_errmsg[512];
+static __thread int g_errcode = 0;
 static __thread int g_my_pe = 0;
 
 int osc_set_error(int code, const char *fmt, ...) {
   va_list ap;
+  g_errcode = code;
   va_start(ap, fmt);
   vsnprintf(g_errmsg, sizeof(g_errmsg), fmt, ap);
   va_end(ap);
@@ -33,6 +35,11 @@ int osc_hip_error(int code, const char *what) {
 }
 
 const char *osc_last_error(void) { return g_errmsg; }
+int osc_last_error_code(void) { return g_errcode; }
+void osc_clear_error(void) {
+  g_errcode = OSC_OK;
+  g_errmsg[0] = '\0';
+}
 
 const char *osc_strerror(int code) {
   switch (code) {
@@ -131,6 +138,9 @@ int osc_finalize(void) {
   oscd_stream_sync(osc_g.stream);
   oscd_stream_destroy(osc_g.stream);
   if (osc_g.aux_dev) oscd_free(osc_g.aux_dev);
+  if (osc_g.tab_event) oscd_event_destroy(osc_g.tab_event);
+  if (osc_g.tab_dev) oscd_free(osc_g.tab_dev);
+  if (osc_g.tab_host) oscd_host_free(osc_g.tab_host);
   oscd_free(osc_g.base);
   osc_heap_fini();
   pthread_mutex_destroy(&osc_g.lock);

@@ -48,6 +48,12 @@ typedef struct osc_state {
   int fp_default;
   int reduce_shape; /* -1 = per-policy default; OSC_REDUCE_SHAPE overrides */
   void *aux_dev;    /* 256 device bytes for single-call descriptors */
+  /* per-member (offset, size) table of the variable-size collect:
+   * pinned staging, its device copy, and the event of its last use */
+  uint64_t *tab_host;
+  uint64_t *tab_dev;
+  void *tab_event;
+  int tab_used;
 } osc_state;
 
 extern osc_state osc_g;
@@ -85,5 +91,13 @@ int osc_tune_bucket(uint64_t bytes_per_pe);
 
 /* symmetric-argument validation (osc_coll.c) */
 int osc_sym_range(const void *p, size_t bytes, size_t *off);
+
+/* Active set (PE_start, 2^logPE_stride, PE_size) of the pre-1.5 API as a
+ * transient team (not registered; valid while `t` lives). */
+int osc_active_set(int PE_start, int logPE_stride, int PE_size, osc_team *t);
+/* broadcast with the root's dest left alone when skip_root (active-set
+ * form) */
+int osc_broadcast_impl(void *team, void *dest, const void *src, size_t nbytes,
+                       int root, int skip_root, void *stream);
 
 #endif

@@ -357,5 +357,6 @@ int osc_call_launch_single(const osc_call *c, void *stream) {
     case OSC_CALL_ALLTOALL:
       return oscd_alltoall(&g, c->dest_off, c->src_off, c->count, stream);
   }
-  return oscd_broadcast(&g, c->root, c->dest_off, c->src_off, c->count, stream);
+  return oscd_broadcast(&g, c->root, c->dest_off, c->src_off, c->count, 0,
+                        stream);
 }

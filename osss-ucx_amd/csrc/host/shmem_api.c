@@ -9,6 +9,7 @@
  * the library stream is synchronised before returning, so dest is final
  * on every member when the call returns, as in the reference.
  */
+#include <sched.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -182,7 +183,9 @@ int shmem_addr_accessible(const void *addr, int pe) {
 }
 
 /* ---- RMA between emulated PEs ------------------------------------------------ */
-/* The local side may be a symmetric (device) address or host memory. */
+/* The local side may be a symmetric (device) address or host memory; the
+ * remote side is the symmetric object on PE pe (reference putget.h:403-600:
+ * typed and sized forms are byte counts over the same copy). */
 
 static int is_sym(const void *p) {
   const char *c = (const char *)p;
@@ -190,36 +193,288 @@ static int is_sym(const void *p) {
          c < osc_g.base + (size_t)osc_g.npes * osc_g.slice;
 }
 
-static void put_get(void *dst, const void *src, size_t nbytes, int blocking) {
+/* device address of [sym + lo, sym + lo + bytes) on PE pe, range-checked */
+static char *remote(const void *sym, ptrdiff_t lo, size_t bytes, int pe) {
+  size_t off;
+  if (!osc_g.initialized) {
+    osc_set_error(OSC_ERR_NOTINIT, "not initialised");
+    return NULL;
+  }
+  if (pe < 0 || pe >= osc_g.npes) {
+    osc_set_error(OSC_ERR_ARG, "PE %d outside [0, %d)", pe, osc_g.npes);
+    return NULL;
+  }
+  if (osc_sym_range((const char *)sym + lo, bytes, &off) != OSC_OK) return NULL;
+  return osc_g.base + (size_t)pe * osc_g.slice + off - lo;
+}
+
+static void xfer(void *dst, const void *src, size_t nbytes, int blocking) {
+  int rc;
   if (nbytes == 0) return;
   if (is_sym(dst) && is_sym(src)) {
-    oscd_copy(dst, src, nbytes, osc_g.stream);
-    if (blocking) oscd_stream_sync(osc_g.stream);
+    rc = oscd_copy(dst, src, nbytes, osc_g.stream);
+    if (rc == OSC_OK && blocking) rc = oscd_stream_sync(osc_g.stream);
   } else if (is_sym(dst)) {
-    oscd_stream_sync(osc_g.stream);
-    oscd_memcpy_h2d(dst, src, nbytes);
+    rc = oscd_stream_sync(osc_g.stream);
+    if (rc == OSC_OK) rc = oscd_memcpy_h2d(dst, src, nbytes);
   } else {
-    oscd_stream_sync(osc_g.stream);
-    oscd_memcpy_d2h(dst, src, nbytes);
+    rc = oscd_stream_sync(osc_g.stream);
+    if (rc == OSC_OK) rc = oscd_memcpy_d2h(dst, src, nbytes);
   }
+  if (rc != OSC_OK) osc_hip_error(rc, "RMA copy");
 }
 
 void shmem_putmem(void *dest, const void *source, size_t nbytes, int pe) {
-  void *d = osc_pe_ptr(dest, pe);
-  if (d != NULL) put_get(d, source, nbytes, 1);
+  void *d = remote(dest, 0, nbytes, pe);
+  if (d != NULL) xfer(d, source, nbytes, 1);
 }
 void shmem_putmem_nbi(void *dest, const void *source, size_t nbytes, int pe) {
-  void *d = osc_pe_ptr(dest, pe);
-  if (d != NULL) put_get(d, source, nbytes, 0);
+  void *d = remote(dest, 0, nbytes, pe);
+  if (d != NULL) xfer(d, source, nbytes, 0);
 }
 void shmem_getmem(void *dest, const void *source, size_t nbytes, int pe) {
-  const void *s = osc_pe_ptr(source, pe);
-  if (s != NULL) put_get(dest, s, nbytes, 1);
+  const void *s = remote(source, 0, nbytes, pe);
+  if (s != NULL) xfer(dest, s, nbytes, 1);
 }
 void shmem_getmem_nbi(void *dest, const void *source, size_t nbytes, int pe) {
-  const void *s = osc_pe_ptr(source, pe);
-  if (s != NULL) put_get(dest, s, nbytes, 0);
+  const void *s = remote(source, 0, nbytes, pe);
+  if (s != NULL) xfer(dest, s, nbytes, 0);
 }
+
+/* Strided put (is_put) or get of n elements of es bytes; strides in
+ * elements, any sign.  Sequential semantics (putget.h:130-186: one put per
+ * element in index order): with a zero stride on the written side only
+ * the last element survives, so only it is moved.  A host-side array is
+ * packed into (or unpacked from) a contiguous device staging buffer. */
+static void strided(void *dest, const void *src, size_t es, ptrdiff_t tst,
+                    ptrdiff_t sst, size_t n, int pe, int is_put) {
+  const ptrdiff_t rst = is_put ? tst : sst; /* remote stride */
+  const ptrdiff_t lst = is_put ? sst : tst; /* local stride  */
+  void *rsym = is_put ? dest : (void *)src;
+  void *lp = is_put ? (void *)src : dest;
+  ptrdiff_t lo, hi;
+  char *r, *stage = NULL;
+  int rc = OSC_OK;
+  if (n == 0) return;
+  if (tst == 0) { /* every element lands on one target: the last wins */
+    if (is_put)
+      lp = (char *)lp + (ptrdiff_t)(n - 1) * lst * (ptrdiff_t)es;
+    else
+      rsym = (char *)rsym + (ptrdiff_t)(n - 1) * rst * (ptrdiff_t)es;
+    n = 1;
+  }
+  lo = rst < 0 ? (ptrdiff_t)(n - 1) * rst * (ptrdiff_t)es : 0;
+  hi = (rst > 0 ? (ptrdiff_t)(n - 1) * rst * (ptrdiff_t)es : 0) + (ptrdiff_t)es;
+  r = remote(rsym, lo, (size_t)(hi - lo), pe);
+  if (r == NULL) return;
+  if (is_sym(lp)) {
+    rc = is_put ? oscd_copy_strided(r, lp, es, (uint64_t)(rst * (ptrdiff_t)es),
+                                    (uint64_t)(lst * (ptrdiff_t)es), n,
+                                    osc_g.stream)
+                : oscd_copy_strided(lp, r, es, (uint64_t)(lst * (ptrdiff_t)es),
+                                    (uint64_t)(rst * (ptrdiff_t)es), n,
+                                    osc_g.stream);
+    if (rc == OSC_OK) rc = oscd_stream_sync(osc_g.stream);
+    if (rc != OSC_OK) osc_hip_error(rc, "strided RMA");
+    return;
+  }
+  {
+    char *host = (char *)malloc(n * es);
+    size_t t;
+    if (host == NULL) {
+      osc_set_error(OSC_ERR_NOMEM, "strided RMA staging");
+      return;
+    }
+    rc = oscd_malloc((void **)&stage, n * es);
+    if (rc == OSC_OK && is_put) {
+      for (t = 0; t < n; ++t)
+        memcpy(host + t * es, (char *)lp + (ptrdiff_t)t * lst * (ptrdiff_t)es, es);
+      rc = oscd_stream_sync(osc_g.stream);
+      if (rc == OSC_OK) rc = oscd_memcpy_h2d(stage, host, n * es);
+      if (rc == OSC_OK)
+        rc = oscd_copy_strided(r, stage, es, (uint64_t)(rst * (ptrdiff_t)es), es,
+                               n, osc_g.stream);
+      if (rc == OSC_OK) rc = oscd_stream_sync(osc_g.stream);
+    } else if (rc == OSC_OK) {
+      rc = oscd_copy_strided(stage, r, es, es, (uint64_t)(rst * (ptrdiff_t)es), n,
+                             osc_g.stream);
+      if (rc == OSC_OK) rc = oscd_stream_sync(osc_g.stream);
+      if (rc == OSC_OK) rc = oscd_memcpy_d2h(host, stage, n * es);
+      if (rc == OSC_OK)
+        for (t = 0; t < n; ++t)
+          memcpy((char *)lp + (ptrdiff_t)t * lst * (ptrdiff_t)es, host + t * es, es);
+    }
+    if (stage != NULL) oscd_free(stage);
+    free(host);
+    if (rc != OSC_OK) osc_hip_error(rc, "strided RMA");
+  }
+}
+
+/* put then signal update on the same stream, so the data is in place
+ * before the signal changes (put_signal.h:28-50, SHMEM_SIGNAL_SET / ADD) */
+static void put_signal(void *dest, const void *src, size_t nbytes,
+                       uint64_t *sig_addr, uint64_t signal, int sig_op, int pe,
+                       int blocking) {
+  char *d = remote(dest, 0, nbytes, pe);
+  uint64_t *s;
+  int rc;
+  if (d == NULL) return;
+  s = (uint64_t *)remote(sig_addr, 0, sizeof(uint64_t), pe);
+  if (s == NULL) return;
+  if (sig_op != SHMEM_SIGNAL_SET && sig_op != SHMEM_SIGNAL_ADD) {
+    osc_set_error(OSC_ERR_ARG, "signal op %d", sig_op);
+    return;
+  }
+  xfer(d, src, nbytes, 0);
+  rc = oscd_signal(s, signal, sig_op, osc_g.stream);
+  if (rc == OSC_OK && blocking) rc = oscd_stream_sync(osc_g.stream);
+  if (rc != OSC_OK) osc_hip_error(rc, "signal update");
+}
+
+void shmem_putmem_signal(void *dest, const void *source, size_t nbytes,
+                         uint64_t *sig_addr, uint64_t signal, int sig_op,
+                         int pe) {
+  put_signal(dest, source, nbytes, sig_addr, signal, sig_op, pe, 1);
+}
+void shmem_putmem_signal_nbi(void *dest, const void *source, size_t nbytes,
+                             uint64_t *sig_addr, uint64_t signal, int sig_op,
+                             int pe) {
+  put_signal(dest, source, nbytes, sig_addr, signal, sig_op, pe, 0);
+}
+
+static int read_signal(const uint64_t *sig_addr, uint64_t *v) {
+  size_t off;
+  int rc;
+  *v = 0;
+  if (!is_sym(sig_addr) || ((uintptr_t)sig_addr & 7u) != 0 ||
+      osc_sym_range(sig_addr, sizeof(uint64_t), &off) != OSC_OK)
+    return osc_set_error(OSC_ERR_PTR, "signal address %p",
+                         (const void *)sig_addr);
+  rc = oscd_memcpy_d2h(v, sig_addr, sizeof(*v));
+  return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "signal read");
+}
+
+uint64_t shmem_signal_fetch(const uint64_t *sig_addr) {
+  uint64_t v;
+  read_signal(sig_addr, &v);
+  return v;
+}
+
+static int cmp_ok(uint64_t v, int cmp, uint64_t x) {
+  switch (cmp) {
+    case SHMEM_CMP_EQ: return v == x;
+    case SHMEM_CMP_NE: return v != x;
+    case SHMEM_CMP_GT: return v > x;
+    case SHMEM_CMP_LE: return v <= x;
+    case SHMEM_CMP_LT: return v < x;
+    default: return v >= x; /* SHMEM_CMP_GE */
+  }
+}
+
+/* Polls the signal word; work already enqueued by this process is let
+ * finish first.  Like the reference, it blocks until another thread's
+ * put_signal satisfies the comparison if it is not satisfied yet; a bad
+ * address or comparison returns at once with the error recorded. */
+uint64_t shmem_signal_wait_until(uint64_t *sig_addr, int cmp,
+                                 uint64_t cmp_value) {
+  uint64_t v;
+  if (cmp < SHMEM_CMP_EQ || cmp > SHMEM_CMP_GE) {
+    osc_set_error(OSC_ERR_ARG, "comparison %d", cmp);
+    return 0;
+  }
+  oscd_stream_sync(osc_g.stream);
+  for (;;) {
+    if (read_signal(sig_addr, &v) != OSC_OK || cmp_ok(v, cmp, cmp_value))
+      return v;
+    sched_yield();
+  }
+}
+
+#define OSC_DEF_RMA(_type, _typename)                                          \
+  void shmem_##_typename##_put(_type *dest, const _type *src, size_t nelems,   \
+                               int pe) {                                       \
+    shmem_putmem(dest, src, nelems * sizeof(_type), pe);                       \
+  }                                                                            \
+  void shmem_##_typename##_put_nbi(_type *dest, const _type *src,              \
+                                   size_t nelems, int pe) {                    \
+    shmem_putmem_nbi(dest, src, nelems * sizeof(_type), pe);                   \
+  }                                                                            \
+  void shmem_##_typename##_get(_type *dest, const _type *src, size_t nelems,   \
+                               int pe) {                                       \
+    shmem_getmem(dest, src, nelems * sizeof(_type), pe);                       \
+  }                                                                            \
+  void shmem_##_typename##_get_nbi(_type *dest, const _type *src,              \
+                                   size_t nelems, int pe) {                    \
+    shmem_getmem_nbi(dest, src, nelems * sizeof(_type), pe);                   \
+  }                                                                            \
+  void shmem_##_typename##_p(_type *dest, _type value, int pe) {               \
+    shmem_putmem(dest, &value, sizeof(_type), pe);                             \
+  }                                                                            \
+  _type shmem_##_typename##_g(const _type *src, int pe) {                      \
+    _type v;                                                                   \
+    memset(&v, 0, sizeof(v));                                                  \
+    shmem_getmem(&v, src, sizeof(_type), pe);                                  \
+    return v;                                                                  \
+  }                                                                            \
+  void shmem_##_typename##_iput(_type *dest, const _type *src, ptrdiff_t tst,  \
+                                ptrdiff_t sst, size_t nelems, int pe) {        \
+    strided(dest, src, sizeof(_type), tst, sst, nelems, pe, 1);                \
+  }                                                                            \
+  void shmem_##_typename##_iget(_type *dest, const _type *src, ptrdiff_t tst,  \
+                                ptrdiff_t sst, size_t nelems, int pe) {        \
+    strided(dest, src, sizeof(_type), tst, sst, nelems, pe, 0);                \
+  }                                                                            \
+  void shmem_##_typename##_put_signal(_type *dest, const _type *src,           \
+                                      size_t nelems, uint64_t *sig_addr,       \
+                                      uint64_t signal, int sig_op, int pe) {   \
+    put_signal(dest, src, nelems * sizeof(_type), sig_addr, signal, sig_op,    \
+               pe, 1);                                                         \
+  }                                                                            \
+  void shmem_##_typename##_put_signal_nbi(_type *dest, const _type *src,       \
+                                          size_t nelems, uint64_t *sig_addr,   \
+                                          uint64_t signal, int sig_op,         \
+                                          int pe) {                            \
+    put_signal(dest, src, nelems * sizeof(_type), sig_addr, signal, sig_op,    \
+               pe, 0);                                                         \
+  }
+OSC_RMA_TYPES(OSC_DEF_RMA)
+
+#define OSC_DEF_SIZED_RMA(_size)                                               \
+  void shmem_put##_size(void *dest, const void *src, size_t nelems, int pe) {  \
+    shmem_putmem(dest, src, nelems * ((_size) / 8), pe);                       \
+  }                                                                            \
+  void shmem_put##_size##_nbi(void *dest, const void *src, size_t nelems,      \
+                              int pe) {                                        \
+    shmem_putmem_nbi(dest, src, nelems * ((_size) / 8), pe);                   \
+  }                                                                            \
+  void shmem_get##_size(void *dest, const void *src, size_t nelems, int pe) {  \
+    shmem_getmem(dest, src, nelems * ((_size) / 8), pe);                       \
+  }                                                                            \
+  void shmem_get##_size##_nbi(void *dest, const void *src, size_t nelems,      \
+                              int pe) {                                        \
+    shmem_getmem_nbi(dest, src, nelems * ((_size) / 8), pe);                   \
+  }                                                                            \
+  void shmem_iput##_size(void *dest, const void *src, ptrdiff_t tst,           \
+                         ptrdiff_t sst, size_t nelems, int pe) {               \
+    strided(dest, src, (_size) / 8, tst, sst, nelems, pe, 1);                  \
+  }                                                                            \
+  void shmem_iget##_size(void *dest, const void *src, ptrdiff_t tst,           \
+                         ptrdiff_t sst, size_t nelems, int pe) {               \
+    strided(dest, src, (_size) / 8, tst, sst, nelems, pe, 0);                  \
+  }                                                                            \
+  void shmem_put##_size##_signal(void *dest, const void *src, size_t nelems,   \
+                                 uint64_t *sig_addr, uint64_t signal,          \
+                                 int sig_op, int pe) {                         \
+    put_signal(dest, src, nelems * ((_size) / 8), sig_addr, signal, sig_op,    \
+               pe, 1);                                                         \
+  }                                                                            \
+  void shmem_put##_size##_signal_nbi(void *dest, const void *src,              \
+                                     size_t nelems, uint64_t *sig_addr,        \
+                                     uint64_t signal, int sig_op, int pe) {    \
+    put_signal(dest, src, nelems * ((_size) / 8), sig_addr, signal, sig_op,    \
+               pe, 0);                                                         \
+  }
+OSC_SIZED_RMA(OSC_DEF_SIZED_RMA)
 
 /* ---- collectives ---------------------------------------------------------------- */
 
@@ -287,4 +542,175 @@ int shmem_alltoallmem(shmem_team_t team, void *dest, const void *source,
 int shmem_broadcastmem(shmem_team_t team, void *dest, const void *source,
                        size_t nbytes, int PE_root) {
   return finish(osc_broadcast(team, dest, source, nbytes, PE_root, NULL));
+}
+
+/* collect: in team-call mode nelems is every member's count; the
+ * variable-size case is osc_collect() with one size per member. */
+static int uniform_collect(shmem_team_t team, void *dest, const void *source,
+                           size_t nbytes) {
+  const osc_team *t = osc_team_get(team);
+  size_t *sizes;
+  int i, rc;
+  if (!osc_g.initialized) return osc_set_error(OSC_ERR_NOTINIT, "not initialised");
+  if (t == NULL) return osc_set_error(OSC_ERR_TEAM, "invalid team");
+  sizes = (size_t *)malloc(sizeof(size_t) * (size_t)t->npes);
+  if (sizes == NULL) return osc_set_error(OSC_ERR_NOMEM, "collect sizes");
+  for (i = 0; i < t->npes; ++i) sizes[i] = nbytes;
+  rc = osc_collect(team, dest, source, sizes, NULL);
+  free(sizes);
+  return finish(rc);
+}
+
+#define OSC_DEF_MOVE2(_type, _typename)                                        \
+  int shmem_##_typename##_collect(shmem_team_t team, _type *dest,              \
+                                  const _type *source, size_t nelems) {        \
+    if (nelems > ((size_t)-1) / sizeof(_type))                                 \
+      return osc_set_error(OSC_ERR_ARG, "size overflow");                      \
+    return uniform_collect(team, dest, source, nelems * sizeof(_type));        \
+  }                                                                            \
+  int shmem_##_typename##_alltoalls(shmem_team_t team, _type *dest,            \
+                                    const _type *source, ptrdiff_t dst,        \
+                                    ptrdiff_t sst, size_t nelems) {            \
+    return finish(osc_alltoalls(team, dest, source, dst, sst, sizeof(_type),   \
+                                nelems, NULL));                                \
+  }
+OSC_RMA_TYPES(OSC_DEF_MOVE2)
+
+int shmem_collectmem(shmem_team_t team, void *dest, const void *source,
+                     size_t nelems) {
+  return uniform_collect(team, dest, source, nelems);
+}
+/* one element of nelems bytes per member pair (alltoalls.c:391-428) */
+int shmem_alltoallsmem(shmem_team_t team, void *dest, const void *source,
+                       ptrdiff_t dst, ptrdiff_t sst, size_t nelems) {
+  return finish(osc_alltoalls(team, dest, source, dst, sst, nelems, 1, NULL));
+}
+
+/* ---- active-set forms (deprecated in 1.5) ------------------------------------- */
+/* The shim's legacy entry points (shcoll-shim.c:238-266 alltoall,
+ * :373-420 alltoalls, :525-566 collect, :672-714 fcollect, :824-866
+ * broadcast, :867-1014 to_all, :1442-1477 sync/barrier) map the active set
+ * to a transient team and run the same kernels.  They return void. */
+
+static void legacy_reduce(int op, int dtype, void *target, const void *source,
+                          int nreduce, int PE_start, int logPE_stride,
+                          int PE_size) {
+  osc_team t;
+  if (dtype < 0) {
+    osc_set_error(OSC_ERR_UNSUPPORTED, "long double reduction (no 80-bit type on gfx950)");
+    return;
+  }
+  if (nreduce < 0) {
+    osc_set_error(OSC_ERR_ARG, "nreduce %d", nreduce);
+    return;
+  }
+  if (osc_active_set(PE_start, logPE_stride, PE_size, &t) != OSC_OK) return;
+  finish(osc_reduce(&t, op, dtype, target, source, (size_t)nreduce, NULL));
+}
+
+#define OSC_DEF_TO_ALL(_type, _typename, _op, _OPC)                            \
+  void shmem_##_typename##_##_op##_to_all(                                     \
+      _type *target, const _type *source, int nreduce, int PE_start,           \
+      int logPE_stride, int PE_size, _type *pWrk, long *pSync) {               \
+    (void)pWrk;                                                                \
+    (void)pSync;                                                               \
+    legacy_reduce(_OPC, OSC_DT_##_typename, target, source, nreduce,           \
+                  PE_start, logPE_stride, PE_size);                            \
+  }
+#define OSC_DEF_AND_TO_ALL(_t, _n) OSC_DEF_TO_ALL(_t, _n, and, OSC_OP_AND)
+#define OSC_DEF_OR_TO_ALL(_t, _n) OSC_DEF_TO_ALL(_t, _n, or, OSC_OP_OR)
+#define OSC_DEF_XOR_TO_ALL(_t, _n) OSC_DEF_TO_ALL(_t, _n, xor, OSC_OP_XOR)
+#define OSC_DEF_MIN_TO_ALL(_t, _n) OSC_DEF_TO_ALL(_t, _n, min, OSC_OP_MIN)
+#define OSC_DEF_MAX_TO_ALL(_t, _n) OSC_DEF_TO_ALL(_t, _n, max, OSC_OP_MAX)
+#define OSC_DEF_SUM_TO_ALL(_t, _n) OSC_DEF_TO_ALL(_t, _n, sum, OSC_OP_SUM)
+#define OSC_DEF_PROD_TO_ALL(_t, _n) OSC_DEF_TO_ALL(_t, _n, prod, OSC_OP_PROD)
+OSC_TO_ALL_BITWISE_TYPES(OSC_DEF_AND_TO_ALL)
+OSC_TO_ALL_BITWISE_TYPES(OSC_DEF_OR_TO_ALL)
+OSC_TO_ALL_BITWISE_TYPES(OSC_DEF_XOR_TO_ALL)
+OSC_TO_ALL_MINMAX_TYPES(OSC_DEF_MIN_TO_ALL)
+OSC_TO_ALL_MINMAX_TYPES(OSC_DEF_MAX_TO_ALL)
+OSC_TO_ALL_ARITH_TYPES(OSC_DEF_SUM_TO_ALL)
+OSC_TO_ALL_ARITH_TYPES(OSC_DEF_PROD_TO_ALL)
+
+/* *out = nelems * es; OSC_ERR_ARG (recorded) if that overflows */
+static int sized_bytes(size_t nelems, size_t es, size_t *out) {
+  if (nelems > ((size_t)-1) / es) return osc_set_error(OSC_ERR_ARG, "size overflow");
+  *out = nelems * es;
+  return OSC_OK;
+}
+
+#define OSC_DEF_SIZED_COLL(_size)                                              \
+  void shmem_fcollect##_size(void *target, const void *source, size_t nelems, \
+                             int PE_start, int logPE_stride, int PE_size,     \
+                             long *pSync) {                                    \
+    osc_team t;                                                                \
+    size_t nb = 0;                                                             \
+    (void)pSync;                                                               \
+    if (osc_active_set(PE_start, logPE_stride, PE_size, &t) != OSC_OK ||       \
+        sized_bytes(nelems, (_size) / 8, &nb) != OSC_OK)                       \
+      return;                                                                  \
+    finish(osc_fcollect(&t, target, source, nb, NULL));                        \
+  }                                                                            \
+  void shmem_collect##_size(void *target, const void *source, size_t nelems,  \
+                            int PE_start, int logPE_stride, int PE_size,      \
+                            long *pSync) {                                     \
+    osc_team t;                                                                \
+    size_t nb = 0;                                                             \
+    (void)pSync;                                                               \
+    if (osc_active_set(PE_start, logPE_stride, PE_size, &t) != OSC_OK ||       \
+        sized_bytes(nelems, (_size) / 8, &nb) != OSC_OK)                       \
+      return;                                                                  \
+    uniform_collect(&t, target, source, nb);                                   \
+  }                                                                            \
+  void shmem_alltoall##_size(void *target, const void *source, size_t nelems, \
+                             int PE_start, int logPE_stride, int PE_size,     \
+                             long *pSync) {                                    \
+    osc_team t;                                                                \
+    size_t nb = 0;                                                             \
+    (void)pSync;                                                               \
+    if (osc_active_set(PE_start, logPE_stride, PE_size, &t) != OSC_OK ||       \
+        sized_bytes(nelems, (_size) / 8, &nb) != OSC_OK)                       \
+      return;                                                                  \
+    finish(osc_alltoall(&t, target, source, nb, NULL));                        \
+  }                                                                            \
+  void shmem_alltoalls##_size(void *target, const void *source,               \
+                              ptrdiff_t dst, ptrdiff_t sst, size_t nelems,    \
+                              int PE_start, int logPE_stride, int PE_size,    \
+                              long *pSync) {                                   \
+    osc_team t;                                                                \
+    (void)pSync;                                                               \
+    if (osc_active_set(PE_start, logPE_stride, PE_size, &t) != OSC_OK) return; \
+    finish(osc_alltoalls(&t, target, source, dst, sst, (_size) / 8, nelems,    \
+                         NULL));                                               \
+  }                                                                            \
+  void shmem_broadcast##_size(void *target, const void *source,               \
+                              size_t nelems, int PE_root, int PE_start,       \
+                              int logPE_stride, int PE_size, long *pSync) {   \
+    osc_team t;                                                                \
+    size_t nb = 0;                                                             \
+    (void)pSync;                                                               \
+    if (osc_active_set(PE_start, logPE_stride, PE_size, &t) != OSC_OK ||       \
+        sized_bytes(nelems, (_size) / 8, &nb) != OSC_OK)                       \
+      return;                                                                  \
+    finish(osc_broadcast_impl(&t, target, source, nb, PE_root, 1, NULL));      \
+  }
+OSC_DEF_SIZED_COLL(32)
+OSC_DEF_SIZED_COLL(64)
+
+/* With one host thread standing for every PE, a barrier or sync over an
+ * active set is completion of the device work enqueued so far. */
+void shmem_barrier(int PE_start, int logPE_stride, int PE_size, long *pSync) {
+  osc_team t;
+  int rc;
+  (void)pSync;
+  if (osc_active_set(PE_start, logPE_stride, PE_size, &t) != OSC_OK) return;
+  rc = oscd_device_sync();
+  if (rc != OSC_OK) osc_hip_error(rc, "barrier");
+}
+void shmem_sync_deprecated(int PE_start, int logPE_stride, int PE_size,
+                           long *pSync) {
+  shmem_barrier(PE_start, logPE_stride, PE_size, pSync);
+}
+void shmem_sync_all(void) {
+  if (osc_g.initialized) oscd_device_sync();
 }

@@ -70,9 +70,36 @@ int oscd_fcollect(const oscd_geom *g, uint64_t dest_off, uint64_t src_off,
 int oscd_alltoall(const oscd_geom *g, uint64_t dest_off, uint64_t src_off,
                   uint64_t nbytes, void *stream);
 
-/* broadcast: dest_j[0 .. nbytes) = src_root[0 .. nbytes) for all j */
+/* broadcast: dest_j[0 .. nbytes) = src_root[0 .. nbytes) for all j
+ * (j != root when skip_root, as the active-set broadcast does) */
 int oscd_broadcast(const oscd_geom *g, int root, uint64_t dest_off,
-                   uint64_t src_off, uint64_t nbytes, void *stream);
+                   uint64_t src_off, uint64_t nbytes, int skip_root,
+                   void *stream);
+
+/* collect, variable sizes: dest_j[off_i .. off_i + n_i) = src_i[0 .. n_i)
+ * for all members i, j.  d_tab: device array of 2*npes uint64 (off_i, n_i)
+ * in bytes; max_n = max n_i; unit_width W divides every off_i, n_i,
+ * dest_off, src_off and the slice. */
+int oscd_collect(const oscd_geom *g, uint64_t dest_off, uint64_t src_off,
+                 const uint64_t *d_tab, uint64_t max_n, int unit_width,
+                 void *stream);
+
+/* strided alltoall of nelems elements of es bytes per member pair:
+ * dest_j[((i*nelems + t)*dst)*es ..+es) = src_i[((j*nelems + t)*sst)*es ..)
+ * for t < nelems (dst, sst in elements, >= 1). */
+int oscd_alltoalls(const oscd_geom *g, uint64_t dest_off, uint64_t src_off,
+                   uint64_t es, uint64_t dst, uint64_t sst, uint64_t nelems,
+                   void *stream);
+
+/* strided copy: dst[t*dst_step ..+es) = src[t*src_step ..+es), t < n
+ * (steps in bytes) */
+int oscd_copy_strided(void *dst, const void *src, uint64_t es,
+                      uint64_t dst_step, uint64_t src_step, uint64_t n,
+                      void *stream);
+
+/* signal update after the preceding work on `stream`: *sig = value
+ * (op 0, set) or *sig += value (op 1, add), as one 64-bit atomic. */
+int oscd_signal(uint64_t *sig, uint64_t value, int op, void *stream);
 
 /* ---- batched execution ------------------------------------------------- */
 /* One collective call of a batch as the device sees it (64 bytes). */

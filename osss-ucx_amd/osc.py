@@ -85,6 +85,7 @@ def lib() -> C.CDLL:
             f"{LIB_PATH} not found: build it first (make -C osss-ucx_amd)")
     L = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
     vp, sz, i, u64, d = C.c_void_p, C.c_size_t, C.c_int, C.c_uint64, C.c_double
+    pd = C.c_ssize_t  # ptrdiff_t (LP64)
     sig = {
         "osc_init": (i, [i, sz, i]),
         "osc_finalize": (i, []),
@@ -157,7 +158,39 @@ def lib() -> C.CDLL:
         "shmem_fcollectmem": (i, [vp, vp, vp, sz]),
         "shmem_alltoallmem": (i, [vp, vp, vp, sz]),
         "shmem_broadcastmem": (i, [vp, vp, vp, sz, i]),
+        "osc_last_error_code": (i, []),
+        "osc_clear_error": (None, []),
+        "osc_collect": (i, [vp, vp, vp, C.POINTER(sz), vp]),
+        "osc_alltoalls": (i, [vp, vp, vp, pd, pd, sz, sz, vp]),
+        "shmem_collectmem": (i, [vp, vp, vp, sz]),
+        "shmem_alltoallsmem": (i, [vp, vp, vp, pd, pd, sz]),
+        "shmem_putmem_signal": (None, [vp, vp, sz, vp, u64, i, i]),
+        "shmem_putmem_signal_nbi": (None, [vp, vp, sz, vp, u64, i, i]),
+        "shmem_signal_fetch": (u64, [vp]),
+        "shmem_signal_wait_until": (u64, [vp, i, u64]),
+        "shmem_quiet": (None, []),
+        "shmem_fence": (None, []),
+        "shmem_sync_all": (None, []),
+        "shmem_barrier": (None, [i, i, i, vp]),
+        "shmem_sync_deprecated": (None, [i, i, i, vp]),
     }
+    for size in (8, 16, 32, 64, 128):
+        for base, args in (("put%d", [vp, vp, sz, i]),
+                           ("put%d_nbi", [vp, vp, sz, i]),
+                           ("get%d", [vp, vp, sz, i]),
+                           ("get%d_nbi", [vp, vp, sz, i]),
+                           ("iput%d", [vp, vp, pd, pd, sz, i]),
+                           ("iget%d", [vp, vp, pd, pd, sz, i]),
+                           ("put%d_signal", [vp, vp, sz, vp, u64, i, i]),
+                           ("put%d_signal_nbi", [vp, vp, sz, vp, u64, i, i])):
+            sig["shmem_" + base % size] = (None, args)
+    for size in (32, 64):
+        sig[f"shmem_fcollect{size}"] = (None, [vp, vp, sz, i, i, i, vp])
+        sig[f"shmem_collect{size}"] = (None, [vp, vp, sz, i, i, i, vp])
+        sig[f"shmem_alltoall{size}"] = (None, [vp, vp, sz, i, i, i, vp])
+        sig[f"shmem_alltoalls{size}"] = (None, [vp, vp, pd, pd, sz, i, i, i,
+                                                vp])
+        sig[f"shmem_broadcast{size}"] = (None, [vp, vp, sz, i, i, i, i, vp])
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
         fn.restype = res
@@ -175,15 +208,48 @@ def check(rc: int, what: str) -> None:
         raise OscError(rc, what, lib().osc_last_error().decode())
 
 
+# C types of the typed entry points' element arguments (shmem_<T>_p/_g)
+CTYPE = {"float": C.c_float, "double": C.c_double, "char": C.c_char,
+         "schar": C.c_byte, "short": C.c_short, "int": C.c_int,
+         "long": C.c_long, "longlong": C.c_longlong, "uchar": C.c_ubyte,
+         "ushort": C.c_ushort, "uint": C.c_uint, "ulong": C.c_ulong,
+         "ulonglong": C.c_ulonglong, "int8": C.c_int8, "int16": C.c_int16,
+         "int32": C.c_int32, "int64": C.c_int64, "uint8": C.c_uint8,
+         "uint16": C.c_uint16, "uint32": C.c_uint32, "uint64": C.c_uint64,
+         "size": C.c_size_t, "ptrdiff": C.c_ssize_t}
+
+
 def typed(name: str) -> C._CFuncPtr:
-    """An OpenSHMEM typed collective, e.g. typed('shmem_int_sum_reduce')."""
+    """An OpenSHMEM typed entry point, e.g. typed('shmem_int_sum_reduce'),
+    typed('shmem_long_put'), typed('shmem_int_max_to_all')."""
     fn = getattr(lib(), name)
+    vp, sz, i, pd = C.c_void_p, C.c_size_t, C.c_int, C.c_ssize_t
+    tn = name[len("shmem_"):].split("_")[0]
     fn.restype = C.c_int
     if name.endswith("_broadcast"):
-        fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
-                       C.c_int]
+        fn.argtypes = [vp, vp, vp, sz, i]
+    elif name.endswith("_alltoalls"):
+        fn.argtypes = [vp, vp, vp, pd, pd, sz]
+    elif name.endswith("_to_all"):
+        fn.restype = None
+        fn.argtypes = [vp, vp, i, i, i, i, vp, vp]
+    elif name.endswith(("_put", "_get", "_put_nbi", "_get_nbi")):
+        fn.restype = None
+        fn.argtypes = [vp, vp, sz, i]
+    elif name.endswith(("_iput", "_iget")):
+        fn.restype = None
+        fn.argtypes = [vp, vp, pd, pd, sz, i]
+    elif name.endswith(("_put_signal", "_put_signal_nbi")):
+        fn.restype = None
+        fn.argtypes = [vp, vp, sz, vp, C.c_uint64, i, i]
+    elif name.endswith("_p"):
+        fn.restype = None
+        fn.argtypes = [vp, CTYPE[tn], i]
+    elif name.endswith("_g"):
+        fn.restype = CTYPE[tn]
+        fn.argtypes = [vp, i]
     else:
-        fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+        fn.argtypes = [vp, vp, vp, sz]
     return fn
 
 
