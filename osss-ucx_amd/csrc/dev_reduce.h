@@ -131,6 +131,7 @@ template <class T, int OP, bool SPEC>
 struct PolSeq {
   typedef T Acc;
   static constexpr bool TREE = false;
+  static constexpr int SHAPE = 0;  // default kernel shape
   static constexpr bool SPECIAL = SPEC;
   static constexpr int OPC = OP;
   __device__ static Acc lift(T x) { return x; }
@@ -144,6 +145,7 @@ template <class T, int OP>
 struct PolTree {
   typedef T Acc;
   static constexpr bool TREE = true;
+  static constexpr int SHAPE = 2;  // default kernel shape
   static constexpr bool SPECIAL = false;
   static constexpr int OPC = OP;
   __device__ static Acc lift(T x) { return x; }
@@ -157,6 +159,7 @@ template <int OP>
 struct PolF32Acc {
   typedef double Acc;
   static constexpr bool TREE = false;
+  static constexpr int SHAPE = 1;  // default kernel shape
   static constexpr bool SPECIAL = false;
   static constexpr int OPC = OP;
   __device__ static Acc lift(float x) { return (double)x; }
@@ -174,6 +177,7 @@ template <int OP>
 struct PolF64Acc {
   typedef dd Acc;
   static constexpr bool TREE = false;
+  static constexpr int SHAPE = 0;  // default kernel shape
   static constexpr bool SPECIAL = false;
   static constexpr int OPC = OP;
   __device__ static Acc lift(double x) { return {x, 0.0}; }
@@ -192,6 +196,7 @@ template <int OP>
 struct PolC32Acc {
   typedef c64acc Acc;
   static constexpr bool TREE = false;
+  static constexpr int SHAPE = 0;  // default kernel shape
   static constexpr bool SPECIAL = false;
   static constexpr int OPC = OP;
   __device__ static Acc lift(cf32 x) { return {(double)x.re, (double)x.im}; }
@@ -213,6 +218,7 @@ template <int OP>
 struct PolC64Acc {
   typedef cddacc Acc;
   static constexpr bool TREE = false;
+  static constexpr int SHAPE = 0;  // default kernel shape
   static constexpr bool SPECIAL = false;
   static constexpr int OPC = OP;
   __device__ static Acc lift(cf64 x) { return {{x.re, 0.0}, {x.im, 0.0}}; }
@@ -343,13 +349,13 @@ __device__ void special_block(const oscd_geom &g, uint64_t src_off,
 constexpr int kBlock = 256;
 
 // Shapes (units per thread, source slices in flight) a launch can use:
-// 0: 8x2, 1: 4x4, 2: 2x4, 3: 1x8.
+// 0: 8x2, 1: 8x1, 2: 2x4, 3: 1x8.
 constexpr int kNumShapes = 4;
 __host__ __device__ constexpr int shape_upt(int s) {
-  return s == 0 ? 8 : s == 1 ? 4 : s == 2 ? 2 : 1;
+  return s == 0 ? 8 : s == 1 ? 8 : s == 2 ? 2 : 1;
 }
 __host__ __device__ constexpr int shape_infl(int s) {
-  return s == 0 ? 2 : s == 1 ? 4 : s == 2 ? 4 : 8;
+  return s == 0 ? 2 : s == 1 ? 1 : s == 2 ? 4 : 8;
 }
 
 // One block-iteration: the workgroup reduces units [ub, ub + 256*UPT).
@@ -396,34 +402,62 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
   if (any) {
     Acc acc[UPT][VEC];
     if constexpr (!Pol::TREE) {
+      // Slice 0 seeds the accumulator; then groups of INFL slices whose
+      // loads are unconditional (so the compiler's wait counts stay exact
+      // and the next group's loads are not held back by a drained queue),
+      // then the < INFL leftover slices one at a time.
       uint32_t nanm = 0;
-      for (int i0 = 0; i0 < P; i0 += INFL) {
-        T buf[INFL][UPT][VEC];
-        const int m = (P - i0) < INFL ? (P - i0) : INFL;
+      {
+        T buf[UPT][VEC];
 #pragma unroll
-        for (int j = 0; j < INFL; ++j)
-          if (j < m) {
-            const char *sp = s0 + (i0 + j) * step;
+        for (int k = 0; k < UPT; ++k)
+          if (FAST || cnt[k] > 0)
+            load_unit<T, VEC>(s0 + loff[k], buf[k], FAST || full[k], cnt[k]);
 #pragma unroll
-            for (int k = 0; k < UPT; ++k)
-              if (FAST || cnt[k] > 0)
-                load_unit<T, VEC>(sp + loff[k], buf[j][k], FAST || full[k],
-                                  cnt[k]);
+        for (int k = 0; k < UPT; ++k)
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) {
+            if (Pol::SPECIAL && is_nan(buf[k][v])) nanm |= 1u << (k * VEC + v);
+            acc[k][v] = Pol::lift(buf[k][v]);
           }
+      }
+      int i = 1;
+      for (; i + INFL <= P; i += INFL) {
+        T buf[INFL][UPT][VEC];
 #pragma unroll
         for (int j = 0; j < INFL; ++j) {
-          if (j < m) {
+          const char *sp = s0 + (i + j) * step;
 #pragma unroll
-            for (int k = 0; k < UPT; ++k)
-#pragma unroll
-              for (int v = 0; v < VEC; ++v) {
-                if (Pol::SPECIAL && is_nan(buf[j][k][v]))
-                  nanm |= 1u << (k * VEC + v);
-                acc[k][v] = (i0 + j == 0) ? Pol::lift(buf[j][k][v])
-                                          : Pol::seq(acc[k][v], buf[j][k][v]);
-              }
-          }
+          for (int k = 0; k < UPT; ++k)
+            if (FAST || cnt[k] > 0)
+              load_unit<T, VEC>(sp + loff[k], buf[j][k], FAST || full[k],
+                                cnt[k]);
         }
+#pragma unroll
+        for (int j = 0; j < INFL; ++j)
+#pragma unroll
+          for (int k = 0; k < UPT; ++k)
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) {
+              if (Pol::SPECIAL && is_nan(buf[j][k][v]))
+                nanm |= 1u << (k * VEC + v);
+              acc[k][v] = Pol::seq(acc[k][v], buf[j][k][v]);
+            }
+      }
+      for (; i < P; ++i) {
+        T buf[UPT][VEC];
+        const char *sp = s0 + i * step;
+#pragma unroll
+        for (int k = 0; k < UPT; ++k)
+          if (FAST || cnt[k] > 0)
+            load_unit<T, VEC>(sp + loff[k], buf[k], FAST || full[k], cnt[k]);
+#pragma unroll
+        for (int k = 0; k < UPT; ++k)
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) {
+            if (Pol::SPECIAL && is_nan(buf[k][v])) nanm |= 1u << (k * VEC + v);
+            acc[k][v] = Pol::seq(acc[k][v], buf[k][v]);
+          }
       }
 #pragma unroll
       for (int k = 0; k < UPT; ++k)
@@ -444,75 +478,82 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
       int kl = 0;
       int mk = 0;
       int mn = member_of(1, P, lg);
-      for (int i0 = 0; i0 < P; i0 += INFL) {
-        T buf[INFL][UPT][VEC];
-        const int m = (P - i0) < INFL ? (P - i0) : INFL;
-#pragma unroll
-        for (int j = 0; j < INFL; ++j)
-          if (j < m) {
-            const char *sp = s0 + (i0 + j) * step;
+      // one slice, in team-index order: fold, then carry the new subtree
+      // up while level L is occupied; every stk index is a constant after
+      // unrolling (stays in VGPRs)
+      auto eat = [&](const T (&b)[UPT][VEC], const int i) {
+        Acc leaf[UPT][VEC];
+        bool ready = true;
+        if (i == mk) {
+          if (mn - mk == 2) {
 #pragma unroll
             for (int k = 0; k < UPT; ++k)
-              if (FAST || cnt[k] > 0)
-                load_unit<T, VEC>(sp + loff[k], buf[j][k], FAST || full[k],
-                                  cnt[k]);
+#pragma unroll
+              for (int v = 0; v < VEC; ++v) pend[k][v] = b[k][v];
+            ready = false;
+          } else {
+#pragma unroll
+            for (int k = 0; k < UPT; ++k)
+#pragma unroll
+              for (int v = 0; v < VEC; ++v) leaf[k][v] = Pol::lift(b[k][v]);
           }
+        } else {
 #pragma unroll
-        for (int j = 0; j < INFL; ++j) {
-          if (j < m) {
-            const int i = i0 + j;
-            Acc leaf[UPT][VEC];
-            bool ready = true;
-            if (i == mk) {
-              if (mn - mk == 2) {
+          for (int k = 0; k < UPT; ++k)
 #pragma unroll
-                for (int k = 0; k < UPT; ++k)
+            for (int v = 0; v < VEC; ++v)
+              leaf[k][v] = Pol::comb(Pol::lift(b[k][v]), Pol::lift(pend[k][v]));
+        }
+        if (ready) {
+          bool carry = true;
 #pragma unroll
-                  for (int v = 0; v < VEC; ++v) pend[k][v] = buf[j][k][v];
-                ready = false;
-              } else {
+          for (int L = 0; L < MAXLV; ++L) {
+            if (carry) {
+              if ((kl >> L) & 1) {
 #pragma unroll
                 for (int k = 0; k < UPT; ++k)
 #pragma unroll
                   for (int v = 0; v < VEC; ++v)
-                    leaf[k][v] = Pol::lift(buf[j][k][v]);
+                    leaf[k][v] = Pol::comb(stk[L][k][v], leaf[k][v]);
+              } else {
+#pragma unroll
+                for (int k = 0; k < UPT; ++k)
+#pragma unroll
+                  for (int v = 0; v < VEC; ++v) stk[L][k][v] = leaf[k][v];
+                carry = false;
               }
-            } else {
-#pragma unroll
-              for (int k = 0; k < UPT; ++k)
-#pragma unroll
-                for (int v = 0; v < VEC; ++v)
-                  leaf[k][v] = Pol::comb(Pol::lift(buf[j][k][v]),
-                                         Pol::lift(pend[k][v]));
-            }
-            if (ready) {
-              // carry the new subtree up while level L is occupied; every
-              // stk index is a constant after unrolling (stays in VGPRs)
-              bool carry = true;
-#pragma unroll
-              for (int L = 0; L < MAXLV; ++L) {
-                if (carry) {
-                  if ((kl >> L) & 1) {
-#pragma unroll
-                    for (int k = 0; k < UPT; ++k)
-#pragma unroll
-                      for (int v = 0; v < VEC; ++v)
-                        leaf[k][v] = Pol::comb(stk[L][k][v], leaf[k][v]);
-                  } else {
-#pragma unroll
-                    for (int k = 0; k < UPT; ++k)
-#pragma unroll
-                      for (int v = 0; v < VEC; ++v) stk[L][k][v] = leaf[k][v];
-                    carry = false;
-                  }
-                }
-              }
-              ++kl;
-              mk = mn;
-              mn = member_of(kl + 1, P, lg);
             }
           }
+          ++kl;
+          mk = mn;
+          mn = member_of(kl + 1, P, lg);
         }
+      };
+      // groups of INFL slices with unconditional loads, then the leftover
+      // slices one at a time (as in the order-free branch above)
+      int i = 0;
+      for (; i + INFL <= P; i += INFL) {
+        T buf[INFL][UPT][VEC];
+#pragma unroll
+        for (int j = 0; j < INFL; ++j) {
+          const char *sp = s0 + (i + j) * step;
+#pragma unroll
+          for (int k = 0; k < UPT; ++k)
+            if (FAST || cnt[k] > 0)
+              load_unit<T, VEC>(sp + loff[k], buf[j][k], FAST || full[k],
+                                cnt[k]);
+        }
+#pragma unroll
+        for (int j = 0; j < INFL; ++j) eat(buf[j], i + j);
+      }
+      for (; i < P; ++i) {
+        T buf[UPT][VEC];
+        const char *sp = s0 + i * step;
+#pragma unroll
+        for (int k = 0; k < UPT; ++k)
+          if (FAST || cnt[k] > 0)
+            load_unit<T, VEC>(sp + loff[k], buf[k], FAST || full[k], cnt[k]);
+        eat(buf, i);
       }
 #pragma unroll
       for (int L = 0; L < MAXLV; ++L)
@@ -549,6 +590,25 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
   }
 }
 
+// The ragged last block-iteration of a call, [ub, min(ub + 256*UPT,
+// units)): 256 units at a time through the one-unit-per-thread body with
+// its guards.  Keeping the guarded code out of the UPT-wide body keeps the
+// kernel's register footprint at that of the guard-free body, which sets
+// how many workgroups stay resident per CU.
+template <class T, class Pol, int VEC, int UPT, int MAXLV>
+__device__ __forceinline__ void reduce_tail(const oscd_geom &g,
+                                            const uint64_t dest_off,
+                                            const uint64_t src_off,
+                                            const uint64_t n, const int lg,
+                                            const uint64_t ub, T *lds) {
+  const uint64_t nunits = (n + VEC - 1) / VEC;
+  const uint64_t span_end = ub + (uint64_t)kBlock * UPT;
+  const uint64_t end = span_end < nunits ? span_end : nunits;
+  for (uint64_t sub = ub; sub < end; sub += kBlock)
+    reduce_block<false, T, Pol, VEC, 1, 1, MAXLV>(g, dest_off, src_off, n, lg,
+                                                  sub, lds);
+}
+
 template <class T, class Pol, int VEC, int UPT, int INFL, int MAXLV>
 __global__ __launch_bounds__(kBlock) void k_reduce(const oscd_geom g,
                                                    const uint64_t dest_off,
@@ -561,15 +621,16 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const oscd_geom g,
   constexpr uint64_t kSpan = (uint64_t)kBlock * UPT;
   const uint64_t nunits = (n + VEC - 1) / VEC;
   const uint64_t nfull = n / VEC;
-  for (uint64_t ub = (uint64_t)blockIdx.x * kSpan; ub < nunits;
-       ub += (uint64_t)gridDim.x * kSpan) {
-    if (ub + kSpan <= nfull)
-      reduce_block<true, T, Pol, VEC, UPT, INFL, MAXLV>(
-          g, dest_off, src_off, n, lg, ub, (T *)k_reduce_lds);
-    else
-      reduce_block<false, T, Pol, VEC, UPT, INFL, MAXLV>(
-          g, dest_off, src_off, n, lg, ub, (T *)k_reduce_lds);
-  }
+  // one span per workgroup (the grid covers every span): straight-line
+  // code measures ~4% faster than a grid-stride loop around the same body
+  const uint64_t nspan = nfull / kSpan;
+  const uint64_t b = blockIdx.x;
+  if (b < nspan)
+    reduce_block<true, T, Pol, VEC, UPT, INFL, MAXLV>(
+        g, dest_off, src_off, n, lg, b * kSpan, (T *)k_reduce_lds);
+  else if (b * kSpan < nunits)
+    reduce_tail<T, Pol, VEC, UPT, MAXLV>(g, dest_off, src_off, n, lg,
+                                         b * kSpan, (T *)k_reduce_lds);
 }
 
 // ---- host-side launch ------------------------------------------------------
@@ -584,8 +645,9 @@ static void launch_shape(const oscd_geom *g, int lg, uint64_t doff,
                          hipStream_t s) {
   const uint64_t units = (n + VEC - 1) / VEC;
   const uint64_t span = (uint64_t)kBlock * UPT;
-  uint64_t grid = (units + span - 1) / span;
-  if (grid > (1u << 20)) grid = 1u << 20;
+  // every span gets its own workgroup; launch() bounds n so that the grid
+  // stays within 2^24 workgroups (2^32 threads)
+  const uint64_t grid = (units + span - 1) / span;
   if constexpr (Pol::TREE) {
     if (lg >= 8) {
       hipLaunchKernelGGL((k_reduce<T, Pol, VEC, UPT, INFL, 13>),
@@ -599,12 +661,13 @@ static void launch_shape(const oscd_geom *g, int lg, uint64_t doff,
                      soff, n, lg);
 }
 
-// Default shape per policy: wide for the register-light order-free kernels,
-// narrower where the accumulator (double-double, reference tree stack)
-// takes registers.
+// Default shape per policy (Pol::SHAPE), from the 64-PE x 4M sweeps on
+// MI355X: 8x2 for the order-free kernels, 8x1 for FP32 accumulated in FP64
+// (its FP64 accumulators make 8x2 too register-heavy), 2x4 for the
+// reference tree (its subtree stack takes registers).
 template <class Pol>
 constexpr int default_shape() {
-  return Pol::TREE ? 2 : 0;
+  return Pol::SHAPE;
 }
 // The reference tree only runs the narrow shapes (its subtree stack takes
 // MAXLV*UPT*VEC registers).
@@ -624,6 +687,10 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
                       (g->slice % AL == 0) && ((uintptr_t)g->base % AL == 0);
   const size_t lds = Pol::SPECIAL ? (size_t)g->npes * sizeof(T) : 0;
   if (n == 0) return OSC_OK;
+  // one workgroup per span of >= 256 units: at most 2^32 units keeps the
+  // grid within 2^24 workgroups (the reference itself takes nreduce < 2^31)
+  if ((n + (vec_ok ? VB : 1) - 1) / (vec_ok ? VB : 1) > (1ull << 32))
+    return OSC_ERR_ARG;
   shape = clamp_shape<Pol>(shape);
   if (!vec_ok) {
     launch_shape<T, Pol, 1, 1, 8>(g, lg, doff, soff, n, lds, s);
@@ -666,14 +733,15 @@ __global__ __launch_bounds__(kBlock) void k_batch_reduce(
     g.pad_ = 0;
     const uint64_t n = c.count;
     const uint64_t nfull = n / VEC;
-    for (uint64_t ub = tl.begin; ub < tl.end; ub += kSpan) {
-      if (ub + kSpan <= nfull)
-        reduce_block<true, T, Pol, VEC, UPT, INFL, MAXLV>(
-            g, c.dest_off, c.src_off, n, c.lg, ub, (T *)k_batch_lds);
-      else
-        reduce_block<false, T, Pol, VEC, UPT, INFL, MAXLV>(
-            g, c.dest_off, c.src_off, n, c.lg, ub, (T *)k_batch_lds);
-    }
+    // tiles start on span boundaries; only a call's last span can be
+    // ragged, and it ends the tile that holds it
+    uint64_t ub = tl.begin;
+    for (; ub < tl.end && ub + kSpan <= nfull; ub += kSpan)
+      reduce_block<true, T, Pol, VEC, UPT, INFL, MAXLV>(
+          g, c.dest_off, c.src_off, n, c.lg, ub, (T *)k_batch_lds);
+    if (ub < tl.end)
+      reduce_tail<T, Pol, VEC, UPT, MAXLV>(g, c.dest_off, c.src_off, n, c.lg,
+                                           ub, (T *)k_batch_lds);
   }
 }
 

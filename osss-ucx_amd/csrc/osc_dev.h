@@ -58,7 +58,7 @@ const char *oscd_last_hip_error(void);
 /* Team reduction of n elements of `dtype` with `op`; every member's
  * dest_off gets the result.  fpmode: 0 accurate, 1 reference order.
  * shape: kernel shape index 0..3 (units per thread x slices in flight:
- * 8x2, 4x4, 2x4, 1x8), or -1 for the per-policy default. */
+ * 8x2, 8x1, 2x4, 1x8), or -1 for the per-policy default. */
 int oscd_reduce(const oscd_geom *g, int op, int dtype, int fpmode, int shape,
                 uint64_t dest_off, uint64_t src_off, uint64_t n, void *stream);
 
