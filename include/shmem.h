@@ -1,12 +1,15 @@
 /*
  * shmem.h - the OpenSHMEM 1.5 entry points this library exports.
  *
- * Only the collective hot path and what callers need to place data for it:
- * init/finalize, PE queries, the symmetric heap, teams, the typed team
- * reductions, fcollect, alltoall and broadcast, plus put/get between PEs.
- * Names, argument order and meaning match the OpenSHMEM specification and
- * the reference's declarations (include/shmem/api.h:3079-3260 reductions,
- * :3449-3467 fcollect, :3513-3530 alltoall, :3325-3344 broadcast;
+ * The collective hot path and what callers need around it: init/finalize,
+ * PE queries, the symmetric heap, teams, the typed team reductions,
+ * fcollect, alltoall, broadcast, collect and alltoalls, the pre-1.5
+ * active-set forms, and RMA between PEs (typed, sized and strided put/get,
+ * put-with-signal).  Names, argument order and meaning match the OpenSHMEM
+ * specification and the reference's declarations (include/shmem/api.h:
+ * 3079-3260 reductions, :3449-3467 fcollect, :3513-3530 alltoall,
+ * :3325-3344 broadcast, :3395-3415 collect, :3565-3587 alltoalls,
+ * :3262-3609 active-set forms, :403-1021 RMA and signals;
  * include/shmem/teams.h:16-129 teams), so a program written against the
  * reference links against libosc.so unchanged for these calls.
  *
