@@ -118,6 +118,20 @@ int osc_last_error_code(void); /* code of the calling thread's last error */
 void osc_clear_error(void);
 int osc_set_my_pe(int pe); /* which emulated PE the calling thread is */
 
+/* ---- SPMD calling mode ----------------------------------------------------
+ * osc_spmd_run initialises the library (npes PEs of heap_bytes_per_pe on
+ * `device`), runs pe_main(pe, arg) on one host thread per PE, waits for all
+ * of them and finalises.  While it runs, the OpenSHMEM entry points follow
+ * SPMD semantics: every member calls each collective, heap call, barrier
+ * and team split with its own copy of the arguments; the last member to
+ * arrive launches once for all, and each member's nelems of collect is its
+ * own contribution.  Returns 0, the first nonzero pe_main result, or an
+ * OSC_ERR_* code.  Symmetric memory is device memory: move data between
+ * host variables and symmetric objects with shmem_putmem/getmem. */
+int osc_spmd_run(int npes, size_t heap_bytes_per_pe, int device,
+                 int (*pe_main)(int pe, void *arg), void *arg);
+int osc_spmd_active(void); /* 1 inside osc_spmd_run */
+
 /* ---- symmetric addresses ------------------------------------------------ */
 /* Split a symmetric address into (pe, offset).  Returns OSC_ERR_PTR if the
  * address is outside the heap. */

@@ -43,6 +43,7 @@ typedef struct osc_state {
   osc_block *blocks; /* sorted by offset, covers [0, heap_bytes) */
   pthread_mutex_t lock;
   int log_calls;
+  int spmd; /* 1 while osc_spmd_run drives one host thread per PE */
   /* resolved algorithm choice: float mode per (op, dtype) */
   unsigned char fpmode[OSC_OP_COUNT][OSC_T_COUNT];
   int fp_default;
@@ -91,6 +92,44 @@ int osc_tune_bucket(uint64_t bytes_per_pe);
 
 /* symmetric-argument validation (osc_coll.c) */
 int osc_sym_range(const void *p, size_t bytes, size_t *off);
+
+/* symmetric-heap operations on offsets, for every PE at once
+ * (shmem_api.c); zero: clear the new block on every PE */
+int osc_heap_call_alloc(size_t size, size_t align, int zero, size_t *off);
+int osc_heap_call_free(size_t off);
+int osc_heap_call_realloc(size_t off, size_t size, size_t *new_off);
+
+/* SPMD calling mode (osc_spmd.c): every member thread calls; the last to
+ * arrive runs the team-call path once for all and the others wait for it.
+ * Kinds for spmd_move: */
+enum { OSC_SPMD_FCOLLECT = 1, OSC_SPMD_ALLTOALL, OSC_SPMD_BROADCAST };
+int osc_spmd_reduce(void *team, int op, int dtype, void *dest,
+                    const void *src, size_t n);
+int osc_spmd_move(int kind, void *team, void *dest, const void *src,
+                  size_t nbytes, int root, int skip_root);
+int osc_spmd_collect(void *team, void *dest, const void *src,
+                     size_t my_nbytes);
+int osc_spmd_alltoalls(void *team, void *dest, const void *src, ptrdiff_t dst,
+                       ptrdiff_t sst, size_t elem_bytes, size_t nelems);
+int osc_spmd_sync(void *team);
+void *osc_spmd_heap(int kind, void *ptr, size_t a, size_t b);
+enum { OSC_SPMD_MALLOC = 1, OSC_SPMD_CALLOC, OSC_SPMD_ALIGN, OSC_SPMD_REALLOC,
+       OSC_SPMD_FREE };
+int osc_spmd_split_strided(void *parent, int start, int stride, int size,
+                           const void *config, long config_mask,
+                           void **new_team);
+int osc_spmd_split_2d(void *parent, int xrange, const void *xconfig,
+                      long xmask, void **xteam, const void *yconfig,
+                      long ymask, void **yteam);
+void osc_spmd_team_destroy(void *team);
+/* team-call bodies the SPMD leader runs (osc_team.c) */
+int osc_team_split_strided_tc(void *parent, int start, int stride, int size,
+                              const void *config, long config_mask,
+                              void **new_team);
+int osc_team_rank(const osc_team *t, int pe);
+int osc_team_split_2d_all(const osc_team *p, int xrange, const void *xc,
+                          long xm, const void *yc, long ym, osc_team **rows,
+                          osc_team **cols);
 
 /* Active set (PE_start, 2^logPE_stride, PE_size) of the pre-1.5 API as a
  * transient team (not registered; valid while `t` lives). */

@@ -39,12 +39,13 @@ void osc_team_geom(const osc_team *t, oscd_geom *g) {
 }
 
 /* rank of global PE `pe` in team t, or -1 */
-static int rank_of(const osc_team *t, int pe) {
+int osc_team_rank(const osc_team *t, int pe) {
   int d = pe - t->start;
   if (d < 0 || d % t->stride != 0) return -1;
   d /= t->stride;
   return d < t->npes ? d : -1;
 }
+#define rank_of osc_team_rank
 
 int shmem_team_my_pe(shmem_team_t team) {
   const osc_team *t = osc_team_get(team);
@@ -89,11 +90,23 @@ static osc_team *new_team(const osc_team *parent, int start, int stride,
 }
 
 /* Team-call semantics: the host creates the team once for all of its
- * members and gets a valid handle (in an SPMD library only members get
- * one, others get SHMEM_TEAM_INVALID). */
+ * members and gets a valid handle.  In the SPMD mode every member of the
+ * parent calls, the team is created once, and PEs outside it get
+ * SHMEM_TEAM_INVALID (reference src/shmemc/teams.c:489-555). */
 int shmem_team_split_strided(shmem_team_t parent_team, int start, int stride,
                              int size, const shmem_team_config_t *config,
                              long config_mask, shmem_team_t *new_team_out) {
+  if (osc_g.spmd)
+    return osc_spmd_split_strided(parent_team, start, stride, size, config,
+                                  config_mask, new_team_out);
+  return osc_team_split_strided_tc(parent_team, start, stride, size, config,
+                                   config_mask, new_team_out);
+}
+
+int osc_team_split_strided_tc(void *parent_team, int start, int stride,
+                              int size, const void *config_v,
+                              long config_mask, void **new_team_out) {
+  const shmem_team_config_t *config = (const shmem_team_config_t *)config_v;
   const osc_team *p = osc_team_get(parent_team);
   osc_team *t;
   if (new_team_out == NULL) return -1;
@@ -120,6 +133,10 @@ int shmem_team_split_2d(shmem_team_t parent_team, int xrange,
   const osc_team *p = osc_team_get(parent_team);
   int me, row, col, xs, ys;
   osc_team *x, *y;
+  if (osc_g.spmd)
+    return osc_spmd_split_2d(parent_team, xrange, xaxis_config, xaxis_mask,
+                             xaxis_team, yaxis_config, yaxis_mask,
+                             yaxis_team);
   if (xaxis_team == NULL || yaxis_team == NULL) return -1;
   *xaxis_team = *yaxis_team = SHMEM_TEAM_INVALID;
   if (p == NULL || xrange < 1) return osc_set_error(OSC_ERR_ARG, "bad 2d split");
@@ -145,6 +162,39 @@ int shmem_team_split_2d(shmem_team_t parent_team, int xrange,
 void shmem_team_destroy(shmem_team_t team) {
   osc_team *t = (osc_team *)team;
   if (osc_team_get(team) == NULL || t->predefined) return;
+  if (osc_g.spmd) {
+    osc_spmd_team_destroy(team);
+    return;
+  }
   t->magic = 0;
   free(t);
+}
+
+/* Every row team (PEs row*xrange .. of the parent) and every column team
+ * (col, col + xrange, ...) of a 2-d split, for the SPMD leader: rows has
+ * ceil(P / xrange) entries, cols min(xrange, P). */
+int osc_team_split_2d_all(const osc_team *p, int xrange, const void *xc,
+                          long xm, const void *yc, long ym, osc_team **rows,
+                          osc_team **cols) {
+  const int P = p->npes;
+  const int nrows = (P + xrange - 1) / xrange;
+  const int ncols = xrange < P ? xrange : P;
+  int r, c;
+  for (r = 0; r < nrows; ++r) rows[r] = NULL;
+  for (c = 0; c < ncols; ++c) cols[c] = NULL;
+  for (r = 0; r < nrows; ++r) {
+    const int xs = (r + 1) * xrange <= P ? xrange : P - r * xrange;
+    rows[r] = new_team(p, r * xrange, 1, xs, (const shmem_team_config_t *)xc, xm);
+    if (rows[r] == NULL) goto fail;
+  }
+  for (c = 0; c < ncols; ++c) {
+    const int ys = (P - c + xrange - 1) / xrange;
+    cols[c] = new_team(p, c, xrange, ys, (const shmem_team_config_t *)yc, ym);
+    if (cols[c] == NULL) goto fail;
+  }
+  return OSC_OK;
+fail:
+  for (r = 0; r < nrows; ++r) free(rows[r]);
+  for (c = 0; c < ncols; ++c) free(cols[c]);
+  return osc_set_error(OSC_ERR_NOMEM, "team allocation");
 }

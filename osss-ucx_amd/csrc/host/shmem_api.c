@@ -61,6 +61,10 @@ void shmem_init(void) {
   size_t heap = (size_t)32 << 20; /* reference default, configure.ac:23 */
   int npes = np ? atoi(np) : 8;
   int rc;
+  if (osc_g.spmd) { /* the launcher initialised; every PE meets here */
+    osc_spmd_sync(SHMEM_TEAM_WORLD);
+    return;
+  }
   if (osc_g.initialized) return;
   if (hs != NULL && osc_parse_size(hs, &heap) != OSC_OK) {
     fprintf(stderr, "shmem_init: bad SHMEM_SYMMETRIC_SIZE=%s\n", hs);
@@ -73,18 +77,29 @@ void shmem_init(void) {
   }
 }
 
-void shmem_finalize(void) { osc_finalize(); }
+/* In the SPMD mode the launcher finalises after every PE has returned. */
+void shmem_finalize(void) {
+  if (osc_g.spmd) {
+    osc_spmd_sync(SHMEM_TEAM_WORLD);
+    return;
+  }
+  osc_finalize();
+}
 
 int shmem_my_pe(void) { return osc_g.initialized ? osc_my_pe_get() : -1; }
 int shmem_n_pes(void) { return osc_g.initialized ? osc_g.npes : -1; }
 
 /* One host thread acts for all PEs, so a barrier is completion of all
- * enqueued device work. */
+ * enqueued device work; in the SPMD mode the members also meet. */
 void shmem_barrier_all(void) {
-  if (osc_g.initialized) oscd_device_sync();
+  if (osc_g.spmd)
+    osc_spmd_sync(SHMEM_TEAM_WORLD);
+  else if (osc_g.initialized)
+    oscd_device_sync();
 }
 int shmem_team_sync(shmem_team_t team) {
   if (osc_team_get(team) == NULL) return -1;
+  if (osc_g.spmd) return osc_spmd_sync(team) == OSC_OK ? 0 : -1;
   return oscd_device_sync() == OSC_OK ? 0 : -1;
 }
 void shmem_quiet(void) {
@@ -101,79 +116,101 @@ void shmem_info_get_name(char *name) {
 }
 
 /* ---- symmetric heap ---------------------------------------------------------- */
+/* The offset-level operations act for every PE at once; the entry points
+ * turn the offset into the calling PE's address.  In the SPMD mode every
+ * PE calls and one of them runs the operation (osc_spmd_heap). */
 
-static void *heap_alloc(size_t size, size_t align) {
-  size_t off;
-  int rc;
-  if (!osc_g.initialized) return NULL;
+int osc_heap_call_alloc(size_t size, size_t align, int zero, size_t *off) {
+  int rc, pe;
+  if (!osc_g.initialized) return osc_set_error(OSC_ERR_NOTINIT, "not initialised");
   pthread_mutex_lock(&osc_g.lock);
-  rc = osc_heap_alloc(size, align, &off);
+  rc = osc_heap_alloc(size, align, off);
   pthread_mutex_unlock(&osc_g.lock);
-  if (rc != OSC_OK) {
-    osc_set_error(rc, "symmetric heap exhausted (%zu bytes requested)", size);
-    return NULL;
+  if (rc != OSC_OK)
+    return osc_set_error(rc, "symmetric heap exhausted (%zu bytes requested)", size);
+  if (zero && size > 0) {
+    for (pe = 0; pe < osc_g.npes; ++pe)
+      oscd_memset_async(osc_g.base + (size_t)pe * osc_g.slice + *off, 0, size,
+                        osc_g.stream);
+    rc = oscd_stream_sync(osc_g.stream);
+    if (rc != OSC_OK) return osc_hip_error(rc, "calloc clear");
   }
+  return OSC_OK;
+}
+
+int osc_heap_call_free(size_t off) {
+  int rc;
+  /* outstanding kernels may still use the object */
+  oscd_stream_sync(osc_g.stream);
+  pthread_mutex_lock(&osc_g.lock);
+  rc = osc_heap_free(off);
+  pthread_mutex_unlock(&osc_g.lock);
+  return rc;
+}
+
+int osc_heap_call_realloc(size_t off, size_t size, size_t *new_off) {
+  size_t old, keep;
+  int pe, rc;
+  pthread_mutex_lock(&osc_g.lock);
+  rc = osc_heap_block_size(off, &old);
+  pthread_mutex_unlock(&osc_g.lock);
+  if (rc != OSC_OK) return osc_set_error(rc, "realloc of a non-heap block");
+  rc = osc_heap_call_alloc(size, 0, 0, new_off);
+  if (rc != OSC_OK) return rc;
+  keep = old < size ? old : size;
+  for (pe = 0; pe < osc_g.npes && keep > 0; ++pe)
+    oscd_copy(osc_g.base + (size_t)pe * osc_g.slice + *new_off,
+              osc_g.base + (size_t)pe * osc_g.slice + off, keep, osc_g.stream);
+  return osc_heap_call_free(off);
+}
+
+static void *my_addr(size_t off) {
   return osc_g.base + (size_t)osc_my_pe_get() * osc_g.slice + off;
 }
 
-void *shmem_malloc(size_t size) { return heap_alloc(size, 0); }
+void *shmem_malloc(size_t size) {
+  size_t off;
+  if (osc_g.spmd) return osc_spmd_heap(OSC_SPMD_MALLOC, NULL, size, 0);
+  return osc_heap_call_alloc(size, 0, 0, &off) == OSC_OK ? my_addr(off) : NULL;
+}
 
 void *shmem_align(size_t alignment, size_t size) {
+  size_t off;
   if (alignment == 0 || (alignment & (alignment - 1)) != 0) return NULL;
-  return heap_alloc(size, alignment);
+  if (osc_g.spmd) return osc_spmd_heap(OSC_SPMD_ALIGN, NULL, size, alignment);
+  return osc_heap_call_alloc(size, alignment, 0, &off) == OSC_OK ? my_addr(off)
+                                                                : NULL;
 }
 
 void *shmem_calloc(size_t count, size_t size) {
-  char *p;
-  size_t off, bytes;
-  int pe;
+  size_t off;
   if (size != 0 && count > ((size_t)-1) / size) return NULL;
-  bytes = count * size;
-  p = (char *)heap_alloc(bytes, 0);
-  if (p == NULL) return NULL;
-  osc_sym_offset(p, NULL, &off);
-  for (pe = 0; pe < osc_g.npes; ++pe)
-    oscd_memset_async(osc_g.base + (size_t)pe * osc_g.slice + off, 0, bytes,
-                      osc_g.stream);
-  oscd_stream_sync(osc_g.stream);
-  return p;
+  if (osc_g.spmd) return osc_spmd_heap(OSC_SPMD_CALLOC, NULL, count * size, 0);
+  return osc_heap_call_alloc(count * size, 0, 1, &off) == OSC_OK ? my_addr(off)
+                                                                 : NULL;
 }
 
 void shmem_free(void *ptr) {
   size_t off;
+  if (osc_g.spmd) {
+    osc_spmd_heap(OSC_SPMD_FREE, ptr, 0, 0);
+    return;
+  }
   if (ptr == NULL || osc_sym_offset(ptr, NULL, &off) != OSC_OK) return;
-  /* outstanding kernels may still use the object */
-  oscd_stream_sync(osc_g.stream);
-  pthread_mutex_lock(&osc_g.lock);
-  osc_heap_free(off);
-  pthread_mutex_unlock(&osc_g.lock);
+  osc_heap_call_free(off);
 }
 
 void *shmem_realloc(void *ptr, size_t size) {
-  size_t off, old;
-  char *np;
-  int pe, rc;
+  size_t off, noff;
+  if (osc_g.spmd) return osc_spmd_heap(OSC_SPMD_REALLOC, ptr, size, 0);
   if (ptr == NULL) return shmem_malloc(size);
   if (size == 0) {
     shmem_free(ptr);
     return NULL;
   }
   if (osc_sym_offset(ptr, NULL, &off) != OSC_OK) return NULL;
-  pthread_mutex_lock(&osc_g.lock);
-  rc = osc_heap_block_size(off, &old);
-  pthread_mutex_unlock(&osc_g.lock);
-  if (rc != OSC_OK) return NULL;
-  np = (char *)shmem_malloc(size);
-  if (np == NULL) return NULL;
-  {
-    size_t noff, keep = old < size ? old : size;
-    osc_sym_offset(np, NULL, &noff);
-    for (pe = 0; pe < osc_g.npes; ++pe)
-      oscd_copy(osc_g.base + (size_t)pe * osc_g.slice + noff,
-                osc_g.base + (size_t)pe * osc_g.slice + off, keep, osc_g.stream);
-  }
-  shmem_free(ptr);
-  return np;
+  return osc_heap_call_realloc(off, size, &noff) == OSC_OK ? my_addr(noff)
+                                                           : NULL;
 }
 
 void *shmem_ptr(const void *dest, int pe) { return osc_pe_ptr(dest, pe); }
@@ -486,9 +523,35 @@ static int finish(int rc) {
 
 static int blocking_reduce(shmem_team_t team, int op, int dtype, void *dest,
                            const void *src, size_t n) {
+  if (osc_g.spmd) /* every member arrives, dtype < 0 included */
+    return osc_spmd_reduce(team, op, dtype, dest, src, n);
   if (dtype < 0)
     return osc_set_error(OSC_ERR_UNSUPPORTED, "long double reduction (no 80-bit type on gfx950)");
   return finish(osc_reduce(team, op, dtype, dest, src, n, NULL));
+}
+
+/* blocking data movement, team-call or SPMD */
+static int do_fcollect(void *team, void *dest, const void *src, size_t nb) {
+  if (osc_g.spmd)
+    return osc_spmd_move(OSC_SPMD_FCOLLECT, team, dest, src, nb, 0, 0);
+  return finish(osc_fcollect(team, dest, src, nb, NULL));
+}
+static int do_alltoall(void *team, void *dest, const void *src, size_t nb) {
+  if (osc_g.spmd)
+    return osc_spmd_move(OSC_SPMD_ALLTOALL, team, dest, src, nb, 0, 0);
+  return finish(osc_alltoall(team, dest, src, nb, NULL));
+}
+static int do_broadcast(void *team, void *dest, const void *src, size_t nb,
+                        int root, int skip_root) {
+  if (osc_g.spmd)
+    return osc_spmd_move(OSC_SPMD_BROADCAST, team, dest, src, nb, root,
+                         skip_root);
+  return finish(osc_broadcast_impl(team, dest, src, nb, root, skip_root, NULL));
+}
+static int do_alltoalls(void *team, void *dest, const void *src, ptrdiff_t dst,
+                        ptrdiff_t sst, size_t es, size_t n) {
+  if (osc_g.spmd) return osc_spmd_alltoalls(team, dest, src, dst, sst, es, n);
+  return finish(osc_alltoalls(team, dest, src, dst, sst, es, n, NULL));
 }
 
 #define OSC_DEF_REDUCE(_type, _typename, _op, _OPC)                            \
@@ -515,33 +578,31 @@ OSC_ARITH_TYPES(OSC_DEF_PROD)
 #define OSC_DEF_MOVE(_type, _typename)                                         \
   int shmem_##_typename##_fcollect(shmem_team_t team, _type *dest,             \
                                    const _type *source, size_t nelems) {       \
-    return finish(osc_fcollect(team, dest, source, nelems * sizeof(_type),     \
-                               NULL));                                         \
+    return do_fcollect(team, dest, source, nelems * sizeof(_type));            \
   }                                                                            \
   int shmem_##_typename##_alltoall(shmem_team_t team, _type *dest,             \
                                    const _type *source, size_t nelems) {       \
-    return finish(osc_alltoall(team, dest, source, nelems * sizeof(_type),     \
-                               NULL));                                         \
+    return do_alltoall(team, dest, source, nelems * sizeof(_type));            \
   }                                                                            \
   int shmem_##_typename##_broadcast(shmem_team_t team, _type *dest,            \
                                     const _type *source, size_t nelems,        \
                                     int PE_root) {                             \
-    return finish(osc_broadcast(team, dest, source, nelems * sizeof(_type),    \
-                                PE_root, NULL));                               \
+    return do_broadcast(team, dest, source, nelems * sizeof(_type), PE_root,   \
+                        0);                                                    \
   }
 OSC_RMA_TYPES(OSC_DEF_MOVE)
 
 int shmem_fcollectmem(shmem_team_t team, void *dest, const void *source,
                       size_t nbytes) {
-  return finish(osc_fcollect(team, dest, source, nbytes, NULL));
+  return do_fcollect(team, dest, source, nbytes);
 }
 int shmem_alltoallmem(shmem_team_t team, void *dest, const void *source,
                       size_t nbytes) {
-  return finish(osc_alltoall(team, dest, source, nbytes, NULL));
+  return do_alltoall(team, dest, source, nbytes);
 }
 int shmem_broadcastmem(shmem_team_t team, void *dest, const void *source,
                        size_t nbytes, int PE_root) {
-  return finish(osc_broadcast(team, dest, source, nbytes, PE_root, NULL));
+  return do_broadcast(team, dest, source, nbytes, PE_root, 0);
 }
 
 /* collect: in team-call mode nelems is every member's count; the
@@ -551,6 +612,8 @@ static int uniform_collect(shmem_team_t team, void *dest, const void *source,
   const osc_team *t = osc_team_get(team);
   size_t *sizes;
   int i, rc;
+  /* SPMD: nbytes is this member's own contribution (the variable case) */
+  if (osc_g.spmd) return osc_spmd_collect(team, dest, source, nbytes);
   if (!osc_g.initialized) return osc_set_error(OSC_ERR_NOTINIT, "not initialised");
   if (t == NULL) return osc_set_error(OSC_ERR_TEAM, "invalid team");
   sizes = (size_t *)malloc(sizeof(size_t) * (size_t)t->npes);
@@ -571,8 +634,7 @@ static int uniform_collect(shmem_team_t team, void *dest, const void *source,
   int shmem_##_typename##_alltoalls(shmem_team_t team, _type *dest,            \
                                     const _type *source, ptrdiff_t dst,        \
                                     ptrdiff_t sst, size_t nelems) {            \
-    return finish(osc_alltoalls(team, dest, source, dst, sst, sizeof(_type),   \
-                                nelems, NULL));                                \
+    return do_alltoalls(team, dest, source, dst, sst, sizeof(_type), nelems);  \
   }
 OSC_RMA_TYPES(OSC_DEF_MOVE2)
 
@@ -583,7 +645,7 @@ int shmem_collectmem(shmem_team_t team, void *dest, const void *source,
 /* one element of nelems bytes per member pair (alltoalls.c:391-428) */
 int shmem_alltoallsmem(shmem_team_t team, void *dest, const void *source,
                        ptrdiff_t dst, ptrdiff_t sst, size_t nelems) {
-  return finish(osc_alltoalls(team, dest, source, dst, sst, nelems, 1, NULL));
+  return do_alltoalls(team, dest, source, dst, sst, nelems, 1);
 }
 
 /* ---- active-set forms (deprecated in 1.5) ------------------------------------- */
@@ -596,6 +658,12 @@ static void legacy_reduce(int op, int dtype, void *target, const void *source,
                           int nreduce, int PE_start, int logPE_stride,
                           int PE_size) {
   osc_team t;
+  if (osc_g.spmd) {
+    if (osc_active_set(PE_start, logPE_stride, PE_size, &t) != OSC_OK) return;
+    osc_spmd_reduce(&t, op, dtype, target, source,
+                    nreduce < 0 ? (size_t)-1 : (size_t)nreduce);
+    return;
+  }
   if (dtype < 0) {
     osc_set_error(OSC_ERR_UNSUPPORTED, "long double reduction (no 80-bit type on gfx950)");
     return;
@@ -649,7 +717,7 @@ static int sized_bytes(size_t nelems, size_t es, size_t *out) {
     if (osc_active_set(PE_start, logPE_stride, PE_size, &t) != OSC_OK ||       \
         sized_bytes(nelems, (_size) / 8, &nb) != OSC_OK)                       \
       return;                                                                  \
-    finish(osc_fcollect(&t, target, source, nb, NULL));                        \
+    do_fcollect(&t, target, source, nb);                                       \
   }                                                                            \
   void shmem_collect##_size(void *target, const void *source, size_t nelems,  \
                             int PE_start, int logPE_stride, int PE_size,      \
@@ -671,7 +739,7 @@ static int sized_bytes(size_t nelems, size_t es, size_t *out) {
     if (osc_active_set(PE_start, logPE_stride, PE_size, &t) != OSC_OK ||       \
         sized_bytes(nelems, (_size) / 8, &nb) != OSC_OK)                       \
       return;                                                                  \
-    finish(osc_alltoall(&t, target, source, nb, NULL));                        \
+    do_alltoall(&t, target, source, nb);                                       \
   }                                                                            \
   void shmem_alltoalls##_size(void *target, const void *source,               \
                               ptrdiff_t dst, ptrdiff_t sst, size_t nelems,    \
@@ -680,8 +748,7 @@ static int sized_bytes(size_t nelems, size_t es, size_t *out) {
     osc_team t;                                                                \
     (void)pSync;                                                               \
     if (osc_active_set(PE_start, logPE_stride, PE_size, &t) != OSC_OK) return; \
-    finish(osc_alltoalls(&t, target, source, dst, sst, (_size) / 8, nelems,    \
-                         NULL));                                               \
+    do_alltoalls(&t, target, source, dst, sst, (_size) / 8, nelems);           \
   }                                                                            \
   void shmem_broadcast##_size(void *target, const void *source,               \
                               size_t nelems, int PE_root, int PE_start,       \
@@ -692,7 +759,7 @@ static int sized_bytes(size_t nelems, size_t es, size_t *out) {
     if (osc_active_set(PE_start, logPE_stride, PE_size, &t) != OSC_OK ||       \
         sized_bytes(nelems, (_size) / 8, &nb) != OSC_OK)                       \
       return;                                                                  \
-    finish(osc_broadcast_impl(&t, target, source, nb, PE_root, 1, NULL));      \
+    do_broadcast(&t, target, source, nb, PE_root, 1);                          \
   }
 OSC_DEF_SIZED_COLL(32)
 OSC_DEF_SIZED_COLL(64)
@@ -704,6 +771,10 @@ void shmem_barrier(int PE_start, int logPE_stride, int PE_size, long *pSync) {
   int rc;
   (void)pSync;
   if (osc_active_set(PE_start, logPE_stride, PE_size, &t) != OSC_OK) return;
+  if (osc_g.spmd) {
+    osc_spmd_sync(&t);
+    return;
+  }
   rc = oscd_device_sync();
   if (rc != OSC_OK) osc_hip_error(rc, "barrier");
 }
@@ -712,5 +783,8 @@ void shmem_sync_deprecated(int PE_start, int logPE_stride, int PE_size,
   shmem_barrier(PE_start, logPE_stride, PE_size, pSync);
 }
 void shmem_sync_all(void) {
-  if (osc_g.initialized) oscd_device_sync();
+  if (osc_g.spmd)
+    osc_spmd_sync(SHMEM_TEAM_WORLD);
+  else if (osc_g.initialized)
+    oscd_device_sync();
 }
