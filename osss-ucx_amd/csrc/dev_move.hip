@@ -71,6 +71,39 @@ __global__ __launch_bounds__(kMoveBlock) void k_gather(
   }
 }
 
+// fcollect with small blocks (nunits < 1024 per source): each destination's
+// region of P*nunits units is contiguous (block i at i*nunits), so it is cut
+// into chunks of 256*UPT units regardless of source boundaries; unit u of
+// the region comes from source u / nunits.  Virtual block vb -> destination
+// vb % P, chunk vb / P.  This keeps every lane busy where per-source chunks
+// would leave most of a workgroup idle.  Requires P*nunits < 2^32.
+template <int W, int UPT>
+__global__ __launch_bounds__(kMoveBlock) void k_gather_flat(
+    const oscd_geom g, const uint64_t soff, const uint64_t doff,
+    const uint32_t nunits, const uint64_t total) {
+  const uint64_t P = (uint64_t)g.npes;
+  const uint32_t region = (uint32_t)P * nunits;
+  for (uint64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const int j = (int)(vb % P);
+    const uint32_t c0 = (uint32_t)(vb / P) * (uint32_t)(kMoveBlock * UPT);
+    char *dst = member_ptr(g, j, doff);
+    typename Raw<W>::type v[UPT];
+#pragma unroll
+    for (int k = 0; k < UPT; ++k) {
+      const uint32_t u = c0 + threadIdx.x + (uint32_t)k * kMoveBlock;
+      if (u < region) {
+        const uint32_t i = u / nunits;
+        v[k] = ld<W>(member_ptr(g, (int)i, soff) + (uint64_t)(u - i * nunits) * W);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < UPT; ++k) {
+      const uint32_t u = c0 + threadIdx.x + (uint32_t)k * kMoveBlock;
+      if (u < region) st_nt<W>(dst + (uint64_t)u * W, v[k]);
+    }
+  }
+}
+
 // Collect with per-member sizes: member j's [doff + off_i, + n_i) =
 // member i's [soff, + n_i); tab[2i] = off_i, tab[2i+1] = n_i (bytes).
 // Same virtual-block order as k_gather, with cps chunks per source sized
@@ -318,6 +351,17 @@ static int gather(const oscd_geom *g, int src_member, uint64_t soff,
   const int W = alignment_of(g, doff | doff_step, soff, nbytes);
   const uint64_t nunits = nbytes / W;
   const bool wide = nunits >= 4 * kMoveBlock;
+  if (src_member < 0 && !wide && skip < 0 && doff_step == nbytes) {
+    // small fcollect blocks: chunk each destination's contiguous region
+    const uint64_t region = (uint64_t)g->npes * nunits;
+    const uint64_t chunks = (region + 4 * kMoveBlock - 1) / (4 * kMoveBlock);
+    const uint64_t total = (uint64_t)g->npes * chunks;
+    const unsigned grid = (unsigned)(total < (1u << 22) ? total : (1u << 22));
+    OSC_W_DISPATCH(W, hipLaunchKernelGGL((k_gather_flat<WW, 4>), dim3(grid),
+                                         dim3(kMoveBlock), 0, s, *g, soff,
+                                         doff, (uint32_t)nunits, total));
+    return check_launch();
+  }
   const uint64_t per = wide ? 4 * kMoveBlock : kMoveBlock;
   const uint64_t cps = (nunits + per - 1) / per;
   const uint64_t total = (uint64_t)g->npes * nsrc * cps;

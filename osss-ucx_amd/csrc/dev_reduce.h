@@ -591,10 +591,11 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
 }
 
 // The ragged last block-iteration of a call, [ub, min(ub + 256*UPT,
-// units)): 256 units at a time through the one-unit-per-thread body with
-// its guards.  Keeping the guarded code out of the UPT-wide body keeps the
-// kernel's register footprint at that of the guard-free body, which sets
-// how many workgroups stay resident per CU.
+// units)): 256 units at a time through the guarded one-unit-per-thread
+// body with 8 slices in flight.  Keeping the guarded code out of the
+// UPT-wide body keeps the kernel's register footprint at that of the
+// guard-free body, which sets how many workgroups stay resident per CU.
+// (k_reduce gives each 256-unit piece its own workgroup instead.)
 template <class T, class Pol, int VEC, int UPT, int MAXLV>
 __device__ __forceinline__ void reduce_tail(const oscd_geom &g,
                                             const uint64_t dest_off,
@@ -605,7 +606,7 @@ __device__ __forceinline__ void reduce_tail(const oscd_geom &g,
   const uint64_t span_end = ub + (uint64_t)kBlock * UPT;
   const uint64_t end = span_end < nunits ? span_end : nunits;
   for (uint64_t sub = ub; sub < end; sub += kBlock)
-    reduce_block<false, T, Pol, VEC, 1, 1, MAXLV>(g, dest_off, src_off, n, lg,
+    reduce_block<false, T, Pol, VEC, 1, 8, MAXLV>(g, dest_off, src_off, n, lg,
                                                   sub, lds);
 }
 
@@ -621,16 +622,21 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const oscd_geom g,
   constexpr uint64_t kSpan = (uint64_t)kBlock * UPT;
   const uint64_t nunits = (n + VEC - 1) / VEC;
   const uint64_t nfull = n / VEC;
-  // one span per workgroup (the grid covers every span): straight-line
-  // code measures ~4% faster than a grid-stride loop around the same body
+  // one full span per workgroup, then one workgroup per 256-unit piece of
+  // the ragged rest (the grid covers both): straight-line code measures ~4%
+  // faster than a grid-stride loop around the same body, and the pieces of
+  // a small call's only span run side by side instead of one after another
   const uint64_t nspan = nfull / kSpan;
   const uint64_t b = blockIdx.x;
-  if (b < nspan)
+  if (b < nspan) {
     reduce_block<true, T, Pol, VEC, UPT, INFL, MAXLV>(
         g, dest_off, src_off, n, lg, b * kSpan, (T *)k_reduce_lds);
-  else if (b * kSpan < nunits)
-    reduce_tail<T, Pol, VEC, UPT, MAXLV>(g, dest_off, src_off, n, lg,
-                                         b * kSpan, (T *)k_reduce_lds);
+  } else {
+    const uint64_t sub = nspan * kSpan + (b - nspan) * kBlock;
+    if (sub < nunits)
+      reduce_block<false, T, Pol, VEC, 1, 8, MAXLV>(
+          g, dest_off, src_off, n, lg, sub, (T *)k_reduce_lds);
+  }
 }
 
 // ---- host-side launch ------------------------------------------------------
@@ -645,9 +651,11 @@ static void launch_shape(const oscd_geom *g, int lg, uint64_t doff,
                          hipStream_t s) {
   const uint64_t units = (n + VEC - 1) / VEC;
   const uint64_t span = (uint64_t)kBlock * UPT;
-  // every span gets its own workgroup; launch() bounds n so that the grid
-  // stays within 2^24 workgroups (2^32 threads)
-  const uint64_t grid = (units + span - 1) / span;
+  // every full span gets its own workgroup, and every 256 units of the
+  // ragged rest; launch() bounds n so that the grid stays within 2^23
+  // workgroups
+  const uint64_t nspan = (n / VEC) / span;
+  const uint64_t grid = nspan + (units - nspan * span + kBlock - 1) / kBlock;
   if constexpr (Pol::TREE) {
     if (lg >= 8) {
       hipLaunchKernelGGL((k_reduce<T, Pol, VEC, UPT, INFL, 13>),
@@ -687,9 +695,9 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
                       (g->slice % AL == 0) && ((uintptr_t)g->base % AL == 0);
   const size_t lds = Pol::SPECIAL ? (size_t)g->npes * sizeof(T) : 0;
   if (n == 0) return OSC_OK;
-  // one workgroup per span of >= 256 units: at most 2^32 units keeps the
-  // grid within 2^24 workgroups (the reference itself takes nreduce < 2^31)
-  if ((n + (vec_ok ? VB : 1) - 1) / (vec_ok ? VB : 1) > (1ull << 32))
+  // one workgroup per span of >= 256 units: at most 2^31 units keeps the
+  // grid within 2^23 workgroups (the reference itself takes nreduce < 2^31)
+  if ((n + (vec_ok ? VB : 1) - 1) / (vec_ok ? VB : 1) > (1ull << 31))
     return OSC_ERR_ARG;
   shape = clamp_shape<Pol>(shape);
   if (!vec_ok) {
