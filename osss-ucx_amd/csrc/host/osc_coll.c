@@ -10,6 +10,7 @@
  * (two WORLD barriers per reduce, reduce.c:1090-1100) and the pSync
  * resets have no counterpart: the device stream orders everything.
  */
+#include <pthread.h>
 #include <stdio.h>
 
 #include "osc_internal.h"
@@ -130,43 +131,27 @@ int osc_broadcast(void *team, void *dest, const void *src, size_t nbytes,
   return osc_broadcast_impl(team, dest, src, nbytes, root, 0, stream);
 }
 
-/* Variable-size collect (collect.c:406-492 closed form): the per-member
- * byte counts are known to the one host caller, so the exclusive prefix
- * sum the reference computes with a scan over the PEs (util/scan.c:10-60)
- * is a host loop here, and the (offset, size) table goes to the device
- * through a pinned staging buffer on the call's stream.  The staging is
- * reused only after the previous collect's kernel has consumed it. */
-int osc_collect(void *team, void *dest, const void *src, const size_t *nbytes,
-                void *stream) {
-  const osc_team *t = osc_team_get(team);
-  size_t doff, soff, total = 0, maxn = 0, m;
+/* The (offset, size) table of the variable-size collect lives in one
+ * pinned staging buffer and one device buffer, reused once the previous
+ * collect's kernel has read them; tab_mu serialises host threads. */
+static pthread_mutex_t tab_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static int collect_launch(const osc_team *t, size_t doff, size_t soff,
+                          const size_t *nbytes, size_t maxn, void *s) {
   uint64_t align;
+  size_t m;
   oscd_geom g;
-  void *s = stream ? stream : osc_g.stream;
   int i, rc, W;
-  if (!osc_g.initialized) return osc_set_error(OSC_ERR_NOTINIT, "not initialised");
-  if (t == NULL) return osc_set_error(OSC_ERR_TEAM, "invalid team");
-  if (nbytes == NULL) return osc_set_error(OSC_ERR_ARG, "NULL size table");
-  for (i = 0; i < t->npes; ++i) {
-    if (nbytes[i] > ((size_t)-1) - total)
-      return osc_set_error(OSC_ERR_ARG, "size overflow");
-    total += nbytes[i];
-    if (nbytes[i] > maxn) maxn = nbytes[i];
-  }
-  rc = osc_sym_range(dest, total, &doff);
-  if (rc != OSC_OK) return rc;
-  rc = osc_sym_range(src, maxn, &soff);
-  if (rc != OSC_OK) return rc;
-  if (total > 0 && maxn > 0 && overlap(doff, total, soff, maxn))
-    return osc_set_error(OSC_ERR_ARG, "dest and source overlap");
-  log_call("collect", t, -1, -1, total);
-  if (maxn == 0) return OSC_OK;
-  if (osc_g.tab_host == NULL) {
+  if (osc_g.tab_host == NULL || osc_g.tab_dev == NULL ||
+      osc_g.tab_event == NULL) {
     const size_t tb = (size_t)2 * OSC_MAX_NPES * sizeof(uint64_t);
-    rc = oscd_host_alloc((void **)&osc_g.tab_host, tb);
-    if (rc == OSC_OK) rc = oscd_malloc((void **)&osc_g.tab_dev, tb);
-    if (rc == OSC_OK) rc = oscd_event_create(&osc_g.tab_event);
+    rc = osc_g.tab_host ? OSC_OK : oscd_host_alloc((void **)&osc_g.tab_host, tb);
+    if (rc == OSC_OK && osc_g.tab_dev == NULL)
+      rc = oscd_malloc((void **)&osc_g.tab_dev, tb);
+    if (rc == OSC_OK && osc_g.tab_event == NULL)
+      rc = oscd_event_create(&osc_g.tab_event);
     if (rc != OSC_OK) return osc_hip_error(rc, "collect table");
+    osc_g.tab_used = 0;
   }
   if (osc_g.tab_used) {
     rc = oscd_event_sync(osc_g.tab_event);
@@ -192,6 +177,41 @@ int osc_collect(void *team, void *dest, const void *src, const size_t *nbytes,
   if (rc != OSC_OK) return osc_hip_error(rc, "collect table event");
   osc_g.tab_used = 1;
   return OSC_OK;
+}
+
+/* Variable-size collect (collect.c:406-492 closed form): the per-member
+ * byte counts are known to the one host caller, so the exclusive prefix
+ * sum the reference computes with a scan over the PEs (util/scan.c:10-60)
+ * is a host loop here, and the (offset, size) table goes to the device
+ * through a pinned staging buffer on the call's stream.  The staging is
+ * reused only after the previous collect's kernel has consumed it. */
+int osc_collect(void *team, void *dest, const void *src, const size_t *nbytes,
+                void *stream) {
+  const osc_team *t = osc_team_get(team);
+  size_t doff, soff, total = 0, maxn = 0;
+  void *s = stream ? stream : osc_g.stream;
+  int i, rc;
+  if (!osc_g.initialized) return osc_set_error(OSC_ERR_NOTINIT, "not initialised");
+  if (t == NULL) return osc_set_error(OSC_ERR_TEAM, "invalid team");
+  if (nbytes == NULL) return osc_set_error(OSC_ERR_ARG, "NULL size table");
+  for (i = 0; i < t->npes; ++i) {
+    if (nbytes[i] > ((size_t)-1) - total)
+      return osc_set_error(OSC_ERR_ARG, "size overflow");
+    total += nbytes[i];
+    if (nbytes[i] > maxn) maxn = nbytes[i];
+  }
+  rc = osc_sym_range(dest, total, &doff);
+  if (rc != OSC_OK) return rc;
+  rc = osc_sym_range(src, maxn, &soff);
+  if (rc != OSC_OK) return rc;
+  if (total > 0 && maxn > 0 && overlap(doff, total, soff, maxn))
+    return osc_set_error(OSC_ERR_ARG, "dest and source overlap");
+  log_call("collect", t, -1, -1, total);
+  if (maxn == 0) return OSC_OK;
+  pthread_mutex_lock(&tab_mu);
+  rc = collect_launch(t, doff, soff, nbytes, maxn, s);
+  pthread_mutex_unlock(&tab_mu);
+  return rc;
 }
 
 /* Strided alltoall (alltoalls.c:54-90 closed form); dst/sst in elements
