@@ -6,8 +6,8 @@ reduce of 8-byte elements, fcollect, alltoall, broadcast) x size bucket
 (destination bytes per PE: <64 KiB, <1 MiB, <16 MiB, >=16 MiB) -- it builds
 a batch of calls of that class and size on a 64-PE world, then tunes by
 coordinate descent: the reduce kernel shape (wide 8x2 or narrow 1x8), the
-tile length (block iterations or KiB), and the grid cap of the family
-launch.  Each candidate is timed as the median of 3 graph launches with
+tile length (block iterations or KiB; 0 = each call alone through the
+per-call kernel), and the grid cap of the family launch.  Each candidate is timed as the median of 3 graph launches with
 HIP events.  The winners are written to tuning/mi355x.tune (next to lib/),
 which osc_init loads.
 
@@ -108,7 +108,9 @@ def main():
                 return time_batch(calls, stream)
 
             shapes = [0, 3] if cls < 2 else [0]
-            tiles = [1, 2, 4, 8] if cls < 2 else [16, 32, 64, 128, 256]
+            # tile 0: every call of the row runs alone through the per-call
+            # kernel (a launch of its own inside the batch)
+            tiles = [0, 1, 2, 4, 8] if cls < 2 else [0, 16, 32, 64, 128, 256]
             grids = [512, 1024, 2048, 4096, 8192]
             for sh in shapes:
                 if sh == best[0]:

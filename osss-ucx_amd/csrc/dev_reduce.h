@@ -351,6 +351,9 @@ constexpr int kBlock = 256;
 // Shapes (units per thread, source slices in flight) a launch can use:
 // 0: 8x2, 1: 8x1, 2: 2x4, 3: 1x8.
 constexpr int kNumShapes = 4;
+// Below this many default-shape spans (one per CU of the 256) a call takes
+// the narrow shape 3 instead.
+constexpr uint64_t kSmallSpans = 256;
 __host__ __device__ constexpr int shape_upt(int s) {
   return s == 0 ? 8 : s == 1 ? 8 : s == 2 ? 2 : 1;
 }
@@ -699,6 +702,14 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
   // grid within 2^23 workgroups (the reference itself takes nreduce < 2^31)
   if ((n + (vec_ok ? VB : 1) - 1) / (vec_ok ? VB : 1) > (1ull << 31))
     return OSC_ERR_ARG;
+  if (shape < 0) {
+    // Small calls: at the default shape they would fill fewer workgroups
+    // than the chip has CUs; one unit per thread with 8 slices in flight
+    // spreads the same work over UPT times more workgroups.
+    const int d = default_shape<Pol>();
+    const uint64_t spans = ((n + VB - 1) / VB) / ((uint64_t)kBlock * shape_upt(d));
+    shape = spans < kSmallSpans ? 3 : d;
+  }
   shape = clamp_shape<Pol>(shape);
   if (!vec_ok) {
     launch_shape<T, Pol, 1, 1, 8>(g, lg, doff, soff, n, lds, s);

@@ -149,7 +149,7 @@ int osc_batch_launch(osc_batch *b, void *stream) {
     int rc2;
     rc = oscd_graph_begin(stream);
     if (rc != OSC_OK) return osc_hip_error(rc, "hipStreamBeginCapture");
-    rc = osc_plan_enqueue(&b->plan, b->d_calls, b->d_tiles, stream);
+    rc = osc_plan_enqueue(&b->plan, b->calls, b->d_calls, b->d_tiles, stream);
     rc2 = oscd_graph_end(stream, &exec);
     if (rc != OSC_OK) {
       if (exec) oscd_graph_destroy(exec);

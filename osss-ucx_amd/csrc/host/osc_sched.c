@@ -7,7 +7,9 @@
  * only the calls by (wave, family), counts each call's tiles in closed
  * form and writes them straight into the caller's tile array; each run of
  * one family becomes one launch of the persistent tile-loop kernel, with a
- * grid capped by the table.  Planning cost is O(calls log calls + tiles).
+ * grid capped by the table.  Calls whose table row says tile 0 get a launch
+ * of their own through the per-call kernel instead, ordered with the rest
+ * of their wave.  Planning cost is O(calls log calls + tiles).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -91,7 +93,7 @@ int osc_call_validate(const osc_call *c, int i) {
 
 /* The family a call's tiles run in and how they are cut. */
 typedef struct {
-  int wave, fam_kind; /* 0 reduce, 1 data movement */
+  int wave, fam_kind; /* 0 reduce, 1 data movement, 2 direct (no tiles) */
   int dtype, op, mode, shape, width;
   int grid;
   uint64_t nunits;  /* units of the call (per block for data movement) */
@@ -109,6 +111,10 @@ static void call_geometry(const osc_call *c, call_geo *g) {
   osc_tune_get(call_class(c), osc_tune_bucket(osc_call_dest_bytes(c)), &shape, &tile, &grid);
   g->grid = grid;
   if (c->count == 0) return;
+  if (tile == 0) { /* the call runs alone through the per-call kernel */
+    g->fam_kind = 2;
+    return;
+  }
   if (c->kind == OSC_CALL_REDUCE) {
     const int vec = oscd_reduce_vec(c->dtype);
     const int mode = osc_g.fpmode[c->op][c->dtype];
@@ -270,8 +276,28 @@ int osc_plan_build_into(osc_plan *p, const osc_call *calls, int ncalls,
     const call_geo *g = &geo[ci];
     const osc_call *c = &calls[ci];
     osc_launch *L;
+    if (g->fam_kind == 2 && c->count > 0) {
+      if (p->nlaunches == p->cap_launches) {
+        const int nc = p->cap_launches ? 2 * p->cap_launches : 64;
+        osc_launch *nl = (osc_launch *)realloc(p->launches, sizeof(osc_launch) * (size_t)nc);
+        if (nl == NULL) {
+          rc = osc_set_error(OSC_ERR_NOMEM, "plan launches");
+          goto out;
+        }
+        p->launches = nl;
+        p->cap_launches = nc;
+      }
+      L = &p->launches[p->nlaunches++];
+      memset(L, 0, sizeof(*L));
+      L->wave = g->wave;
+      L->kind = OSC_LAUNCH_DIRECT;
+      L->call = ci;
+      L->tile0 = k;
+      continue;
+    }
     if (g->ntiles == 0) continue;
     if (p->nlaunches == 0 || geo_cmp(g, &geo[order[i - 1]]) != 0 ||
+        p->launches[p->nlaunches - 1].kind == OSC_LAUNCH_DIRECT ||
         p->launches[p->nlaunches - 1].ntiles == 0) {
       if (p->nlaunches == p->cap_launches) {
         const int nc = p->cap_launches ? 2 * p->cap_launches : 64;
@@ -286,7 +312,7 @@ int osc_plan_build_into(osc_plan *p, const osc_call *calls, int ncalls,
       L = &p->launches[p->nlaunches++];
       memset(L, 0, sizeof(*L));
       L->wave = g->wave;
-      L->kind = g->fam_kind == 0 ? OSC_CALL_REDUCE : -1;
+      L->kind = g->fam_kind == 0 ? OSC_CALL_REDUCE : OSC_LAUNCH_MOVE;
       L->dtype = g->dtype;
       L->op = g->op;
       L->mode = g->mode;
@@ -303,7 +329,8 @@ int osc_plan_build_into(osc_plan *p, const osc_call *calls, int ncalls,
     if ((uint32_t)g->grid > L->grid) L->grid = (uint32_t)g->grid;
   }
   for (i = 0; i < p->nlaunches; ++i)
-    if (p->launches[i].grid > p->launches[i].ntiles)
+    if (p->launches[i].kind != OSC_LAUNCH_DIRECT &&
+        p->launches[i].grid > p->launches[i].ntiles)
       p->launches[i].grid = (uint32_t)p->launches[i].ntiles;
   p->ntiles = k;
 out:
@@ -317,13 +344,16 @@ int osc_plan_build(osc_plan *p, const osc_call *calls, int ncalls,
   return osc_plan_build_into(p, calls, ncalls, use_waves, NULL, 0);
 }
 
-int osc_plan_enqueue(const osc_plan *p, const oscd_bcall *d_calls,
-                     const oscd_tile *d_tiles, void *stream) {
+int osc_plan_enqueue(const osc_plan *p, const osc_call *calls,
+                     const oscd_bcall *d_calls, const oscd_tile *d_tiles,
+                     void *stream) {
   int i, rc = OSC_OK;
   for (i = 0; i < p->nlaunches && rc == OSC_OK; ++i) {
     const osc_launch *L = &p->launches[i];
     const oscd_tile *tiles = d_tiles + L->tile0;
-    if (L->kind == OSC_CALL_REDUCE) {
+    if (L->kind == OSC_LAUNCH_DIRECT) {
+      rc = osc_call_launch_single(&calls[L->call], stream);
+    } else if (L->kind == OSC_CALL_REDUCE) {
       const size_t lds = (L->dtype == OSC_T_F32 || L->dtype == OSC_T_F64) &&
                                  (L->op == OSC_OP_MIN || L->op == OSC_OP_MAX)
                              ? (size_t)L->maxpes * osc_dtype_size(L->dtype)

@@ -10,9 +10,14 @@
 
 #include "osc_internal.h"
 
+/* launch kinds besides OSC_CALL_REDUCE */
+#define OSC_LAUNCH_MOVE (-1)   /* tile loop over data-movement tiles */
+#define OSC_LAUNCH_DIRECT (-2) /* one call through the per-call kernel */
+
 typedef struct {
   int wave;
-  int kind; /* OSC_CALL_REDUCE, or -1 for data movement */
+  int kind; /* OSC_CALL_REDUCE, OSC_LAUNCH_MOVE or OSC_LAUNCH_DIRECT */
+  int call; /* OSC_LAUNCH_DIRECT: index of the call */
   int dtype, op, mode, shape, width;
   int maxlg, maxpes;
   uint32_t grid;
@@ -45,9 +50,11 @@ int osc_plan_build(osc_plan *p, const osc_call *calls, int ncalls,
  * p->tiles; OSC_ERR_BUDGET if they do not fit. */
 int osc_plan_build_into(osc_plan *p, const osc_call *calls, int ncalls,
                         int use_waves, oscd_tile *out, uint64_t cap);
-/* Issue the plan's launches; d_tiles holds p->tiles (same order). */
-int osc_plan_enqueue(const osc_plan *p, const oscd_bcall *d_calls,
-                     const oscd_tile *d_tiles, void *stream);
+/* Issue the plan's launches; d_tiles holds p->tiles (same order), calls
+ * and d_calls the host and device call arrays the plan was built from. */
+int osc_plan_enqueue(const osc_plan *p, const osc_call *calls,
+                     const oscd_bcall *d_calls, const oscd_tile *d_tiles,
+                     void *stream);
 /* One call through the per-call kernels (the "one by one" path). */
 int osc_call_launch_single(const osc_call *c, void *stream);
 

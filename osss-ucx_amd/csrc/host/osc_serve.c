@@ -375,7 +375,7 @@ int osc_serve_run(osc_request *reqs, int nreq, const osc_serve_cfg *cfg,
         if (rc == OSC_OK && s->plan.ntiles)
           rc = oscd_memcpy_h2d_async(s->d_tiles, s->h_tiles, sizeof(oscd_tile) * s->plan.ntiles,
                                      s->stream);
-        if (rc == OSC_OK) rc = osc_plan_enqueue(&s->plan, s->d_calls, s->d_tiles, s->stream);
+        if (rc == OSC_OK) rc = osc_plan_enqueue(&s->plan, s->calls, s->d_calls, s->d_tiles, s->stream);
         if (rc == OSC_OK) rc = oscd_event_record(s->event, s->stream);
         if (rc != OSC_OK) {
           osc_hip_error(rc, "group launch");

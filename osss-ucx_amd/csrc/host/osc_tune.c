@@ -6,7 +6,9 @@
  * 1..4-byte elements, 1 reduce of 8/16-byte elements, 2 fcollect,
  * 3 alltoall, 4 broadcast.  Buckets by destination bytes per PE of the
  * call: <64 KiB, <1 MiB, <16 MiB, >=16 MiB.  The tile length is in block
- * iterations for reductions and in KiB for data movement.
+ * iterations for reductions and in KiB for data movement; tile 0 runs each
+ * call of the row on its own through the per-call kernel (a call that
+ * fills the chip alone gains nothing from sharing a tile-loop launch).
  *
  * The table starts from built-in defaults.  The autotuner (the library's
  * osc_tune_* calls driven by osss-ucx_amd/autotune.py) sweeps candidates on
@@ -50,7 +52,7 @@ int osc_tune_bucket(uint64_t bytes_per_pe) {
 }
 
 int osc_tune_set(int cls, int bucket, int shape, int tile, int grid) {
-  if (cls < 0 || cls >= NCLS || bucket < 0 || bucket >= NBKT || tile < 1 ||
+  if (cls < 0 || cls >= NCLS || bucket < 0 || bucket >= NBKT || tile < 0 ||
       grid < 1 || shape < 0 || shape > 3)
     return OSC_ERR_ARG;
   g_tab[cls][bucket].shape = shape;

@@ -106,4 +106,6 @@ def test_tuning_table_roundtrip(tmp_path):
     assert L.osc_tune_get(3, 2, C.byref(s), C.byref(t), C.byref(g)) == 0
     assert (t.value, g.value) == (96, 1024)
     assert L.osc_tune_set(9, 0, 0, 1, 1) != 0
+    assert L.osc_tune_set(3, 2, 0, -1, 64) != 0   # tile 0 = direct; < 0 bad
+    assert L.osc_tune_set(3, 2, 0, 0, 64) == 0
     assert L.osc_tune_set(3, 2, s0.value, t0.value, g0.value) == 0
