@@ -96,7 +96,8 @@ int osc_tune_save(const char *path) {
   fprintf(f, "# class bucket shape tile grid\n"
              "# class: 0 reduce<=4B, 1 reduce 8/16B, 2 fcollect, 3 alltoall, 4 broadcast\n"
              "# bucket: dest bytes per PE <64KiB, <1MiB, <16MiB, >=16MiB\n"
-             "# tile: block iterations (reduce) or KiB (data movement)\n");
+             "# tile: block iterations (reduce) or KiB (data movement);\n"
+             "#   0 = each call alone through the per-call kernel\n");
   for (c = 0; c < NCLS; ++c)
     for (b = 0; b < NBKT; ++b)
       fprintf(f, "%d %d %d %d %d\n", c, b, g_tab[c][b].shape, g_tab[c][b].tile,
