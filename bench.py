@@ -51,9 +51,9 @@ ESIZE = {"u64": 8, "i64": 8, "f32": 4, "f64": 8}
 
 def kernel_key(op, dtype, mode):
     """The kernel instance the library launches for one cell at the default
-    shape (8 units x 2 slices in flight; FP32 accumulated in FP64: 8 x 1;
-    reference-order tree: 2 x 4), in the short form the profiles/
-    summaries use."""
+    shape (8 units x 2 slices in flight; FP32 accumulated in FP64 and the
+    FP64 double-double sum: 8 x 1; reference-order tree: 2 x 4), in the
+    short form the profiles/ summaries use."""
     o = osc.OPS[op]
     if dtype in ("u64", "i64"):
         t = "long" if (dtype == "i64" and op in ("min", "max")) else "u64"
@@ -66,7 +66,8 @@ def kernel_key(op, dtype, mode):
     if mode == 0:
         if dtype == "f32":
             return "k_reduce<float, PolF32Acc<%d>, 4, 8, 1, 1>" % o
-        return "k_reduce<double, PolF64Acc<%d>, 2, 8, 2, 1>" % o
+        return "k_reduce<double, PolF64Acc<%d>, 2, 8, %d, 1>" % (
+            o, 1 if op == "sum" else 2)
     return "k_reduce<%s, PolTree<%s, %d>, %d, 2, 4, 8>" % (t, t, o, vec)
 
 

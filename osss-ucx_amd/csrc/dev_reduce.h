@@ -132,6 +132,7 @@ struct PolSeq {
   typedef T Acc;
   static constexpr bool TREE = false;
   static constexpr int SHAPE = 0;  // default kernel shape
+  static constexpr int WAVES81 = 0;
   static constexpr bool SPECIAL = SPEC;
   static constexpr int OPC = OP;
   __device__ static Acc lift(T x) { return x; }
@@ -146,6 +147,7 @@ struct PolTree {
   typedef T Acc;
   static constexpr bool TREE = true;
   static constexpr int SHAPE = 2;  // default kernel shape
+  static constexpr int WAVES81 = 0;
   static constexpr bool SPECIAL = false;
   static constexpr int OPC = OP;
   __device__ static Acc lift(T x) { return x; }
@@ -160,6 +162,7 @@ struct PolF32Acc {
   typedef double Acc;
   static constexpr bool TREE = false;
   static constexpr int SHAPE = 1;  // default kernel shape
+  static constexpr int WAVES81 = 0;
   static constexpr bool SPECIAL = false;
   static constexpr int OPC = OP;
   __device__ static Acc lift(float x) { return (double)x; }
@@ -177,7 +180,11 @@ template <int OP>
 struct PolF64Acc {
   typedef dd Acc;
   static constexpr bool TREE = false;
-  static constexpr int SHAPE = 0;  // default kernel shape
+  // The double-double sum keeps 2 doubles per element: at 8x2 it takes 172
+  // VGPRs (2 waves per SIMD).  8x1 held to 4 waves per SIMD measured 3%
+  // faster at 64 PEs x 4M.
+  static constexpr int SHAPE = OP == OSC_OP_SUM ? 1 : 0;  // default shape
+  static constexpr int WAVES81 = OP == OSC_OP_SUM ? 4 : 0;
   static constexpr bool SPECIAL = false;
   static constexpr int OPC = OP;
   __device__ static Acc lift(double x) { return {x, 0.0}; }
@@ -197,6 +204,7 @@ struct PolC32Acc {
   typedef c64acc Acc;
   static constexpr bool TREE = false;
   static constexpr int SHAPE = 0;  // default kernel shape
+  static constexpr int WAVES81 = 0;
   static constexpr bool SPECIAL = false;
   static constexpr int OPC = OP;
   __device__ static Acc lift(cf32 x) { return {(double)x.re, (double)x.im}; }
@@ -219,6 +227,7 @@ struct PolC64Acc {
   typedef cddacc Acc;
   static constexpr bool TREE = false;
   static constexpr int SHAPE = 0;  // default kernel shape
+  static constexpr int WAVES81 = 0;
   static constexpr bool SPECIAL = false;
   static constexpr int OPC = OP;
   __device__ static Acc lift(cf64 x) { return {{x.re, 0.0}, {x.im, 0.0}}; }
@@ -613,8 +622,16 @@ __device__ __forceinline__ void reduce_tail(const oscd_geom &g,
                                                   sub, lds);
 }
 
+// Waves per SIMD the register allocator must leave room for: a policy's
+// WAVES81 at the 8x1 shape, else no constraint.
+template <class Pol, int UPT, int INFL>
+constexpr int kernel_waves() {
+  return (UPT == 8 && INFL == 1 && Pol::WAVES81 > 0) ? Pol::WAVES81 : 1;
+}
+
 template <class T, class Pol, int VEC, int UPT, int INFL, int MAXLV>
-__global__ __launch_bounds__(kBlock) void k_reduce(const oscd_geom g,
+__global__ __launch_bounds__(kBlock) __attribute__((
+    amdgpu_waves_per_eu(kernel_waves<Pol, UPT, INFL>()))) void k_reduce(const oscd_geom g,
                                                    const uint64_t dest_off,
                                                    const uint64_t src_off,
                                                    const uint64_t n,
