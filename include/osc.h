@@ -112,6 +112,11 @@ void *osc_heap_base(void);    /* PE 0's slice                            */
 void *osc_default_stream(void);
 int osc_set_fp_mode(int mode);
 int osc_get_fp_mode(void);
+/* Reduce kernel shape for per-call launches: -1 (default) picks by
+ * accumulation policy and call size; 0..3 force 8x2, 8x1, 2x4 or 1x8 units
+ * per thread x slices in flight (OSC_REDUCE_SHAPE sets it at init). */
+int osc_set_reduce_shape(int shape);
+int osc_get_reduce_shape(void);
 const char *osc_strerror(int code);
 const char *osc_last_error(void);
 int osc_last_error_code(void); /* code of the calling thread's last error */

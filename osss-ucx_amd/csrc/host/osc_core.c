@@ -151,6 +151,14 @@ int osc_finalize(void) {
 }
 
 int osc_initialized(void) { return osc_g.initialized; }
+
+/* -1: per-call default (by policy and size); 0..3: force one shape */
+int osc_set_reduce_shape(int shape) {
+  if (shape < -1 || shape > 3) return osc_set_error(OSC_ERR_ARG, "shape %d", shape);
+  osc_g.reduce_shape = shape;
+  return OSC_OK;
+}
+int osc_get_reduce_shape(void) { return osc_g.reduce_shape; }
 int osc_npes(void) { return osc_g.npes; }
 size_t osc_slice_bytes(void) { return osc_g.slice; }
 size_t osc_heap_bytes(void) { return osc_g.heap_bytes; }
