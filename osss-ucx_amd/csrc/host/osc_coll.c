@@ -28,6 +28,12 @@ static void log_call(const char *what, const osc_team *t, int op, int dtype,
 
 int osc_reduce(void *team, int op, int dtype, void *dest, const void *src,
                size_t nreduce, void *stream) {
+  return osc_reduce_mode(team, op, dtype, dest, src, nreduce, -1, stream);
+}
+
+/* fpmode < 0: the mode the registry resolved for (op, dtype) */
+int osc_reduce_mode(void *team, int op, int dtype, void *dest, const void *src,
+                    size_t nreduce, int fpmode, void *stream) {
   const osc_team *t = osc_team_get(team);
   const size_t es = osc_dtype_size(dtype);
   size_t doff, soff;
@@ -53,7 +59,8 @@ int osc_reduce(void *team, int op, int dtype, void *dest, const void *src,
   log_call("reduce", t, op, dtype, nreduce);
   if (nreduce == 0) return OSC_OK;
   osc_team_geom(t, &g);
-  rc = oscd_reduce(&g, op, dtype, osc_g.fpmode[op][dtype], osc_g.reduce_shape,
+  rc = oscd_reduce(&g, op, dtype, fpmode >= 0 ? fpmode : osc_g.fpmode[op][dtype],
+                   osc_g.reduce_shape,
                    doff, soff, nreduce, stream ? stream : osc_g.stream);
   return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "reduce launch");
 }

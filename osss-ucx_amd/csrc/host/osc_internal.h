@@ -46,6 +46,7 @@ typedef struct osc_state {
   int spmd; /* 1 while osc_spmd_run drives one host thread per PE */
   /* resolved algorithm choice: float mode per (op, dtype) */
   unsigned char fpmode[OSC_OP_COUNT][OSC_T_COUNT];
+  unsigned char fpmode_to_all[OSC_OP_COUNT][OSC_T_COUNT]; /* active-set forms */
   int fp_default;
   int reduce_shape; /* -1 = per-policy default; OSC_REDUCE_SHAPE overrides */
   void *aux_dev;    /* 256 device bytes for single-call descriptors */
@@ -103,8 +104,12 @@ int osc_heap_call_realloc(size_t off, size_t size, size_t *new_off);
  * arrive runs the team-call path once for all and the others wait for it.
  * Kinds for spmd_move: */
 enum { OSC_SPMD_FCOLLECT = 1, OSC_SPMD_ALLTOALL, OSC_SPMD_BROADCAST };
+/* fpmode < 0: the team reductions' table */
 int osc_spmd_reduce(void *team, int op, int dtype, void *dest,
-                    const void *src, size_t n);
+                    const void *src, size_t n, int fpmode);
+/* osc_reduce with an explicit float mode (osc_coll.c) */
+int osc_reduce_mode(void *team, int op, int dtype, void *dest, const void *src,
+                    size_t nreduce, int fpmode, void *stream);
 int osc_spmd_move(int kind, void *team, void *dest, const void *src,
                   size_t nbytes, int root, int skip_root);
 int osc_spmd_collect(void *team, void *dest, const void *src,

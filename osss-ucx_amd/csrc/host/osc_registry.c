@@ -2,9 +2,10 @@
  * osc_registry.c - collective algorithm selection.
  *
  * Keeps the reference's configuration surface: SHMEM_<OP>_REDUCE_ALGO,
- * SHMEM_FCOLLECT_ALGO, SHMEM_ALLTOALL_ALGO and SHMEM_BROADCAST_ALGO hold
- * "algo" or "algo:type" (src/shmemc/env.c:218-312), and an unknown name is
- * an error (src/api/collectives/shcoll-shim.c:26-35).  The difference is
+ * SHMEM_<OP>_TO_ALL_ALGO, SHMEM_FCOLLECT_ALGO, SHMEM_ALLTOALL_ALGO,
+ * SHMEM_BROADCAST_ALGO and the other per-collective variables hold "algo"
+ * or "algo:type" (src/shmemc/env.c:187-312), and an unknown name is an
+ * error (src/api/collectives/shcoll-shim.c:26-35).  The difference is
  * when it is resolved: the reference formats "algo:type" and scans its
  * registry table on every call (shcoll-shim.c:43-59, table.c:642-670);
  * here every (op, type) cell is resolved once, at init, into a table the
@@ -59,7 +60,31 @@ static const char *const k_move_aliases[] = {
     "color_pairwise_exchange_counter", "color_pairwise_exchange_signal",
     /* broadcast (src/shcoll/src/broadcast.c:545-551) */
     "complete_tree", "binomial_tree", "knomial_tree", "knomial_tree_signal",
-    "scatter_collect", NULL};
+    "scatter_collect",
+    /* collect (src/shcoll/src/collect.c:682-738) */
+    "rec_dbl_signal", "simple",
+    /* barrier / sync (src/shcoll/src/barrier.c:328-366) */
+    "dissemination", NULL};
+
+/* the reference's other per-collective algorithm variables
+ * (src/shmemc/env.c:187-252): permutations or synchronisation, the same
+ * under every algorithm, so they are only checked for a known name */
+static const char *const k_other_env[] = {
+    "SHMEM_BARRIER_ALGO",        "SHMEM_BARRIER_ALL_ALGO",
+    "SHMEM_SYNC_ALGO",           "SHMEM_TEAM_SYNC_ALGO",
+    "SHMEM_SYNC_ALL_ALGO",       "SHMEM_BROADCASTMEM_ALGO",
+    "SHMEM_COLLECT_ALGO",        "SHMEM_COLLECTMEM_ALGO",
+    "SHMEM_FCOLLECTMEM_ALGO",    "SHMEM_ALLTOALLMEM_ALGO",
+    "SHMEM_ALLTOALLS_ALGO",      "SHMEM_ALLTOALLSMEM_ALGO",
+    "SHMEM_ALLTOALL_SIZE_ALGO",  "SHMEM_ALLTOALLS_SIZE_ALGO",
+    "SHMEM_COLLECT_SIZE_ALGO",   "SHMEM_FCOLLECT_SIZE_ALGO",
+    "SHMEM_BROADCAST_SIZE_ALGO", NULL};
+
+/* the active-set reductions have their own variables (env.c:257-281) */
+static const char *const k_to_all_env[OSC_OP_COUNT] = {
+    "SHMEM_AND_TO_ALL_ALGO", "SHMEM_OR_TO_ALL_ALGO",  "SHMEM_XOR_TO_ALL_ALGO",
+    "SHMEM_MIN_TO_ALL_ALGO", "SHMEM_MAX_TO_ALL_ALGO", "SHMEM_SUM_TO_ALL_ALGO",
+    "SHMEM_PROD_TO_ALL_ALGO"};
 
 static int type_of(const char *name) {
   size_t i;
@@ -113,20 +138,34 @@ int osc_registry_init_from_env(void) {
       return osc_set_error(OSC_ERR_ALGO, "OSC_FP_MODE=%s (want accurate|reference)", fp);
   }
   for (op = 0; op < OSC_OP_COUNT; ++op)
-    for (t = 0; t < OSC_T_COUNT; ++t) osc_g.fpmode[op][t] = (unsigned char)osc_g.fp_default;
+    for (t = 0; t < OSC_T_COUNT; ++t)
+      osc_g.fpmode[op][t] = osc_g.fpmode_to_all[op][t] =
+          (unsigned char)osc_g.fp_default;
   for (op = 0; op < OSC_OP_COUNT; ++op) {
-    const char *spec = getenv(k_op_env[op]);
-    const char *colon;
-    if (spec == NULL || spec[0] == '\0') continue;
-    if (osc_registry_resolve(OSC_KIND_REDUCE, op, spec, &mode) != OSC_OK)
-      return osc_set_error(OSC_ERR_ALGO, "couldn't register collective \"%s\"=%s",
-                           k_op_env[op], spec);
-    colon = strchr(spec, ':');
-    if (colon != NULL) {
-      osc_g.fpmode[op][type_of(colon + 1)] = (unsigned char)mode;
-    } else {
-      for (t = 0; t < OSC_T_COUNT; ++t) osc_g.fpmode[op][t] = (unsigned char)mode;
+    int pass;
+    for (pass = 0; pass < 2; ++pass) {
+      const char *name = pass == 0 ? k_op_env[op] : k_to_all_env[op];
+      unsigned char(*tab)[OSC_T_COUNT] = pass == 0 ? osc_g.fpmode : osc_g.fpmode_to_all;
+      const char *spec = getenv(name);
+      const char *colon;
+      if (spec == NULL || spec[0] == '\0') continue;
+      if (osc_registry_resolve(OSC_KIND_REDUCE, op, spec, &mode) != OSC_OK)
+        return osc_set_error(OSC_ERR_ALGO, "couldn't register collective \"%s\"=%s",
+                             name, spec);
+      colon = strchr(spec, ':');
+      if (colon != NULL) {
+        tab[op][type_of(colon + 1)] = (unsigned char)mode;
+      } else {
+        for (t = 0; t < OSC_T_COUNT; ++t) tab[op][t] = (unsigned char)mode;
+      }
     }
+  }
+  for (t = 0; k_other_env[t] != NULL; ++t) {
+    const char *spec = getenv(k_other_env[t]);
+    if (spec != NULL && spec[0] != '\0' &&
+        osc_registry_resolve(OSC_KIND_FCOLLECT, 0, spec, NULL) != OSC_OK)
+      return osc_set_error(OSC_ERR_ALGO, "couldn't register collective \"%s\"=%s",
+                           k_other_env[t], spec);
   }
   for (t = 0; t < 3; ++t) {
     const char *spec = getenv(mv[t]);
@@ -144,7 +183,8 @@ int osc_set_fp_mode(int mode) {
     return osc_set_error(OSC_ERR_ARG, "fp mode %d", mode);
   osc_g.fp_default = mode;
   for (op = 0; op < OSC_OP_COUNT; ++op)
-    for (t = 0; t < OSC_T_COUNT; ++t) osc_g.fpmode[op][t] = (unsigned char)mode;
+    for (t = 0; t < OSC_T_COUNT; ++t)
+      osc_g.fpmode[op][t] = osc_g.fpmode_to_all[op][t] = (unsigned char)mode;
   return OSC_OK;
 }
 

@@ -181,7 +181,7 @@ typedef struct {
 
 typedef struct {
   void *team;
-  int op, dtype, root, skip, move;
+  int op, dtype, root, skip, move, fpmode;
   void *dest;
   const void *src;
   size_t n, es;
@@ -199,11 +199,12 @@ static int lead_reduce(meet *m, void *c) {
   (void)m;
   if (x->dtype < 0)
     return osc_set_error(OSC_ERR_UNSUPPORTED, "long double reduction (no 80-bit type on gfx950)");
-  return synced(osc_reduce(x->team, x->op, x->dtype, x->dest, x->src, x->n, NULL));
+  return synced(osc_reduce_mode(x->team, x->op, x->dtype, x->dest, x->src, x->n,
+                                x->fpmode, NULL));
 }
 
 int osc_spmd_reduce(void *team, int op, int dtype, void *dest,
-                    const void *src, size_t n) {
+                    const void *src, size_t n, int fpmode) {
   const osc_team *t;
   call_key k;
   call_ctx x;
@@ -223,6 +224,7 @@ int osc_spmd_reduce(void *team, int op, int dtype, void *dest,
   x.dest = dest;
   x.src = src;
   x.n = n;
+  x.fpmode = fpmode;
   return meet_run(t, r, &k, sizeof(k), 0, lead_reduce, &x, NULL);
 }
 

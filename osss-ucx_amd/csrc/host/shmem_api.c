@@ -524,7 +524,7 @@ static int finish(int rc) {
 static int blocking_reduce(shmem_team_t team, int op, int dtype, void *dest,
                            const void *src, size_t n) {
   if (osc_g.spmd) /* every member arrives, dtype < 0 included */
-    return osc_spmd_reduce(team, op, dtype, dest, src, n);
+    return osc_spmd_reduce(team, op, dtype, dest, src, n, -1);
   if (dtype < 0)
     return osc_set_error(OSC_ERR_UNSUPPORTED, "long double reduction (no 80-bit type on gfx950)");
   return finish(osc_reduce(team, op, dtype, dest, src, n, NULL));
@@ -661,7 +661,8 @@ static void legacy_reduce(int op, int dtype, void *target, const void *source,
   if (osc_g.spmd) {
     if (osc_active_set(PE_start, logPE_stride, PE_size, &t) != OSC_OK) return;
     osc_spmd_reduce(&t, op, dtype, target, source,
-                    nreduce < 0 ? (size_t)-1 : (size_t)nreduce);
+                    nreduce < 0 ? (size_t)-1 : (size_t)nreduce,
+                    dtype < 0 ? -1 : osc_g.fpmode_to_all[op][dtype]);
     return;
   }
   if (dtype < 0) {
@@ -673,7 +674,8 @@ static void legacy_reduce(int op, int dtype, void *target, const void *source,
     return;
   }
   if (osc_active_set(PE_start, logPE_stride, PE_size, &t) != OSC_OK) return;
-  finish(osc_reduce(&t, op, dtype, target, source, (size_t)nreduce, NULL));
+  finish(osc_reduce_mode(&t, op, dtype, target, source, (size_t)nreduce,
+                         osc_g.fpmode_to_all[op][dtype], NULL));
 }
 
 #define OSC_DEF_TO_ALL(_type, _typename, _op, _OPC)                            \

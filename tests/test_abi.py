@@ -112,6 +112,20 @@ def test_bad_algorithm_env_is_an_error(monkeypatch):
     assert lib.osc_initialized() == 0
 
 
+@pytest.mark.parametrize("var,val", [
+    ("SHMEM_SUM_TO_ALL_ALGO", "bogus"), ("SHMEM_COLLECT_ALGO", "nope"),
+    ("SHMEM_ALLTOALLS_ALGO", "bruck2"), ("SHMEM_BARRIER_ALGO", "fast"),
+    ("SHMEM_FCOLLECT_SIZE_ALGO", "x")])
+def test_other_algorithm_variables_are_checked(monkeypatch, var, val):
+    """The reference's other per-collective variables (src/shmemc/env.c:
+    187-281) are read too: an unknown algorithm name fails init."""
+    monkeypatch.setenv(var, val)
+    lib = osc.lib()
+    assert lib.osc_init(4, 1 << 20, 0) == -8          # OSC_ERR_ALGO
+    assert var.encode() in lib.osc_last_error()
+    assert lib.osc_initialized() == 0
+
+
 def test_calls_before_init_fail_cleanly():
     lib = osc.lib()
     assert lib.osc_reduce(None, 5, 6, None, None, 10, None) == -2

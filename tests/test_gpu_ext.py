@@ -385,3 +385,28 @@ def test_put_signal(heap):
         L.osc_clear_error()
         heap.free(d)
         heap.free(sig)
+
+
+def test_to_all_algorithm_variable(monkeypatch):
+    """SHMEM_SUM_TO_ALL_ALGO=rec_dbl gives the active-set sum the reference's
+    bits while the team sum keeps its own (accurate) setting."""
+    from gpu_util import max_rel_err
+    monkeypatch.setenv("SHMEM_SUM_TO_ALL_ALGO", "rec_dbl")
+    if osc.Heap._current is not None:
+        osc.Heap._current.close()
+    P, n = 7, 4096
+    with osc.Heap(npes=P, heap_bytes=4 << 20) as h:
+        rng = np.random.default_rng(3)
+        src = rng.standard_normal((P, n)).astype(np.float32)
+        s, d = h.malloc(n * 4), h.malloc(n * 4)
+        for pe in range(P):
+            h.put(s, src[pe], pe)
+        h.L.osc_clear_error()
+        osc.typed("shmem_float_sum_to_all")(d, s, n, 0, 0, P, None, None)
+        assert h.L.osc_last_error_code() == 0
+        got = np.stack([h.get(d, n, np.float32, pe) for pe in range(P)])
+        assert np.array_equal(got.view(np.uint32),
+                              O.rec_dbl("sum", "f32", src).view(np.uint32))
+        assert osc.typed("shmem_float_sum_reduce")(h.world, d, s, n) == 0
+        team = np.stack([h.get(d, n, np.float32, pe) for pe in range(P)])
+        assert max_rel_err(team, src, "sum", "f32") <= 2.0 ** -24
