@@ -409,4 +409,4 @@ def test_to_all_algorithm_variable(monkeypatch):
                               O.rec_dbl("sum", "f32", src).view(np.uint32))
         assert osc.typed("shmem_float_sum_reduce")(h.world, d, s, n) == 0
         team = np.stack([h.get(d, n, np.float32, pe) for pe in range(P)])
-        assert max_rel_err(team, src, "sum", "f32") <= 2.0 ** -24
+        assert max_rel_err(team, src, "sum", "f32") <= 2.0 ** -23
