@@ -293,6 +293,15 @@ static void strided(void *dest, const void *src, size_t es, ptrdiff_t tst,
   char *r, *stage = NULL;
   int rc = OSC_OK;
   if (n == 0) return;
+  {
+    /* every byte offset below, (n-1)*|stride|*es + es, must fit ptrdiff_t */
+    const size_t at = (size_t)(rst < 0 ? -rst : rst), al = (size_t)(lst < 0 ? -lst : lst);
+    const size_t amax = (at > al ? at : al) > 0 ? (at > al ? at : al) : 1;
+    if (es > (size_t)PTRDIFF_MAX / 2 || n - 1 > ((size_t)PTRDIFF_MAX / 2) / es / amax) {
+      osc_set_error(OSC_ERR_ARG, "strided RMA of %zu elements overflows", n);
+      return;
+    }
+  }
   if (tst == 0) { /* every element lands on one target: the last wins */
     if (is_put)
       lp = (char *)lp + (ptrdiff_t)(n - 1) * lst * (ptrdiff_t)es;
