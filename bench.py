@@ -49,6 +49,60 @@ CELLS = [  # (op, dtype, dist, a, b)
 ESIZE = {"u64": 8, "i64": 8, "f32": 4, "f64": 8}
 
 
+def max_rel_err_exact(got, src, op):
+    """Largest |got - exact| / |exact| over the columns of src[P, k]; the
+    exact sum or product comes from rational arithmetic on the inputs."""
+    from fractions import Fraction
+    worst = 0.0
+    for j in range(src.shape[1]):
+        col = [Fraction(float(x)) for x in src[:, j]]
+        ex = col[0]
+        for x in col[1:]:
+            ex = ex + x if op == "sum" else ex * x
+        if ex != 0:
+            worst = max(worst, float(abs(Fraction(float(got[j])) - ex) / abs(ex)))
+    return worst
+
+
+def accuracy(L, h, src, dest, windows=4, width=256, seed=9):
+    """Float sum/prod cells of the matrix: max relative error vs the exact
+    value on sampled columns (windows of `width` consecutive elements), in
+    the default accurate mode and in the reference's recursive-doubling
+    order in the element type's own arithmetic (bit-identical to the
+    reference's results), for every member's copy of PE 0 and PE 63."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    starts = sorted(int(x) * width for x in
+                    rng.choice(N // width, windows, replace=False))
+    res = {}
+    for op, dt, dist, a, b in CELLS:
+        if dt not in ("f32", "f64") or op not in ("sum", "prod"):
+            continue
+        npt = np.float32 if dt == "f32" else np.float64
+        es = ESIZE[dt]
+        s = src[(dt, dist, a, b)]
+        cols = np.concatenate([np.stack([h.get(s + o * es, width, npt, pe)
+                                         for pe in range(P)])
+                               for o in starts], axis=1)
+        cell = {}
+        for name, mode in (("accurate", osc.FP_ACCURATE),
+                           ("reference_order", osc.FP_REFERENCE)):
+            osc.check(L.osc_set_fp_mode(mode), "fp mode")
+            osc.check(L.osc_reduce(h.world, osc.OPS[op], osc.DTYPES[dt], dest,
+                                   s, N, None), "osc_reduce")
+            h.sync()
+            err = 0.0
+            for pe in (0, P - 1):
+                got = np.concatenate([h.get(dest + o * es, width, npt, pe)
+                                      for o in starts])
+                err = max(err, max_rel_err_exact(got, cols, op))
+            cell[name + "_max_rel_err"] = err
+        res[op + "_" + dt] = cell
+    res["sample"] = "%d columns (%d windows of %d), all %d PEs' inputs" % (
+        windows * width, windows, width, P)
+    return res
+
+
 def kernel_key(op, dtype, mode):
     """The kernel instance the library launches for one cell at the default
     shape (8 units x 2 slices in flight; FP32 accumulated in FP64 and the
@@ -125,10 +179,10 @@ def _median_event_ms(L, stream, fn, reps=10):
 
 
 def bench_movement():
-    """Configs 3a/3b at full size: shmem_int64_alltoall with 65,536-byte
+    """The data-movement workloads at full size: shmem_int64_alltoall with 65,536-byte
     blocks on 64 PEs (2*P^2*b bytes) and shmem_int32_fcollect of 1 MiB per
     PE on 256 PEs (P*m + P^2*m bytes), inputs resident, HIP-event medians.
-    Also config 1's latency: shmem_int_sum_reduce, 8 PEs x 65,536 int32,
+    Also the latency of the small plain reduce: shmem_int_sum_reduce, 8 PEs x 65,536 int32,
     kernel time and the blocking call's wall time."""
     import numpy as np
     L = osc.lib()
@@ -171,14 +225,14 @@ def bench_movement():
             fn(h.world, d, s, n)
             walls.append((time.perf_counter() - t0) * 1e6)
         walls.sort()
-        out["c1_int_sum_8pe_64K"] = {"kernel_us": round(kern * 1e3, 2),
+        out["int_sum_8pe_64K"] = {"kernel_us": round(kern * 1e3, 2),
                                      "call_p50_us": round(walls[100], 2),
                                      "call_p99_us": round(walls[198], 2)}
     return out
 
 
 def bench_batched(seed=4, ncalls=4096):
-    """Config 4: ncalls independent calls (osc_calls.batched_mix) in one
+    """The batched workload: ncalls independent calls (osc_calls.batched_mix) in one
     captured graph vs. the same calls issued one per kernel, on a 128-PE
     world; every call's result is verified batched == one-by-one by
     per-call digests, wave by wave."""
@@ -235,19 +289,24 @@ def bench_batched(seed=4, ncalls=4096):
         world.close()
 
 
+# served admission order: shortest first, with one second of waiting worth
+# 250 GB of algorithmic bytes (about 5% of a second of work at the measured
+# rate), so large requests are not starved
+AGE_BYTES_PER_S = 2.5e11
+
+
 def bench_served(seed=5, nreq=2000, rate=20000.0, verify=96):
-    """Config 5: 16 teams of 8..128 PEs in a 128-PE world send Poisson
+    """The served workload: 16 teams of 8..128 PEs in a 128-PE world send Poisson
     requests at `rate` (60% f32/f64/i64 sum|max reduce with nreduce
     log-uniform 256..4M, 20% fcollect, 20% alltoall, blocks 1 KiB..1 MiB).
     Heap (inputs + destination arena) is sized to a 200 GB budget; the
     admission budget is what the inputs leave of it.  Runs continuous
     batching and the per-call baseline on the same request stream, then a
     verification run with digests: sampled requests must give identical
-    results alone, and the f32 sums' error vs an FP64 oracle is measured
+    results alone, and the f32 sums' error vs the exact sum is measured
     on sampled columns."""
     import numpy as np
     import osc_calls
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     budget_total = 200e9
     npes = 128
     inputs = (4 + 8 + 8) * 4194304 + (128 << 20)   # f32, f64, i64 + moves
@@ -263,9 +322,10 @@ def bench_served(seed=5, nreq=2000, rate=20000.0, verify=96):
         teams = osc_calls.served_teams(rng, world, 16, 8, 128)
         reqs = osc_calls.served_mix(rng, world, teams, nreq, rate)
 
-        def cfg(mode, digests):
+        def cfg(mode, digests, order=osc.ORDER_SIZE):
             c = osc.ServeCfg()
             c.mode, c.max_inflight, c.nstreams, c.max_group = mode, 256, 4, 64
+            c.order, c.age_bytes_per_s = order, AGE_BYTES_PER_S
             c.dest_base, c.dest_bytes = world.dest_off, arena
             c.page_bytes = 256 << 10
             c.budget_bytes = budget_total - npes * inputs
@@ -284,14 +344,14 @@ def bench_served(seed=5, nreq=2000, rate=20000.0, verify=96):
 
         osc.serve(reqs[:64], cfg(osc.SERVE_BATCHED, 0))  # warm-up
         _, st_b = osc.serve(reqs, cfg(osc.SERVE_BATCHED, 0))
-        _, st_p = osc.serve(reqs, cfg(osc.SERVE_PER_CALL, 0))
+        _, st_a = osc.serve(reqs, cfg(osc.SERVE_BATCHED, 0, osc.ORDER_ARRIVAL))
+        _, st_p = osc.serve(reqs, cfg(osc.SERVE_PER_CALL, 0, osc.ORDER_ARRIVAL))
         # verification: same stream with digests, then sampled requests alone
         out, _ = osc.serve(reqs, cfg(osc.SERVE_BATCHED, 1))
         pick = rng.choice(len(out), size=min(verify, len(out)), replace=False)
         same = all(osc.request_alone(out[i].call, world.dest_off + arena) ==
                    out[i].digest for i in pick)
-        # FP32 sum error vs the FP64 oracle on sampled columns
-        import oracle as O
+        # FP32 sum error vs the exact sum on sampled columns
         h, base0 = world.h, world.h.L.osc_heap_base()
         max_err = 0.0
         f32_sums = [r for r in out if r.call.kind == osc.CALL_REDUCE and
@@ -307,21 +367,22 @@ def bench_served(seed=5, nreq=2000, rate=20000.0, verify=96):
                                   p)[cols] for p in pes])
             got = h.get(base0 + world.dest_off + arena, c.count, np.float32,
                         pes[0])[cols]
-            hi, lo = O.accurate("sum", "f32", src)
-            err = np.abs((got.astype(np.float64) - hi) - lo) / np.abs(hi)
-            max_err = max(max_err, float(err.max()))
+            max_err = max(max_err, max_rel_err_exact(got, src, "sum"))
         return {"requests": nreq, "rate_per_s": rate, "teams": 16,
                 "world_pes": npes,
                 "hbm_budget_GB": budget_total / 1e9,
                 "admission_budget_GB": round((budget_total - npes * inputs)
                                              / 1e9, 1),
                 "offered_GB": round(st_b.algorithmic_bytes / 1e9, 1),
+                "order": "size-aware, %.3g B/s aging" % AGE_BYTES_PER_S,
                 "served": summary(st_b), "per_call": summary(st_p),
+                "served_arrival_order": summary(st_a),
+                "p50_speedup": round(st_p.p50_ms / st_b.p50_ms, 3),
                 "p99_speedup": round(st_p.p99_ms / st_b.p99_ms, 3),
                 "GBps_speedup": round(st_b.gbps / st_p.gbps, 3),
                 "same_alone_and_in_mix": bool(same),
                 "verified_requests": int(len(pick)),
-                "max_fp32_sum_rel_err_vs_fp64": max_err}
+                "max_fp32_sum_rel_err_vs_exact": max_err}
     finally:
         world.close()
 
@@ -401,6 +462,9 @@ def main():
     dom_b = 2.0 * P * N * ESIZE[CELLS[dom][1]]
     achieved = dom_b / dom_avg / 1e9
 
+    acc = accuracy(L, h, src, dest)
+    osc.check(L.osc_set_fp_mode(mode), "fp mode")
+
     # plain-copy ceiling on the same arena after the timed region (it
     # overwrites slices 32.. of the heap): 1 GiB read + 1 GiB written
     cp_bytes = 1 << 31
@@ -446,6 +510,7 @@ def main():
                      "avg_launch_us": round(dom_avg * 1e6, 1)},
         "copy_ceiling_GBps": round(copy_gbps, 1),
         "cells": cells,
+        "accuracy": acc,
     }
     h.close()
     out["movement"] = None if args.no_movement else bench_movement()

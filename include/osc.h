@@ -239,15 +239,20 @@ int osc_tune_get(int cls, int bucket, int *shape, int *tile_iters, int *grid);
 
 /* ---- served mode -----------------------------------------------------------
  * A request loop for many concurrent teams: requests arrive over time
- * (arrival_s), are admitted in arrival order when a slot and destination
- * memory inside the HBM budget are free, and run on the device as soon as
- * admitted.  OSC_SERVE_BATCHED is continuous batching: every admitted
+ * (arrival_s), are admitted when a slot and destination memory inside the
+ * HBM budget are free, and run on the device as soon as admitted.  The
+ * arrived requests wait in a queue ordered by cfg->order: arrival order,
+ * or size-aware (shortest first with aging: key = algorithmic bytes +
+ * age_bytes_per_s * arrival_s, so a request that has waited longer may be
+ * proportionally larger and still go first; age_bytes_per_s = 0 is pure
+ * shortest-first).  OSC_SERVE_BATCHED is continuous batching: every admitted
  * request that is not yet running joins the next launch group (one tiled
  * persistent launch per kernel family), groups run concurrently on several
  * streams, and memory is released as each group completes.
  * OSC_SERVE_PER_CALL is the baseline: each request is its own per-call
  * kernel launch, in arrival order on one stream. */
 typedef enum { OSC_SERVE_BATCHED = 0, OSC_SERVE_PER_CALL = 1 } osc_serve_mode_t;
+typedef enum { OSC_SERVE_ORDER_ARRIVAL = 0, OSC_SERVE_ORDER_SIZE = 1 } osc_serve_order_t;
 
 typedef struct osc_request {
   osc_call call;     /* dest_off is assigned by the server */
@@ -271,7 +276,9 @@ typedef struct osc_serve_cfg {
   double budget_bytes;  /* admission budget: in-flight destination bytes
                            summed over every team PE */
   int32_t digests;      /* digest each result before its memory is reused */
-  int32_t pad_;
+  int32_t order;        /* OSC_SERVE_ORDER_*: admission order of the queue */
+  double age_bytes_per_s; /* OSC_SERVE_ORDER_SIZE: priority gained per second
+                             of earlier arrival, in algorithmic bytes (>= 0) */
 } osc_serve_cfg;
 
 typedef struct osc_serve_stats {

@@ -5,10 +5,16 @@
  * streams.  Each iteration:
  *   1. retire: launch groups whose event has completed release their
  *      destination pages and budget (after digesting results if asked);
- *   2. admit: requests that have arrived are admitted in arrival order
- *      while a slot is free, the HBM budget allows their destination bytes
- *      and the team-span page allocator finds room for them;
- *   3. launch: in batched mode every admitted, not yet running request
+ *   2. arrive: requests whose arrival time has passed are validated
+ *      (those that can never run are rejected) and enter the ready queue,
+ *      a binary heap ordered by arrival time or, in the size-aware order,
+ *      by algorithmic bytes + age_bytes_per_s * arrival time (shortest
+ *      first with aging, after size-based scheduling of web servers,
+ *      Harchol-Balter et al., ACM TOCS 21(2), 2003);
+ *   3. admit: the head of the ready queue is admitted while a slot is
+ *      free, the HBM budget allows its destination bytes and the team-span
+ *      page allocator finds room for it;
+ *   4. launch: in batched mode every admitted, not yet running request
  *      joins the next launch group on a free stream (tiled persistent
  *      launches per kernel family, see osc_sched.c); in per-call mode each
  *      request is one per-call kernel on a single stream, in order.
@@ -107,9 +113,39 @@ typedef struct {
 
 typedef struct {
   int first, npages, lo, hi;
-  double area;
+  double area, key;
   uint64_t tiles;
 } req_state;
+
+/* ---- ready queue: a binary min-heap of request indices by (key, index) ----- */
+static int rq_before(const req_state *rs, int a, int b) {
+  return rs[a].key < rs[b].key || (rs[a].key == rs[b].key && a < b);
+}
+
+static void rq_push(int *h, int *n, const req_state *rs, int r) {
+  int i = (*n)++;
+  while (i > 0) {
+    const int up = (i - 1) / 2;
+    if (!rq_before(rs, r, h[up])) break;
+    h[i] = h[up];
+    i = up;
+  }
+  h[i] = r;
+}
+
+static void rq_pop(int *h, int *n, const req_state *rs) {
+  const int last = h[--(*n)];
+  int i = 0;
+  for (;;) {
+    int c = 2 * i + 1;
+    if (c >= *n) break;
+    if (c + 1 < *n && rq_before(rs, h[c + 1], h[c])) ++c;
+    if (!rq_before(rs, h[c], last)) break;
+    h[i] = h[c];
+    i = c;
+  }
+  if (*n > 0) h[i] = last;
+}
 
 static int slot_init(slot_t *s, int max_group, void *stream) {
   int rc;
@@ -178,6 +214,8 @@ int osc_serve_run(osc_request *reqs, int nreq, const osc_serve_cfg *cfg,
   void **streams = NULL;
   void **events = NULL; /* per-call mode: one event per request */
   req_state *rs = NULL;
+  int *ready = NULL;   /* arrived, not yet admitted (heap) */
+  int nready = 0;
   int *waiting = NULL; /* admitted, not yet launched (FIFO) */
   int nwait = 0, wait_head = 0;
   int *running = NULL; /* per-call mode: launched, not retired (FIFO) */
@@ -189,6 +227,10 @@ int osc_serve_run(osc_request *reqs, int nreq, const osc_serve_cfg *cfg,
   if (!osc_g.initialized) return osc_set_error(OSC_ERR_NOTINIT, "not initialised");
   if (reqs == NULL || nreq < 0 || cfg == NULL || st == NULL || cfg->max_inflight < 1)
     return osc_set_error(OSC_ERR_ARG, "serve arguments");
+  if ((cfg->order != OSC_SERVE_ORDER_ARRIVAL && cfg->order != OSC_SERVE_ORDER_SIZE) ||
+      !(cfg->age_bytes_per_s >= 0.0 && cfg->age_bytes_per_s <= 1e30))
+    return osc_set_error(OSC_ERR_ARG, "serve order %d / age weight %g", cfg->order,
+                         cfg->age_bytes_per_s);
   if (cfg->dest_base + cfg->dest_bytes > osc_g.heap_bytes || page % 256 != 0 ||
       cfg->dest_base % 256 != 0)
     return osc_set_error(OSC_ERR_ARG, "destination arena outside the heap or misaligned");
@@ -197,11 +239,12 @@ int osc_serve_run(osc_request *reqs, int nreq, const osc_serve_cfg *cfg,
       return osc_set_error(OSC_ERR_ARG, "arrivals must be non-decreasing (request %d)", i);
   memset(st, 0, sizeof(*st));
   rs = (req_state *)calloc((size_t)(nreq ? nreq : 1), sizeof(req_state));
+  ready = (int *)calloc((size_t)(nreq ? nreq : 1), sizeof(int));
   waiting = (int *)calloc((size_t)(nreq ? nreq : 1), sizeof(int));
   running = (int *)calloc((size_t)(nreq ? nreq : 1), sizeof(int));
   slots = (slot_t *)calloc((size_t)nslots, sizeof(slot_t));
   streams = (void **)calloc((size_t)nslots, sizeof(void *));
-  if (!rs || !waiting || !running || !slots || !streams) {
+  if (!rs || !ready || !waiting || !running || !slots || !streams) {
     rc = osc_set_error(OSC_ERR_NOMEM, "serve state");
     goto out;
   }
@@ -298,22 +341,21 @@ int osc_serve_run(osc_request *reqs, int nreq, const osc_serve_cfg *cfg,
       }
     }
     if (rc != OSC_OK) break;
-    /* 2. admit, in arrival order */
+    /* 2. arrivals: reject what can never run, queue the rest */
     now = now_s() - t0;
-    while (next < nreq && reqs[next].arrival_s <= now && inflight < cfg->max_inflight) {
+    while (next < nreq && reqs[next].arrival_s <= now) {
       osc_request *q = &reqs[next];
       const osc_call *c = &q->call;
       const uint64_t db = osc_call_dest_bytes(c);
-      const int np = (int)((db + page - 1) / page);
-      const int lo = c->team_start, hi = c->team_start + (c->team_npes - 1) * c->team_stride;
       const double area = (double)db * (double)c->team_npes;
-      int first = 0, vrc;
-      /* validate with the destination that will be assigned */
-      {
+      int vrc;
+      { /* validate with the destination that will be assigned */
         osc_call tmp = *c;
         tmp.dest_off = cfg->dest_base;
         vrc = osc_call_validate(&tmp, next);
-        if (vrc == OSC_OK && db > cfg->dest_bytes) vrc = OSC_ERR_BUDGET;
+        /* the arena holds dest_bytes / page whole pages */
+        if (vrc == OSC_OK && (db + page - 1) / page > cfg->dest_bytes / page)
+          vrc = OSC_ERR_BUDGET;
         if (vrc == OSC_OK && area > cfg->budget_bytes) vrc = OSC_ERR_BUDGET;
         if (vrc == OSC_OK && batched) {
           rs[next].tiles = osc_call_tile_count(&tmp);
@@ -325,32 +367,42 @@ int osc_serve_run(osc_request *reqs, int nreq, const osc_serve_cfg *cfg,
         q->admit_s = q->done_s = now;
         ++st->rejected;
         ++done;
-        ++next;
-        progress = 1;
-        continue;
+      } else {
+        rs[next].npages = (int)((db + page - 1) / page);
+        rs[next].lo = c->team_start;
+        rs[next].hi = c->team_start + (c->team_npes - 1) * c->team_stride;
+        rs[next].area = area;
+        rs[next].key = cfg->order == OSC_SERVE_ORDER_SIZE
+                           ? osc_call_alg_bytes(c) + cfg->age_bytes_per_s * q->arrival_s
+                           : q->arrival_s;
+        rq_push(ready, &nready, rs, next);
       }
-      if (inflight_area + area > cfg->budget_bytes) break;
-      if (np > 0) {
-        first = pm_find(&pm, lo, hi, np);
-        if (first < 0) break;
-        pm_mark(&pm, lo, hi, first, np, 1);
-      }
-      rs[next].first = first;
-      rs[next].npages = np;
-      rs[next].lo = lo;
-      rs[next].hi = hi;
-      rs[next].area = area;
-      q->call.dest_off = cfg->dest_base + (uint64_t)first * page;
-      q->admit_s = now;
-      inflight_area += area;
-      if (inflight_area > st->peak_inflight_bytes) st->peak_inflight_bytes = inflight_area;
-      ++inflight;
-      if (inflight > st->max_inflight_seen) st->max_inflight_seen = inflight;
-      waiting[nwait++] = next;
       ++next;
       progress = 1;
     }
-    /* 3. launch */
+    /* 3. admit the head of the ready queue while slots and memory allow */
+    while (nready > 0 && inflight < cfg->max_inflight) {
+      const int r = ready[0];
+      osc_request *q = &reqs[r];
+      int first = 0;
+      if (inflight_area + rs[r].area > cfg->budget_bytes) break;
+      if (rs[r].npages > 0) {
+        first = pm_find(&pm, rs[r].lo, rs[r].hi, rs[r].npages);
+        if (first < 0) break;
+        pm_mark(&pm, rs[r].lo, rs[r].hi, first, rs[r].npages, 1);
+      }
+      rq_pop(ready, &nready, rs);
+      rs[r].first = first;
+      q->call.dest_off = cfg->dest_base + (uint64_t)first * page;
+      q->admit_s = now;
+      inflight_area += rs[r].area;
+      if (inflight_area > st->peak_inflight_bytes) st->peak_inflight_bytes = inflight_area;
+      ++inflight;
+      if (inflight > st->max_inflight_seen) st->max_inflight_seen = inflight;
+      waiting[nwait++] = r;
+      progress = 1;
+    }
+    /* 4. launch */
     if (batched) {
       for (i = 0; i < nslots && wait_head < nwait; ++i) {
         slot_t *s = &slots[i];
@@ -451,6 +503,7 @@ out:
   free(slots);
   free(streams);
   free(rs);
+  free(ready);
   free(waiting);
   free(running);
   return rc;

@@ -415,6 +415,7 @@ def plan(calls, dest_base, dest_bytes, world_npes=None):
 
 
 SERVE_BATCHED, SERVE_PER_CALL = 0, 1
+ORDER_ARRIVAL, ORDER_SIZE = 0, 1
 
 
 class Request(C.Structure):
@@ -430,7 +431,8 @@ class ServeCfg(C.Structure):
                 ("nstreams", C.c_int32), ("max_group", C.c_int32),
                 ("dest_base", C.c_uint64), ("dest_bytes", C.c_uint64),
                 ("page_bytes", C.c_uint64), ("budget_bytes", C.c_double),
-                ("digests", C.c_int32), ("pad_", C.c_int32)]
+                ("digests", C.c_int32), ("order", C.c_int32),
+                ("age_bytes_per_s", C.c_double)]
 
 
 class ServeStats(C.Structure):

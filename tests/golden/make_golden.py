@@ -1,5 +1,5 @@
 """Writes tests/golden/golden.json: fixed vectors for the seeded generator
-and the reference reduce on the config-1 workload, produced by the CPU
+and the reference reduce on the 8-PE int sum workload, produced by the CPU
 oracle (oracle/liborc.so) and the numpy generator, which tests/ check
 against each other and against the GPU.
 
@@ -26,10 +26,10 @@ def main():
     out["hash3"] = [[s, p, i, int(O.lib().orc_hash3(s, p, i))]
                     for s in (0, 1, 12345) for p in (0, 5, 255)
                     for i in (0, 1, 65535, 4194303)]
-    # config 1: 8 PEs x 65536 int32 uniform in [-1000, 1000], seed 0
+    # plain int sum: 8 PEs x 65536 int32 uniform in [-1000, 1000], seed 0
     src = O.gen_int_range(0, range(8), 65536, -1000, 1000, "i32")
     dest = O.rec_dbl("sum", "i32", src)
-    out["c1"] = {
+    out["int_sum_8pe"] = {
         "npes": 8, "n": 65536, "seed": 0, "lo": -1000, "hi": 1000,
         "src_sha256": hashlib.sha256(src.tobytes()).hexdigest(),
         "dest_sha256": hashlib.sha256(dest.tobytes()).hexdigest(),

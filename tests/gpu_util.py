@@ -17,7 +17,7 @@ UINT_VIEW = {"f32": np.uint32, "f64": np.uint64, "c32": np.uint32,
 
 
 def inputs(rng, op, dtype, P, n):
-    """Per-PE inputs [P, n] of the kind the configs use."""
+    """Per-PE inputs [P, n] of the kinds the reduce workloads use."""
     npt = O.NP_DTYPE[dtype]
     if dtype in ("f32", "f64"):
         if op == "prod":

@@ -22,8 +22,8 @@ def test_hash_fixture():
         assert int(O.hash3(seed, pe, idx)) == val
 
 
-def test_c1_fixture():
-    g = load()["c1"]
+def test_int_sum_8pe_fixture():
+    g = load()["int_sum_8pe"]
     src = O.gen_int_range(g["seed"], range(g["npes"]), g["n"], g["lo"],
                           g["hi"], "i32")
     assert src.min() >= -1000 and src.max() <= 1000

@@ -2,7 +2,7 @@
 bit-exact against the oracle's step-by-step restatements of the reference
 (Bruck in-place fcollect with its byte rotation, shift-exchange alltoall,
 linear broadcast) and against the closed-form self-describing patterns of
-the data-movement configs."""
+the data-movement workloads."""
 import ctypes as C
 
 import numpy as np
@@ -98,8 +98,8 @@ def test_overlap_rejected(heap):
     heap.free(s)
 
 
-def test_c3a_alltoall_full_size():
-    """Config 3a: 64 PEs, 65,536-byte blocks of int64
+def test_alltoall_64pe_64KiB_full_size():
+    """alltoall, 64 PEs, 65,536-byte blocks of int64
     src_i[j*8192 + idx] = i<<40 | j<<20 | idx; every dest must read
     i<<40 | j<<20 | idx in block i of PE j."""
     P, blk = 64, 8192
@@ -115,8 +115,8 @@ def test_c3a_alltoall_full_size():
             assert np.array_equal(got, want), j
 
 
-def test_c3b_fcollect_full_size():
-    """Config 3b: 256 PEs x 1 MiB int32 (pe<<24 | idx); dest of sampled PEs
+def test_fcollect_256pe_1MiB_full_size():
+    """fcollect, 256 PEs x 1 MiB int32 (pe<<24 | idx); dest of sampled PEs
     is the PE-ordered concatenation."""
     P, n = 256, 262144
     with osc.Heap(npes=P, heap_bytes=(P + 1) * n * 4 + (1 << 20)) as h:

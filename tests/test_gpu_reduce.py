@@ -60,11 +60,11 @@ def team_of(h, P):
     return h.world if P == h.npes else h.split(0, 1, P)
 
 
-def test_c1_int_sum_world_of_8():
-    """Config 1: shmem_int_sum_reduce on WORLD, 8 PEs x 65536 int32 uniform
+def test_int_sum_world_of_8():
+    """shmem_int_sum_reduce on WORLD, 8 PEs x 65536 int32 uniform
     in [-1000, 1000] from seed 0 (generated on the device), bit-exact."""
     with open(os.path.join(HERE, "golden", "golden.json")) as f:
-        gold = json.load(f)["c1"]
+        gold = json.load(f)["int_sum_8pe"]
     P, n = 8, 65536
     with osc.Heap(npes=P, heap_bytes=4 << 20) as h:
         s, d = h.malloc(n * 4), h.malloc(n * 4)
@@ -207,8 +207,8 @@ def test_large_team_tree_path():
             assert bit_equal(got, O.rec_dbl(op, dtype, src), dtype)
 
 
-def test_c2_full_size_sampled():
-    """Config 2 shape: 64 PEs x 4,194,304 elements.  u64 xor and i64 sum
+def test_reduce_matrix_full_size_sampled():
+    """The reduce matrix shape: 64 PEs x 4,194,304 elements.  u64 xor and i64 sum
     (inputs regenerated on the host: bit-exact on 2048 sampled columns),
     f32 sum in accurate mode (rel. error vs the FP64 oracle <= 1e-6 on the
     sampled columns), and every member's dest identical."""

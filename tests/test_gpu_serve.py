@@ -111,3 +111,79 @@ def test_arrivals_must_be_ordered(sw):
     reqs[1].arrival_s, reqs[2].arrival_s = reqs[2].arrival_s, reqs[1].arrival_s
     with pytest.raises(osc.OscError):
         osc.serve(reqs, sw.cfg(osc.SERVE_BATCHED))
+
+
+def _alg_bytes(c):
+    """Algorithmic bytes of a call: 2*P*n*s (reduce), P*b + P*P*b
+    (fcollect), 2*P*P*b (alltoall)."""
+    P = c.team_npes
+    if c.kind == osc.CALL_REDUCE:
+        names = {v: k for k, v in osc.DTYPES.items()}
+        return 2 * P * c.count * np.dtype(osc.NP_DTYPE[names[c.dtype]]).itemsize
+    if c.kind == osc.CALL_FCOLLECT:
+        return P * c.count + P * P * c.count
+    return 2 * P * P * c.count
+
+
+def _at_once(reqs):
+    for r in reqs:
+        r.arrival_s = 0.0
+    return reqs
+
+
+@pytest.mark.parametrize("mode", [osc.SERVE_BATCHED, osc.SERVE_PER_CALL])
+def test_size_order_admits_shortest_first(sw, mode):
+    """Everything arrives at once and one request runs at a time, so the
+    launch order is the queue order: ascending algorithmic bytes (ties by
+    index) with aging off, index order in arrival order."""
+    reqs = _at_once(requests(sw, 6, 40, 1e3))
+    cfg = sw.cfg(mode, digests=0, max_inflight=1)
+    cfg.order, cfg.age_bytes_per_s = osc.ORDER_SIZE, 0.0
+    out, st = osc.serve(reqs, cfg)
+    assert st.completed == len(reqs)
+    launched = sorted(range(len(out)), key=lambda i: out[i].group)
+    want = sorted(range(len(out)), key=lambda i: (_alg_bytes(out[i].call), i))
+    assert launched == want
+    cfg.order = osc.ORDER_ARRIVAL
+    out, _ = osc.serve(_at_once(requests(sw, 6, 40, 1e3)), cfg)
+    assert sorted(range(len(out)), key=lambda i: out[i].group) == \
+        list(range(len(out)))
+
+
+def test_size_order_ages_waiting_requests(sw):
+    """Aging: a request that arrived earlier goes first unless the later one
+    is smaller by more than age_bytes_per_s times the arrival gap.  The
+    arrivals are picoseconds apart (all queued together); a weight that
+    makes each picosecond worth 1 GB gives arrival order, weight 0 with
+    simultaneous arrivals gives size order."""
+    reqs = requests(sw, 7, 30, 1e3)
+    for i, r in enumerate(reqs):
+        r.arrival_s = i * 1e-12
+    cfg = sw.cfg(osc.SERVE_BATCHED, digests=0, max_inflight=1)
+    cfg.order, cfg.age_bytes_per_s = osc.ORDER_SIZE, 1e21
+    assert max(_alg_bytes(r.call) for r in reqs) < 1e9
+    out, _ = osc.serve(reqs, cfg)
+    assert sorted(range(len(out)), key=lambda i: out[i].group) == \
+        list(range(len(out)))
+    cfg.age_bytes_per_s = 0.0
+    out, _ = osc.serve(_at_once(reqs), cfg)
+    assert sorted(range(len(out)), key=lambda i: out[i].group) == \
+        sorted(range(len(out)), key=lambda i: (_alg_bytes(out[i].call), i))
+
+
+def test_size_order_same_results_and_bad_order_rejected(sw):
+    reqs = requests(sw, 8, 120, 2e5)
+    cfg = sw.cfg(osc.SERVE_BATCHED)
+    cfg.order, cfg.age_bytes_per_s = osc.ORDER_SIZE, 1e11
+    out, st = osc.serve(reqs, cfg)
+    assert st.completed == len(reqs) and st.rejected == 0
+    for r in out:
+        assert r.arrival_s <= r.admit_s <= r.launch_s <= r.done_s
+        assert osc.request_alone(r.call, sw.scratch) == r.digest
+    bad = sw.cfg(osc.SERVE_BATCHED)
+    bad.order = 2
+    with pytest.raises(osc.OscError):
+        osc.serve(reqs[:2], bad)
+    bad.order, bad.age_bytes_per_s = osc.ORDER_SIZE, -1.0
+    with pytest.raises(osc.OscError):
+        osc.serve(reqs[:2], bad)
