@@ -458,13 +458,14 @@ int osc_spmd_split_strided(void *parent, int start, int stride, int size,
 static int lead_split_2d(meet *m, void *c) {
   split_ctx *s = (split_ctx *)c;
   const osc_team *p = osc_team_get(s->parent);
-  const int P = p->npes;
-  const int xr = s->xrange > P ? P : s->xrange;
-  const int nrows = (P + xr - 1) / xr;
-  const int ncols = xr;
+  int P, xr, nrows, ncols;
   osc_team **rows, **cols;
   int rank, rc;
-  if (s->xrange < 1) return osc_set_error(OSC_ERR_ARG, "bad 2d split");
+  if (p == NULL || s->xrange < 1) return osc_set_error(OSC_ERR_ARG, "bad 2d split");
+  P = p->npes;
+  xr = s->xrange > P ? P : s->xrange;
+  nrows = (P + xr - 1) / xr;
+  ncols = xr;
   rows = (osc_team **)calloc((size_t)nrows, sizeof(*rows));
   cols = (osc_team **)calloc((size_t)ncols, sizeof(*cols));
   if (rows == NULL || cols == NULL) {

@@ -145,3 +145,27 @@ def test_spmd_split_2d_and_team_reduce():
         row_sum = sum(range(row * xr, row * xr + xr))
         col_sum = sum(range(col, npes, xr))
         assert got[pe] == (0, 0, 0, row_sum, col_sum, col, row), (pe, got[pe])
+
+
+def test_spmd_split_2d_bad_xrange_fails_every_member():
+    """xrange < 1 is an argument error on every member (no team is made)
+    and the members go on to the next call."""
+    npes = 4
+    got = {}
+    L = osc.lib()
+
+    def pe_main(pe):
+        xt, yt = C.c_void_p(1), C.c_void_p(1)
+        rc = L.shmem_team_split_2d(osc.team_world(), 0, None, 0,
+                                   C.byref(xt), None, 0, C.byref(yt))
+        x2, y2 = C.c_void_p(), C.c_void_p()
+        ok = L.shmem_team_split_2d(osc.team_world(), 2, None, 0,
+                                   C.byref(x2), None, 0, C.byref(y2))
+        got[pe] = (rc, xt.value, yt.value, ok)
+        L.shmem_team_destroy(x2.value)
+        L.shmem_team_destroy(y2.value)
+        return 0
+    assert _spmd(npes, pe_main) == 0
+    for pe in range(npes):
+        rc, xt, yt, ok = got[pe]
+        assert rc != 0 and not xt and not yt and ok == 0, (pe, got[pe])
