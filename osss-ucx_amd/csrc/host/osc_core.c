@@ -81,8 +81,11 @@ int osc_init(int npes, size_t heap_bytes_per_pe, int device) {
   const char *lg = getenv("OSC_LOG");
   if (osc_g.initialized) return osc_set_error(OSC_ERR_STATE, "already initialised");
   const char *shp = getenv("OSC_REDUCE_SHAPE");
+  const char *spin = getenv("OSC_WAIT_SPIN_US");
   osc_g.log_calls = (lg != NULL && lg[0] != '\0' && lg[0] != '0');
   osc_g.reduce_shape = (shp != NULL && shp[0] != '\0') ? atoi(shp) : -1;
+  osc_g.wait_spin_us =
+      (spin != NULL && spin[0] != '\0') ? (unsigned)strtoul(spin, NULL, 10) : 500u;
   if (npes < 1 || npes > OSC_MAX_NPES)
     return osc_set_error(OSC_ERR_ARG, "npes %d outside [1, %d]", npes,
                          OSC_MAX_NPES);
@@ -109,8 +112,15 @@ int osc_init(int npes, size_t heap_bytes_per_pe, int device) {
     oscd_free(base);
     return osc_hip_error(rc, "hipStreamCreate");
   }
+  rc = oscd_wait_init();
+  if (rc != OSC_OK) {
+    oscd_stream_destroy(osc_g.stream);
+    oscd_free(base);
+    return osc_hip_error(rc, "completion slots");
+  }
   rc = osc_heap_init(heap_bytes_per_pe);
   if (rc != OSC_OK) {
+    oscd_wait_fini();
     oscd_stream_destroy(osc_g.stream);
     oscd_free(base);
     return osc_set_error(rc, "heap bookkeeping");
@@ -137,6 +147,7 @@ int osc_finalize(void) {
   if (!osc_g.initialized) return OSC_OK;
   oscd_stream_sync(osc_g.stream);
   oscd_stream_destroy(osc_g.stream);
+  oscd_wait_fini();
   if (osc_g.aux_dev) oscd_free(osc_g.aux_dev);
   if (osc_g.tab_event) oscd_event_destroy(osc_g.tab_event);
   if (osc_g.tab_dev) oscd_free(osc_g.tab_dev);
@@ -244,6 +255,11 @@ int osc_fill(void *team, void *sym, size_t nelems, int dtype, int dist,
   rc = oscd_fill(&g, off, nelems, dtype, dist, a, b, seed,
                  stream ? stream : osc_g.stream);
   return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "fill launch");
+}
+
+int osc_wait(void) {
+  const int rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
+  return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "stream wait");
 }
 
 int osc_stream_sync(void *stream) {

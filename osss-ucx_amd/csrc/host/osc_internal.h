@@ -49,6 +49,8 @@ typedef struct osc_state {
   unsigned char fpmode_to_all[OSC_OP_COUNT][OSC_T_COUNT]; /* active-set forms */
   int fp_default;
   int reduce_shape; /* -1 = per-policy default; OSC_REDUCE_SHAPE overrides */
+  unsigned wait_spin_us; /* blocking calls: bounded spin on a completion
+                            slot before hipStreamSynchronize (0: never) */
   void *aux_dev;    /* 256 device bytes for single-call descriptors */
   /* per-member (offset, size) table of the variable-size collect:
    * pinned staging, its device copy, and the event of its last use */
@@ -61,6 +63,9 @@ typedef struct osc_state {
 extern osc_state osc_g;
 extern osc_team osc_team_world;
 extern osc_team osc_team_shared;
+
+/* completion of everything enqueued on the library stream so far */
+int osc_wait(void);
 
 /* errors */
 int osc_set_error(int code, const char *fmt, ...);

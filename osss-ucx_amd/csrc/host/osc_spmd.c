@@ -190,8 +190,8 @@ typedef struct {
 
 static int synced(int rc) {
   if (rc != OSC_OK) return rc;
-  rc = oscd_stream_sync(osc_g.stream);
-  return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "hipStreamSynchronize");
+  rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
+  return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "stream wait");
 }
 
 static int lead_reduce(meet *m, void *c) {

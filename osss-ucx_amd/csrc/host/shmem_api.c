@@ -132,7 +132,7 @@ int osc_heap_call_alloc(size_t size, size_t align, int zero, size_t *off) {
     for (pe = 0; pe < osc_g.npes; ++pe)
       oscd_memset_async(osc_g.base + (size_t)pe * osc_g.slice + *off, 0, size,
                         osc_g.stream);
-    rc = oscd_stream_sync(osc_g.stream);
+    rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
     if (rc != OSC_OK) return osc_hip_error(rc, "calloc clear");
   }
   return OSC_OK;
@@ -141,7 +141,7 @@ int osc_heap_call_alloc(size_t size, size_t align, int zero, size_t *off) {
 int osc_heap_call_free(size_t off) {
   int rc;
   /* outstanding kernels may still use the object */
-  oscd_stream_sync(osc_g.stream);
+  oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
   pthread_mutex_lock(&osc_g.lock);
   rc = osc_heap_free(off);
   pthread_mutex_unlock(&osc_g.lock);
@@ -250,12 +250,12 @@ static void xfer(void *dst, const void *src, size_t nbytes, int blocking) {
   if (nbytes == 0) return;
   if (is_sym(dst) && is_sym(src)) {
     rc = oscd_copy(dst, src, nbytes, osc_g.stream);
-    if (rc == OSC_OK && blocking) rc = oscd_stream_sync(osc_g.stream);
+    if (rc == OSC_OK && blocking) rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
   } else if (is_sym(dst)) {
-    rc = oscd_stream_sync(osc_g.stream);
+    rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
     if (rc == OSC_OK) rc = oscd_memcpy_h2d(dst, src, nbytes);
   } else {
-    rc = oscd_stream_sync(osc_g.stream);
+    rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
     if (rc == OSC_OK) rc = oscd_memcpy_d2h(dst, src, nbytes);
   }
   if (rc != OSC_OK) osc_hip_error(rc, "RMA copy");
@@ -320,7 +320,7 @@ static void strided(void *dest, const void *src, size_t es, ptrdiff_t tst,
                 : oscd_copy_strided(lp, r, es, (uint64_t)(lst * (ptrdiff_t)es),
                                     (uint64_t)(rst * (ptrdiff_t)es), n,
                                     osc_g.stream);
-    if (rc == OSC_OK) rc = oscd_stream_sync(osc_g.stream);
+    if (rc == OSC_OK) rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
     if (rc != OSC_OK) osc_hip_error(rc, "strided RMA");
     return;
   }
@@ -335,16 +335,16 @@ static void strided(void *dest, const void *src, size_t es, ptrdiff_t tst,
     if (rc == OSC_OK && is_put) {
       for (t = 0; t < n; ++t)
         memcpy(host + t * es, (char *)lp + (ptrdiff_t)t * lst * (ptrdiff_t)es, es);
-      rc = oscd_stream_sync(osc_g.stream);
+      rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
       if (rc == OSC_OK) rc = oscd_memcpy_h2d(stage, host, n * es);
       if (rc == OSC_OK)
         rc = oscd_copy_strided(r, stage, es, (uint64_t)(rst * (ptrdiff_t)es), es,
                                n, osc_g.stream);
-      if (rc == OSC_OK) rc = oscd_stream_sync(osc_g.stream);
+      if (rc == OSC_OK) rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
     } else if (rc == OSC_OK) {
       rc = oscd_copy_strided(stage, r, es, es, (uint64_t)(rst * (ptrdiff_t)es), n,
                              osc_g.stream);
-      if (rc == OSC_OK) rc = oscd_stream_sync(osc_g.stream);
+      if (rc == OSC_OK) rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
       if (rc == OSC_OK) rc = oscd_memcpy_d2h(host, stage, n * es);
       if (rc == OSC_OK)
         for (t = 0; t < n; ++t)
@@ -373,7 +373,7 @@ static void put_signal(void *dest, const void *src, size_t nbytes,
   }
   xfer(d, src, nbytes, 0);
   rc = oscd_signal(s, signal, sig_op, osc_g.stream);
-  if (rc == OSC_OK && blocking) rc = oscd_stream_sync(osc_g.stream);
+  if (rc == OSC_OK && blocking) rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
   if (rc != OSC_OK) osc_hip_error(rc, "signal update");
 }
 
@@ -428,7 +428,7 @@ uint64_t shmem_signal_wait_until(uint64_t *sig_addr, int cmp,
     osc_set_error(OSC_ERR_ARG, "comparison %d", cmp);
     return 0;
   }
-  oscd_stream_sync(osc_g.stream);
+  oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
   for (;;) {
     if (read_signal(sig_addr, &v) != OSC_OK || cmp_ok(v, cmp, cmp_value))
       return v;
@@ -526,8 +526,8 @@ OSC_SIZED_RMA(OSC_DEF_SIZED_RMA)
 
 static int finish(int rc) {
   if (rc != OSC_OK) return rc;
-  rc = oscd_stream_sync(osc_g.stream);
-  return rc == OSC_OK ? 0 : osc_hip_error(rc, "hipStreamSynchronize");
+  rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
+  return rc == OSC_OK ? 0 : osc_hip_error(rc, "stream wait");
 }
 
 static int blocking_reduce(shmem_team_t team, int op, int dtype, void *dest,

@@ -41,6 +41,15 @@ int oscd_memset_async(void *dst, int value, size_t bytes, void *stream);
 int oscd_stream_create(void **s);
 int oscd_stream_destroy(void *s);
 int oscd_stream_sync(void *s);
+/* Completion wait: a one-wave kernel on stream s stores a ticket into its
+ * own slot of coherent pinned host memory with a system-scope release; the
+ * host spins on the slot for at most spin_us microseconds, then falls back
+ * to hipStreamSynchronize (spin_us = 0: synchronize at once).
+ * oscd_wait_init allocates the slots; oscd_wait_fini frees them once no
+ * wait is pending. */
+int oscd_wait_init(void);
+void oscd_wait_fini(void);
+int oscd_stream_wait(void *s, unsigned spin_us);
 int oscd_stream_query(void *s); /* 1 = idle, 0 = busy, <0 error */
 int oscd_event_create(void **e);
 int oscd_event_destroy(void *e);
