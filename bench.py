@@ -157,11 +157,27 @@ def cpu_baseline(seconds_budget=20.0):
         total_s += sec
         if time.time() - t0 > seconds_budget:
             break
+    # the other workloads, each a few seconds at most: the small plain
+    # reduce's latency (one thread per PE, as one process per PE), alltoall
+    # at full size, and fcollect on a sample (its byte-granular rotation
+    # runs at well under 1 GB/s, so 256 PEs x 1 MiB would take minutes)
+    other = {
+        "int_sum_8pe_64K_call_us": round(1e6 * O.cpu_bench_reduce(
+            "sum", "i32", 8, 65536, 51, min(8, threads)), 1)}
+    sec = O.cpu_bench_alltoall(64, 65536, 5, threads)
+    other["alltoall_64pe_64KiB_GBps"] = round(2 * 64 * 64 * 65536 / sec / 1e9, 2)
+    sec = O.cpu_bench_fcollect(64, 65536, 3, threads)
+    other["fcollect_64pe_64KiB_GBps"] = round((64 + 64 * 64) * 65536 / sec / 1e9,
+                                              3)
     return {"value": round(total_b / total_s / 1e9, 3), "unit": "GB/s",
             "cores": threads, "kind": "port",
             "sample": "rec_dbl restatement (oracle/orc.c), 64 PEs as %d "
                       "threads, nreduce=%d (1/16 of the workload), op/type "
-                      "matrix cells, median of 3" % (threads, n)}
+                      "matrix cells, median of 3" % (threads, n),
+            "other": other,
+            "other_sample": "medians: int sum 8 PEs x 65536 (51 calls), "
+                            "shift-exchange alltoall 64 PEs x 64 KiB (5), "
+                            "Bruck fcollect with rotation 64 PEs x 64 KiB (3)"}
 
 
 def _median_event_ms(L, stream, fn, reps=10):

@@ -219,6 +219,26 @@ def cpu_bench_reduce(op, dtype, npes, n, iters, threads) -> float:
     return out.value
 
 
+def cpu_bench_fcollect(npes, nbytes, iters, threads) -> float:
+    """Median seconds of the Bruck in-place fcollect restatement."""
+    out = C.c_double()
+    rc = lib().orc_cpu_bench_fcollect(npes, C.c_size_t(nbytes), iters, threads,
+                                      C.byref(out))
+    if rc != 0:
+        raise RuntimeError("cpu bench failed")
+    return out.value
+
+
+def cpu_bench_alltoall(npes, nbytes, iters, threads) -> float:
+    """Median seconds of the shift-exchange alltoall restatement."""
+    out = C.c_double()
+    rc = lib().orc_cpu_bench_alltoall(npes, C.c_size_t(nbytes), iters, threads,
+                                      C.byref(out))
+    if rc != 0:
+        raise RuntimeError("cpu bench failed")
+    return out.value
+
+
 # ---- independent numpy formulations (pins) ----------------------------------
 
 _G = np.uint64(0x9E3779B97F4A7C15)

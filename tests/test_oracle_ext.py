@@ -111,3 +111,11 @@ def test_strided_copy_negative_and_zero_strides():
     # zero target stride: elements land in order, the last one stays
     got = O.strided_copy(np.zeros(es, np.uint8), src, es, 0, 1, n)
     assert np.array_equal(got, src[(n - 1) * es:])
+
+
+def test_cpu_timing_harness_runs_every_collective():
+    """The CPU baseline's timing entry points run and return a positive
+    median time at tiny sizes."""
+    assert O.cpu_bench_reduce("sum", "i32", 3, 1000, 3, 2) > 0
+    assert O.cpu_bench_alltoall(5, 256, 3, 2) > 0
+    assert O.cpu_bench_fcollect(5, 256, 3, 2) > 0
