@@ -42,16 +42,40 @@ typedef struct shmem_team_config {
 
 #define SHMEM_TEAM_NUM_CONTEXTS (1 << 0)
 
-/* ---- library setup (reference src/api/init.c:100-211) ------------------- */
+/* ---- library setup (reference src/api/init.c:100-211) -------------------
+ * Thread levels, malloc hints and the name length take the reference's
+ * values (include/shmem/defs.h:93-98, :136-139; configure.ac:18). */
+enum {
+  SHMEM_THREAD_SINGLE = 0,
+  SHMEM_THREAD_FUNNELED,
+  SHMEM_THREAD_SERIALIZED,
+  SHMEM_THREAD_MULTIPLE
+};
+enum { SHMEM_MALLOC_ATOMICS_REMOTE = 0, SHMEM_MALLOC_SIGNAL_REMOTE };
+#define SHMEM_MAJOR_VERSION 1
+#define SHMEM_MINOR_VERSION 5
+#define SHMEM_MAX_NAME_LEN 64
+
 void shmem_init(void);
+/* 0 on success, nonzero (no abort) when the library cannot start; the
+ * level provided is SHMEM_THREAD_MULTIPLE at most in team-call mode and
+ * SHMEM_THREAD_FUNNELED at most in the SPMD mode (each PE's own thread) */
+int shmem_init_thread(int requested, int *provided);
+void shmem_query_thread(int *provided);
 void shmem_finalize(void);
+int shmem_initialized(void);
+int shmem_finalized(void);
+/* flushes stdio and ends the process (every PE) with status */
+void shmem_global_exit(int status);
 int shmem_my_pe(void);
 int shmem_n_pes(void);
+int shmem_pe_accessible(int pe);
 void shmem_barrier_all(void);
 void shmem_quiet(void);
 void shmem_fence(void);
 void shmem_info_get_version(int *major, int *minor);
 void shmem_info_get_name(char *name);
+void shmem_pcontrol(const int level, ...); /* profiling control: no effect */
 
 /* ---- symmetric heap (reference src/api/shmalloc.c:47-195) --------------- */
 void *shmem_malloc(size_t size);
@@ -59,7 +83,12 @@ void *shmem_calloc(size_t count, size_t size);
 void *shmem_align(size_t alignment, size_t size);
 void *shmem_realloc(void *ptr, size_t size);
 void shmem_free(void *ptr);
+/* hints only describe expected use; the block is a plain shmem_malloc */
+void *shmem_malloc_with_hints(size_t size, long hints);
+/* the device address of dest's copy on PE pe (a global PE for shmem_ptr,
+ * a team index for shmem_team_ptr), NULL if pe is not one */
 void *shmem_ptr(const void *dest, int pe);
+void *shmem_team_ptr(shmem_team_t team, const void *dest, int pe);
 int shmem_addr_accessible(const void *addr, int pe);
 
 /* ---- teams (reference src/api/teams/teams.c, src/shmemc/teams.c) -------- */
@@ -380,6 +409,132 @@ void shmem_sync_all(void);
   OSC_SYNC_PICK(__VA_ARGS__, shmem_sync_deprecated, shmem_sync_deprecated,     \
                 shmem_sync_deprecated, shmem_team_sync)                        \
   (__VA_ARGS__)
+
+/* ---- C11 type-generic routines -----------------------------------------
+ * The OpenSHMEM 1.5 type-generic forms (reference include/shmem/generics.h)
+ * over the typed routines above.  The routine is chosen by the element type
+ * of the destination (of the source for shmem_g): the controlling
+ * expression *(p) is not evaluated, and its lvalue conversion drops
+ * qualifiers.  Only distinct C types are listed.  On LP64 Linux every
+ * <stdint.h> and <stddef.h> type is a typedef of one of them (int64_t of
+ * long, size_t of unsigned long, ...), so it selects the routine of that C
+ * type, which has the same element size and arithmetic.  An element type
+ * outside a routine's table does not compile.  There are no contexts in
+ * this library, so there are no shmem_ctx_t forms. */
+#if defined(__STDC_VERSION__) && __STDC_VERSION__ >= 201112L && !defined(__cplusplus)
+#define OSC_GEN_INTS(_op)                                                      \
+  char: shmem_char_##_op, signed char: shmem_schar_##_op,                      \
+  short: shmem_short_##_op, int: shmem_int_##_op, long: shmem_long_##_op,      \
+  long long: shmem_longlong_##_op, unsigned char: shmem_uchar_##_op,           \
+  unsigned short: shmem_ushort_##_op, unsigned int: shmem_uint_##_op,          \
+  unsigned long: shmem_ulong_##_op, unsigned long long: shmem_ulonglong_##_op
+#define OSC_GEN_REALS(_op)                                                     \
+  float: shmem_float_##_op, double: shmem_double_##_op,                        \
+  long double: shmem_longdouble_##_op
+/* RMA and data-movement table; min/max table */
+#define OSC_GEN_STD(_p, _op) _Generic(*(_p), OSC_GEN_INTS(_op), OSC_GEN_REALS(_op))
+/* and/or/xor table: unsigned types, int8..int64 */
+#define OSC_GEN_BITWISE(_p, _op)                                               \
+  _Generic(*(_p), unsigned char: shmem_uchar_##_op,                            \
+           unsigned short: shmem_ushort_##_op, unsigned int: shmem_uint_##_op, \
+           unsigned long: shmem_ulong_##_op,                                   \
+           unsigned long long: shmem_ulonglong_##_op,                          \
+           signed char: shmem_int8_##_op, short: shmem_int16_##_op,            \
+           int: shmem_int32_##_op, long: shmem_int64_##_op)
+/* sum/prod table: min/max plus the complex types */
+#define OSC_GEN_ARITH(_p, _op)                                                 \
+  _Generic(*(_p), OSC_GEN_INTS(_op), OSC_GEN_REALS(_op),                       \
+           float _Complex: shmem_complexf_##_op,                               \
+           double _Complex: shmem_complexd_##_op)
+/* the active-set (to_all) tables */
+#define OSC_GEN_TO_ALL_INTS(_op)                                               \
+  short: shmem_short_##_op, int: shmem_int_##_op, long: shmem_long_##_op,      \
+  long long: shmem_longlong_##_op
+#define OSC_GEN_TO_ALL_BITWISE(_p, _op) _Generic(*(_p), OSC_GEN_TO_ALL_INTS(_op))
+#define OSC_GEN_TO_ALL_MINMAX(_p, _op)                                         \
+  _Generic(*(_p), OSC_GEN_TO_ALL_INTS(_op), OSC_GEN_REALS(_op))
+#define OSC_GEN_TO_ALL_ARITH(_p, _op)                                          \
+  _Generic(*(_p), OSC_GEN_TO_ALL_INTS(_op), OSC_GEN_REALS(_op),                \
+           float _Complex: shmem_complexf_##_op,                               \
+           double _Complex: shmem_complexd_##_op)
+
+#define shmem_put(dest, source, nelems, pe)                                    \
+  OSC_GEN_STD(dest, put)(dest, source, nelems, pe)
+#define shmem_get(dest, source, nelems, pe)                                    \
+  OSC_GEN_STD(dest, get)(dest, source, nelems, pe)
+#define shmem_put_nbi(dest, source, nelems, pe)                                \
+  OSC_GEN_STD(dest, put_nbi)(dest, source, nelems, pe)
+#define shmem_get_nbi(dest, source, nelems, pe)                                \
+  OSC_GEN_STD(dest, get_nbi)(dest, source, nelems, pe)
+#define shmem_p(dest, value, pe) OSC_GEN_STD(dest, p)(dest, value, pe)
+#define shmem_g(source, pe) OSC_GEN_STD(source, g)(source, pe)
+#define shmem_iput(dest, source, dst, sst, nelems, pe)                         \
+  OSC_GEN_STD(dest, iput)(dest, source, dst, sst, nelems, pe)
+#define shmem_iget(dest, source, dst, sst, nelems, pe)                         \
+  OSC_GEN_STD(dest, iget)(dest, source, dst, sst, nelems, pe)
+#define shmem_put_signal(dest, source, nelems, sig_addr, signal, sig_op, pe)  \
+  OSC_GEN_STD(dest, put_signal)(dest, source, nelems, sig_addr, signal,        \
+                                sig_op, pe)
+#define shmem_put_signal_nbi(dest, source, nelems, sig_addr, signal, sig_op,  \
+                             pe)                                               \
+  OSC_GEN_STD(dest, put_signal_nbi)(dest, source, nelems, sig_addr, signal,    \
+                                    sig_op, pe)
+
+#define shmem_and_reduce(team, dest, source, nreduce)                          \
+  OSC_GEN_BITWISE(dest, and_reduce)(team, dest, source, nreduce)
+#define shmem_or_reduce(team, dest, source, nreduce)                           \
+  OSC_GEN_BITWISE(dest, or_reduce)(team, dest, source, nreduce)
+#define shmem_xor_reduce(team, dest, source, nreduce)                          \
+  OSC_GEN_BITWISE(dest, xor_reduce)(team, dest, source, nreduce)
+#define shmem_min_reduce(team, dest, source, nreduce)                          \
+  OSC_GEN_STD(dest, min_reduce)(team, dest, source, nreduce)
+#define shmem_max_reduce(team, dest, source, nreduce)                          \
+  OSC_GEN_STD(dest, max_reduce)(team, dest, source, nreduce)
+#define shmem_sum_reduce(team, dest, source, nreduce)                          \
+  OSC_GEN_ARITH(dest, sum_reduce)(team, dest, source, nreduce)
+#define shmem_prod_reduce(team, dest, source, nreduce)                         \
+  OSC_GEN_ARITH(dest, prod_reduce)(team, dest, source, nreduce)
+
+#define shmem_fcollect(team, dest, source, nelems)                             \
+  OSC_GEN_STD(dest, fcollect)(team, dest, source, nelems)
+#define shmem_collect(team, dest, source, nelems)                              \
+  OSC_GEN_STD(dest, collect)(team, dest, source, nelems)
+#define shmem_alltoall(team, dest, source, nelems)                             \
+  OSC_GEN_STD(dest, alltoall)(team, dest, source, nelems)
+#define shmem_alltoalls(team, dest, source, dst, sst, nelems)                  \
+  OSC_GEN_STD(dest, alltoalls)(team, dest, source, dst, sst, nelems)
+#define shmem_broadcast(team, dest, source, nelems, PE_root)                   \
+  OSC_GEN_STD(dest, broadcast)(team, dest, source, nelems, PE_root)
+
+#define shmem_and_to_all(target, source, nreduce, PE_start, logPE_stride,     \
+                         PE_size, pWrk, pSync)                                 \
+  OSC_GEN_TO_ALL_BITWISE(target, and_to_all)(target, source, nreduce, PE_start,  \
+      logPE_stride, PE_size, pWrk, pSync)
+#define shmem_or_to_all(target, source, nreduce, PE_start, logPE_stride,      \
+                        PE_size, pWrk, pSync)                                  \
+  OSC_GEN_TO_ALL_BITWISE(target, or_to_all)(target, source, nreduce, PE_start,  \
+      logPE_stride, PE_size, pWrk, pSync)
+#define shmem_xor_to_all(target, source, nreduce, PE_start, logPE_stride,     \
+                         PE_size, pWrk, pSync)                                 \
+  OSC_GEN_TO_ALL_BITWISE(target, xor_to_all)(target, source, nreduce, PE_start,  \
+      logPE_stride, PE_size, pWrk, pSync)
+#define shmem_min_to_all(target, source, nreduce, PE_start, logPE_stride,     \
+                         PE_size, pWrk, pSync)                                 \
+  OSC_GEN_TO_ALL_MINMAX(target, min_to_all)(target, source, nreduce, PE_start,  \
+      logPE_stride, PE_size, pWrk, pSync)
+#define shmem_max_to_all(target, source, nreduce, PE_start, logPE_stride,     \
+                         PE_size, pWrk, pSync)                                 \
+  OSC_GEN_TO_ALL_MINMAX(target, max_to_all)(target, source, nreduce, PE_start,  \
+      logPE_stride, PE_size, pWrk, pSync)
+#define shmem_sum_to_all(target, source, nreduce, PE_start, logPE_stride,     \
+                         PE_size, pWrk, pSync)                                 \
+  OSC_GEN_TO_ALL_ARITH(target, sum_to_all)(target, source, nreduce, PE_start,  \
+      logPE_stride, PE_size, pWrk, pSync)
+#define shmem_prod_to_all(target, source, nreduce, PE_start, logPE_stride,    \
+                          PE_size, pWrk, pSync)                                \
+  OSC_GEN_TO_ALL_ARITH(target, prod_to_all)(target, source, nreduce, PE_start,  \
+      logPE_stride, PE_size, pWrk, pSync)
+#endif /* C11 */
 
 #ifdef __cplusplus
 }

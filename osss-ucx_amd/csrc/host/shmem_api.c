@@ -13,6 +13,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include "../../../include/shmem.h"
 #include "osc_internal.h"
@@ -54,7 +55,12 @@ int osc_parse_size(const char *s, size_t *out);
 
 /* ---- setup ----------------------------------------------------------------- */
 
-void shmem_init(void) {
+/* shmem_finalize was called: by the process (team-call mode) or by this
+ * PE's thread (SPMD mode); the same for shmem_init in the SPMD mode */
+static int g_finalized;
+static __thread int t_pe_inited, t_pe_finalized;
+
+static int start_from_env(void) {
   const char *np = getenv("OSC_NPES");
   const char *hs = getenv("SHMEM_SYMMETRIC_SIZE");
   const char *dv = getenv("OSC_DEVICE");
@@ -62,28 +68,69 @@ void shmem_init(void) {
   int npes = np ? atoi(np) : 8;
   int rc;
   if (osc_g.spmd) { /* the launcher initialised; every PE meets here */
-    osc_spmd_sync(SHMEM_TEAM_WORLD);
-    return;
+    t_pe_inited = 1;
+    t_pe_finalized = 0;
+    return osc_spmd_sync(SHMEM_TEAM_WORLD);
   }
-  if (osc_g.initialized) return;
-  if (hs != NULL && osc_parse_size(hs, &heap) != OSC_OK) {
-    fprintf(stderr, "shmem_init: bad SHMEM_SYMMETRIC_SIZE=%s\n", hs);
-    abort();
-  }
+  if (osc_g.initialized) return OSC_OK;
+  if (hs != NULL && osc_parse_size(hs, &heap) != OSC_OK)
+    return osc_set_error(OSC_ERR_ARG, "bad SHMEM_SYMMETRIC_SIZE=%s", hs);
   rc = osc_init(npes, heap, dv ? atoi(dv) : 0);
-  if (rc != OSC_OK) {
+  if (rc == OSC_OK) g_finalized = 0;
+  return rc;
+}
+
+void shmem_init(void) {
+  const int rc = start_from_env();
+  if (rc != OSC_OK) { /* fatal, as the reference's init */
     fprintf(stderr, "shmem_init: %s (%s)\n", osc_strerror(rc), osc_last_error());
     abort();
   }
 }
 
-/* In the SPMD mode the launcher finalises after every PE has returned. */
+static int thread_level(int requested) {
+  const int cap = osc_g.spmd ? SHMEM_THREAD_FUNNELED : SHMEM_THREAD_MULTIPLE;
+  return requested < cap ? requested : cap;
+}
+
+int shmem_init_thread(int requested, int *provided) {
+  if (requested < SHMEM_THREAD_SINGLE || requested > SHMEM_THREAD_MULTIPLE)
+    return osc_set_error(OSC_ERR_ARG, "thread level %d", requested), -1;
+  if (start_from_env() != OSC_OK) return -1;
+  if (provided != NULL) *provided = thread_level(requested);
+  return 0;
+}
+
+void shmem_query_thread(int *provided) {
+  if (provided != NULL) *provided = thread_level(SHMEM_THREAD_MULTIPLE);
+}
+
 void shmem_finalize(void) {
   if (osc_g.spmd) {
     osc_spmd_sync(SHMEM_TEAM_WORLD);
+    t_pe_finalized = 1;
     return;
   }
+  if (osc_g.initialized) g_finalized = 1;
   osc_finalize();
+}
+
+int shmem_initialized(void) {
+  if (osc_g.spmd) return t_pe_inited && !t_pe_finalized;
+  return osc_g.initialized;
+}
+
+int shmem_finalized(void) { return osc_g.spmd ? t_pe_finalized : g_finalized; }
+
+void shmem_global_exit(int status) {
+  fflush(NULL);
+  _exit(status);
+}
+
+void shmem_pcontrol(const int level, ...) { (void)level; }
+
+int shmem_pe_accessible(int pe) {
+  return osc_g.initialized && pe >= 0 && pe < osc_g.npes;
 }
 
 int shmem_my_pe(void) { return osc_g.initialized ? osc_my_pe_get() : -1; }
@@ -214,6 +261,17 @@ void *shmem_realloc(void *ptr, size_t size) {
 }
 
 void *shmem_ptr(const void *dest, int pe) { return osc_pe_ptr(dest, pe); }
+
+void *shmem_team_ptr(shmem_team_t team, const void *dest, int pe) {
+  const osc_team *t = osc_team_get(team);
+  if (t == NULL || pe < 0 || pe >= t->npes) return NULL;
+  return osc_pe_ptr(dest, t->start + pe * t->stride);
+}
+
+void *shmem_malloc_with_hints(size_t size, long hints) {
+  (void)hints;
+  return shmem_malloc(size);
+}
 
 int shmem_addr_accessible(const void *addr, int pe) {
   return osc_pe_ptr(addr, pe) != NULL;

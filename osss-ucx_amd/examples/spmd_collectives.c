@@ -27,9 +27,14 @@ int main(void) {
   int *isrc, *idst, ihost[64], iout[64];
   shmem_team_t even = SHMEM_TEAM_INVALID;
 
-  shmem_init();
+  int level = -1;
+  if (shmem_init_thread(SHMEM_THREAD_FUNNELED, &level) != 0) return 1;
   me = shmem_my_pe();
   npes = shmem_n_pes();
+  fails += check(level == SHMEM_THREAD_FUNNELED && shmem_initialized() &&
+                     !shmem_finalized() && shmem_pe_accessible(npes - 1) &&
+                     !shmem_pe_accessible(npes),
+                 me, "init_thread / initialized / pe_accessible");
   host = (long *)malloc(sizeof(long) * N * (size_t)npes);
   out = (long *)malloc(sizeof(long) * N * (size_t)npes);
   src = (long *)shmem_malloc(sizeof(long) * N * (size_t)npes);
@@ -49,7 +54,7 @@ int main(void) {
   idst = (int *)shmem_malloc(sizeof(int) * 64);
   for (i = 0; i < 64; ++i) ihost[i] = (i % npes == me) ? 1000 + i : -i;
   shmem_putmem(isrc, ihost, sizeof(ihost), me);
-  shmem_int_max_reduce(SHMEM_TEAM_WORLD, idst, isrc, 64);
+  shmem_max_reduce(SHMEM_TEAM_WORLD, idst, isrc, 64); /* C11 generic */
   shmem_getmem(iout, idst, sizeof(iout), me);
   for (i = 0, k = 0; i < 64; ++i) k |= iout[i] != 1000 + i;
   fails += check(!k, me, "int max reduce");
@@ -77,7 +82,7 @@ int main(void) {
   for (i = 0; i < npes; ++i)
     for (k = 0; k < 8; ++k) host[i * 8 + k] = (long)me * 100 + i;
   shmem_putmem(src, host, sizeof(long) * 8 * npes, me);
-  shmem_long_alltoall(SHMEM_TEAM_WORLD, dst, src, 8);
+  shmem_alltoall(SHMEM_TEAM_WORLD, dst, src, 8); /* C11 generic */
   shmem_getmem(out, dst, sizeof(long) * 8 * npes, me);
   for (i = 0, k = 0; i < npes * 8; ++i) k |= out[i] != (long)(i / 8) * 100 + me;
   fails += check(!k, me, "alltoall");
@@ -116,7 +121,7 @@ int main(void) {
     static long pWrk[SHMEM_REDUCE_MIN_WRKDATA_SIZE];
     for (i = 0; i < N; ++i) host[i] = (i % npes == me) ? -5 : i;
     shmem_putmem(src, host, sizeof(long) * N, me);
-    shmem_long_min_to_all(dst, src, N, 0, 0, npes, pWrk, pSync);
+    shmem_min_to_all(dst, src, N, 0, 0, npes, pWrk, pSync); /* C11 generic */
     shmem_getmem(out, dst, sizeof(long) * N, me);
     for (i = 0, k = 0; i < N; ++i) k |= out[i] != -5;
     fails += check(!k, me, "long min to_all");
@@ -155,6 +160,7 @@ int main(void) {
   free(host);
   free(out);
   shmem_finalize();
+  fails += check(shmem_finalized() && !shmem_initialized(), me, "finalized");
   printf("PE %d of %d: %s\n", me, npes, fails ? "FAIL" : "PASS");
   return fails;
 }
