@@ -198,6 +198,74 @@ uint64_t shmem_signal_fetch(const uint64_t *sig_addr);
 uint64_t shmem_signal_wait_until(uint64_t *sig_addr, int cmp,
                                  uint64_t cmp_value);
 
+/* ---- point-to-point synchronization (reference include/shmem/api.h:
+ * 1472-2020) ---------------------------------------------------------------
+ * On symmetric variables of the calling PE's copy (device memory, read by
+ * the host).  Elements whose status entry is nonzero are excluded; the
+ * _any forms return SIZE_MAX and the _some forms 0 when nothing is
+ * included.  The wait forms block until another thread's put satisfies
+ * them.  Types: the standard AMO table (api_types.h:93-105), plus short
+ * and unsigned short for the deprecated test/wait_until/wait. */
+#define OSC_P2P_SYNC_TYPES(X)                                                  \
+  X(int, int)                                                                  \
+  X(long, long)                                                                \
+  X(long long, longlong)                                                       \
+  X(unsigned int, uint)                                                        \
+  X(unsigned long, ulong)                                                      \
+  X(unsigned long long, ulonglong)                                             \
+  X(int32_t, int32)                                                            \
+  X(int64_t, int64)                                                            \
+  X(uint32_t, uint32)                                                          \
+  X(uint64_t, uint64)                                                          \
+  X(size_t, size)                                                              \
+  X(ptrdiff_t, ptrdiff)
+#define OSC_P2P_DEPR_TYPES(X) X(short, short) X(unsigned short, ushort)
+#define OSC_DECL_TEST_WAIT(_type, _typename)                                   \
+  int shmem_##_typename##_test(_type *ivar, int cmp, _type cmp_value);         \
+  void shmem_##_typename##_wait_until(_type *ivar, int cmp, _type cmp_value);  \
+  void shmem_##_typename##_wait(_type *ivar, _type cmp_value);
+#define OSC_DECL_P2P_SETS(_type, _typename)                                    \
+  int shmem_##_typename##_test_all(_type *ivars, size_t nelems,                \
+                                   const int *status, int cmp,                 \
+                                   _type cmp_value);                           \
+  size_t shmem_##_typename##_test_any(_type *ivars, size_t nelems,             \
+                                      const int *status, int cmp,              \
+                                      _type cmp_value);                        \
+  size_t shmem_##_typename##_test_some(_type *ivars, size_t nelems,            \
+                                       size_t *indices, const int *status,     \
+                                       int cmp, _type cmp_value);              \
+  void shmem_##_typename##_wait_until_all(_type *ivars, size_t nelems,         \
+                                          const int *status, int cmp,          \
+                                          _type cmp_value);                    \
+  size_t shmem_##_typename##_wait_until_any(_type *ivars, size_t nelems,       \
+                                            const int *status, int cmp,        \
+                                            _type cmp_value);                  \
+  size_t shmem_##_typename##_wait_until_some(_type *ivars, size_t nelems,      \
+                                             size_t *indices,                  \
+                                             const int *status, int cmp,       \
+                                             _type cmp_value);                 \
+  int shmem_##_typename##_test_all_vector(_type *ivars, size_t nelems,         \
+                                          const int *status, int cmp,          \
+                                          _type *cmp_values);                  \
+  size_t shmem_##_typename##_test_any_vector(_type *ivars, size_t nelems,      \
+                                             const int *status, int cmp,       \
+                                             _type *cmp_values);               \
+  size_t shmem_##_typename##_test_some_vector(                                 \
+      _type *ivars, size_t nelems, size_t *indices, const int *status,         \
+      int cmp, _type *cmp_values);                                             \
+  void shmem_##_typename##_wait_until_all_vector(_type *ivars, size_t nelems,  \
+                                                 const int *status, int cmp,   \
+                                                 _type *cmp_values);           \
+  size_t shmem_##_typename##_wait_until_any_vector(                            \
+      _type *ivars, size_t nelems, const int *status, int cmp,                 \
+      _type *cmp_values);                                                      \
+  size_t shmem_##_typename##_wait_until_some_vector(                           \
+      _type *ivars, size_t nelems, size_t *indices, const int *status,         \
+      int cmp, _type *cmp_values);
+OSC_P2P_SYNC_TYPES(OSC_DECL_TEST_WAIT)
+OSC_P2P_DEPR_TYPES(OSC_DECL_TEST_WAIT)
+OSC_P2P_SYNC_TYPES(OSC_DECL_P2P_SETS)
+
 /* ---- type tables -------------------------------------------------------
  * The sets of <T> each collective is defined for.  The lists are those of
  * the OpenSHMEM 1.5 specification (Tables 5 and 10), as in the reference's
@@ -534,6 +602,53 @@ void shmem_sync_all(void);
                           PE_size, pWrk, pSync)                                \
   OSC_GEN_TO_ALL_ARITH(target, prod_to_all)(target, source, nreduce, PE_start,  \
       logPE_stride, PE_size, pWrk, pSync)
+/* point-to-point synchronization: the AMO table; short and unsigned short
+ * also for the deprecated single-variable forms */
+#define OSC_GEN_SYNC_INTS(_op)                                                 \
+  int: shmem_int_##_op, long: shmem_long_##_op,                                \
+  long long: shmem_longlong_##_op, unsigned int: shmem_uint_##_op,             \
+  unsigned long: shmem_ulong_##_op, unsigned long long: shmem_ulonglong_##_op
+#define OSC_GEN_SYNC(_p, _op) _Generic(*(_p), OSC_GEN_SYNC_INTS(_op))
+#define OSC_GEN_SYNC1(_p, _op)                                                 \
+  _Generic(*(_p), OSC_GEN_SYNC_INTS(_op), short: shmem_short_##_op,            \
+           unsigned short: shmem_ushort_##_op)
+#define shmem_test(ivar, cmp, cmp_value)                                       \
+  OSC_GEN_SYNC1(ivar, test)(ivar, cmp, cmp_value)
+#define shmem_wait_until(ivar, cmp, cmp_value)                                 \
+  OSC_GEN_SYNC1(ivar, wait_until)(ivar, cmp, cmp_value)
+#define shmem_wait(ivar, cmp_value) OSC_GEN_SYNC1(ivar, wait)(ivar, cmp_value)
+#define shmem_test_all(ivars, nelems, status, cmp, cmp_value)                  \
+  OSC_GEN_SYNC(ivars, test_all)(ivars, nelems, status, cmp, cmp_value)
+#define shmem_test_any(ivars, nelems, status, cmp, cmp_value)                  \
+  OSC_GEN_SYNC(ivars, test_any)(ivars, nelems, status, cmp, cmp_value)
+#define shmem_test_some(ivars, nelems, indices, status, cmp, cmp_value)        \
+  OSC_GEN_SYNC(ivars, test_some)(ivars, nelems, indices, status, cmp,          \
+                                 cmp_value)
+#define shmem_wait_until_all(ivars, nelems, status, cmp, cmp_value)            \
+  OSC_GEN_SYNC(ivars, wait_until_all)(ivars, nelems, status, cmp, cmp_value)
+#define shmem_wait_until_any(ivars, nelems, status, cmp, cmp_value)            \
+  OSC_GEN_SYNC(ivars, wait_until_any)(ivars, nelems, status, cmp, cmp_value)
+#define shmem_wait_until_some(ivars, nelems, indices, status, cmp, cmp_value)  \
+  OSC_GEN_SYNC(ivars, wait_until_some)(ivars, nelems, indices, status, cmp,    \
+                                       cmp_value)
+#define shmem_test_all_vector(ivars, nelems, status, cmp, cmp_values)          \
+  OSC_GEN_SYNC(ivars, test_all_vector)(ivars, nelems, status, cmp, cmp_values)
+#define shmem_test_any_vector(ivars, nelems, status, cmp, cmp_values)          \
+  OSC_GEN_SYNC(ivars, test_any_vector)(ivars, nelems, status, cmp, cmp_values)
+#define shmem_test_some_vector(ivars, nelems, indices, status, cmp,            \
+                               cmp_values)                                     \
+  OSC_GEN_SYNC(ivars, test_some_vector)(ivars, nelems, indices, status, cmp,   \
+                                        cmp_values)
+#define shmem_wait_until_all_vector(ivars, nelems, status, cmp, cmp_values)    \
+  OSC_GEN_SYNC(ivars, wait_until_all_vector)(ivars, nelems, status, cmp,       \
+                                             cmp_values)
+#define shmem_wait_until_any_vector(ivars, nelems, status, cmp, cmp_values)    \
+  OSC_GEN_SYNC(ivars, wait_until_any_vector)(ivars, nelems, status, cmp,       \
+                                             cmp_values)
+#define shmem_wait_until_some_vector(ivars, nelems, indices, status, cmp,      \
+                                     cmp_values)                               \
+  OSC_GEN_SYNC(ivars, wait_until_some_vector)(ivars, nelems, indices, status,  \
+                                              cmp, cmp_values)
 #endif /* C11 */
 
 #ifdef __cplusplus

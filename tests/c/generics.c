@@ -68,6 +68,29 @@ void uses(shmem_team_t t, int pe) {
   shmem_sum_to_all((double _Complex *)0, (const double _Complex *)0, 4, 0, 0,
                    2, (double _Complex *)0, psync);
   shmem_prod_to_all(ld, ls, 4, 0, 0, 2, pwrk, psync);
+  {
+    int st[4] = {0, 0, 0, 0};
+    size_t idx[4];
+    long cv[4] = {1, 2, 3, 4};
+    unsigned short *us = 0;
+    size_t k = 0;
+    k += (size_t)shmem_test(ld, SHMEM_CMP_EQ, 1L);
+    shmem_wait_until((int *)0, SHMEM_CMP_GE, 1);
+    shmem_wait(us, (unsigned short)0);
+    k += (size_t)shmem_test_all(ld, 4, st, SHMEM_CMP_NE, 0L);
+    k += shmem_test_any((uint64_t *)0, 4, st, SHMEM_CMP_GT, (uint64_t)0);
+    k += shmem_test_some((size_t *)0, 4, idx, st, SHMEM_CMP_LT, (size_t)9);
+    shmem_wait_until_all(ld, 4, NULL, SHMEM_CMP_EQ, 1L);
+    k += shmem_wait_until_any(ld, 4, st, SHMEM_CMP_EQ, 1L);
+    k += shmem_wait_until_some(ld, 4, idx, st, SHMEM_CMP_EQ, 1L);
+    k += (size_t)shmem_test_all_vector(ld, 4, st, SHMEM_CMP_LE, cv);
+    k += shmem_test_any_vector(ld, 4, st, SHMEM_CMP_LE, cv);
+    k += shmem_test_some_vector(ld, 4, idx, st, SHMEM_CMP_LE, cv);
+    shmem_wait_until_all_vector(ld, 4, st, SHMEM_CMP_GE, cv);
+    k += shmem_wait_until_any_vector(ld, 4, st, SHMEM_CMP_GE, cv);
+    k += shmem_wait_until_some_vector(ld, 4, idx, st, SHMEM_CMP_GE, cv);
+    (void)k;
+  }
 }
 
 int main(void) {
@@ -122,6 +145,10 @@ int main(void) {
   SAME(OSC_GEN_ARITH(ppd, sum_reduce), shmem_long_sum_reduce);
   SAME(OSC_GEN_STD(u64, put), shmem_ulong_put);
   SAME(OSC_GEN_STD(p32, fcollect), shmem_int_fcollect);
+  SAME(OSC_GEN_SYNC(u64, test_any), shmem_ulong_test_any);
+  SAME(OSC_GEN_SYNC(psz, wait_until_all), shmem_ulong_wait_until_all);
+  SAME(OSC_GEN_SYNC1(p16, test), shmem_short_test);
+  SAME(OSC_GEN_SYNC1(u16, wait), shmem_ushort_wait);
   if (bad == 0) printf("generics ok\n");
   return bad != 0;
 }

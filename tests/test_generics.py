@@ -42,6 +42,8 @@ def test_generic_selection(tmp_path):
     " (float *)0, ps);",
     "shmem_fcollect(t, (void *)0, (const void *)0, 4);",
     "shmem_sum_reduce(t, (long *)0, (const int *)0, 4);",
+    "shmem_test_any((short *)0, 4, 0, SHMEM_CMP_EQ, (short)0);",
+    "shmem_wait_until((double *)0, SHMEM_CMP_EQ, 0.0);",
 ])
 def test_types_outside_the_tables_do_not_compile(tmp_path, stmt):
     src = tmp_path / "bad.c"

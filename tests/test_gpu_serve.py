@@ -187,3 +187,19 @@ def test_size_order_same_results_and_bad_order_rejected(sw):
     bad.order, bad.age_bytes_per_s = osc.ORDER_SIZE, -1.0
     with pytest.raises(osc.OscError):
         osc.serve(reqs[:2], bad)
+
+
+def test_request_needing_more_whole_pages_than_the_arena_is_rejected(sw):
+    """dest_bytes that is not a multiple of the page holds floor(dest_bytes
+    / page) whole pages; a request that fits in bytes but needs one page
+    more can never be placed: rejected, not left waiting."""
+    reqs = requests(sw, 9, 1, 1e3)
+    c = reqs[0].call
+    c.kind, c.team_start, c.team_npes, c.root = osc.CALL_FCOLLECT, 0, 16, 0
+    c.count = 1048448                      # dest 16 x 1048448 = 16 MiB - 2 KiB
+    cfg = sw.cfg(osc.SERVE_BATCHED)
+    cfg.dest_bytes = ARENA - 1000          # 255 whole 64 KiB pages
+    assert 16 * c.count <= cfg.dest_bytes
+    assert -(-16 * c.count // cfg.page_bytes) > cfg.dest_bytes // cfg.page_bytes
+    out, st = osc.serve(reqs, cfg)
+    assert st.rejected == 1 and out[0].status == -9
