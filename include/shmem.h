@@ -266,6 +266,99 @@ OSC_P2P_SYNC_TYPES(OSC_DECL_TEST_WAIT)
 OSC_P2P_DEPR_TYPES(OSC_DECL_TEST_WAIT)
 OSC_P2P_SYNC_TYPES(OSC_DECL_P2P_SETS)
 
+/* ---- atomic memory operations (reference include/shmem/api.h:2030-2990,
+ * type tables api_types.h:93-154) ------------------------------------------
+ * On the 4- or 8-byte symmetric word dest (source) of PE pe.  The fetching
+ * forms return the value found and wait for it; the others are ordered
+ * before every later operation of the library (shmem_quiet completes
+ * them).  The _nbi forms store the fetched value to *fetch (host memory or
+ * a symmetric address) before they return.  Errors are recorded
+ * (osc_last_error) and the fetching forms then return 0. */
+#define OSC_AMO_STD_TYPES(X)                                                   \
+  X(int, int)                                                                  \
+  X(long, long)                                                                \
+  X(long long, longlong)                                                       \
+  X(unsigned int, uint)                                                        \
+  X(unsigned long, ulong)                                                      \
+  X(unsigned long long, ulonglong)                                             \
+  X(int32_t, int32)                                                            \
+  X(int64_t, int64)                                                            \
+  X(uint32_t, uint32)                                                          \
+  X(uint64_t, uint64)                                                          \
+  X(size_t, size)                                                              \
+  X(ptrdiff_t, ptrdiff)
+#define OSC_AMO_EXT_TYPES(X) X(float, float) X(double, double) OSC_AMO_STD_TYPES(X)
+#define OSC_AMO_BITWISE_TYPES(X)                                               \
+  X(unsigned int, uint)                                                        \
+  X(unsigned long, ulong)                                                      \
+  X(unsigned long long, ulonglong)                                             \
+  X(int32_t, int32)                                                            \
+  X(int64_t, int64)                                                            \
+  X(uint32_t, uint32)                                                          \
+  X(uint64_t, uint64)
+#define OSC_DECL_AMO_EXT(_type, _typename)                                     \
+  _type shmem_##_typename##_atomic_fetch(const _type *source, int pe);         \
+  void shmem_##_typename##_atomic_set(_type *dest, _type value, int pe);       \
+  _type shmem_##_typename##_atomic_swap(_type *dest, _type value, int pe);     \
+  void shmem_##_typename##_atomic_fetch_nbi(_type *fetch, const _type *source, \
+                                            int pe);                           \
+  void shmem_##_typename##_atomic_swap_nbi(_type *fetch, _type *dest,          \
+                                           _type value, int pe);
+#define OSC_DECL_AMO_STD(_type, _typename)                                     \
+  _type shmem_##_typename##_atomic_compare_swap(_type *dest, _type cond,       \
+                                                _type value, int pe);          \
+  _type shmem_##_typename##_atomic_fetch_inc(_type *dest, int pe);             \
+  void shmem_##_typename##_atomic_inc(_type *dest, int pe);                    \
+  _type shmem_##_typename##_atomic_fetch_add(_type *dest, _type value,         \
+                                             int pe);                          \
+  void shmem_##_typename##_atomic_add(_type *dest, _type value, int pe);       \
+  void shmem_##_typename##_atomic_compare_swap_nbi(                            \
+      _type *fetch, _type *dest, _type cond, _type value, int pe);             \
+  void shmem_##_typename##_atomic_fetch_inc_nbi(_type *fetch, _type *dest,     \
+                                                int pe);                       \
+  void shmem_##_typename##_atomic_fetch_add_nbi(_type *fetch,                  \
+                                                const _type *dest,             \
+                                                _type value, int pe);
+#define OSC_DECL_AMO_BITWISE(_type, _typename)                                 \
+  _type shmem_##_typename##_atomic_fetch_and(_type *dest, _type value,         \
+                                             int pe);                          \
+  _type shmem_##_typename##_atomic_fetch_or(_type *dest, _type value, int pe); \
+  _type shmem_##_typename##_atomic_fetch_xor(_type *dest, _type value,         \
+                                             int pe);                          \
+  void shmem_##_typename##_atomic_and(_type *dest, _type value, int pe);       \
+  void shmem_##_typename##_atomic_or(_type *dest, _type value, int pe);        \
+  void shmem_##_typename##_atomic_xor(_type *dest, _type value, int pe);       \
+  void shmem_##_typename##_atomic_fetch_and_nbi(_type *fetch, _type *dest,     \
+                                                _type value, int pe);          \
+  void shmem_##_typename##_atomic_fetch_or_nbi(_type *fetch, _type *dest,      \
+                                               _type value, int pe);           \
+  void shmem_##_typename##_atomic_fetch_xor_nbi(_type *fetch, _type *dest,     \
+                                                _type value, int pe);          \
+  /* the reference also declares this name, with a fetch argument */          \
+  void shmem_##_typename##_atomic_xor_nbi(_type *fetch, _type *dest,           \
+                                          _type value, int pe);
+OSC_AMO_EXT_TYPES(OSC_DECL_AMO_EXT)
+OSC_AMO_STD_TYPES(OSC_DECL_AMO_STD)
+OSC_AMO_BITWISE_TYPES(OSC_DECL_AMO_BITWISE)
+/* deprecated names (OpenSHMEM 1.3): int, long, long long; float and double
+ * for fetch/set/swap */
+#define OSC_AMO_DEPR_INT_TYPES(X) X(int, int) X(long, long) X(long long, longlong)
+#define OSC_DECL_AMO_DEPR_EXT(_type, _typename)                                \
+  _type shmem_##_typename##_fetch(const _type *target, int pe);                \
+  void shmem_##_typename##_set(_type *target, _type value, int pe);            \
+  _type shmem_##_typename##_swap(_type *target, _type value, int pe);
+#define OSC_DECL_AMO_DEPR_INT(_type, _typename)                                \
+  _type shmem_##_typename##_cswap(_type *target, _type cond, _type value,      \
+                                  int pe);                                     \
+  _type shmem_##_typename##_finc(_type *target, int pe);                       \
+  void shmem_##_typename##_inc(_type *target, int pe);                         \
+  _type shmem_##_typename##_fadd(_type *target, _type value, int pe);          \
+  void shmem_##_typename##_add(_type *target, _type value, int pe);
+OSC_AMO_DEPR_INT_TYPES(OSC_DECL_AMO_DEPR_EXT)
+OSC_AMO_DEPR_INT_TYPES(OSC_DECL_AMO_DEPR_INT)
+OSC_DECL_AMO_DEPR_EXT(float, float)
+OSC_DECL_AMO_DEPR_EXT(double, double)
+
 /* ---- type tables -------------------------------------------------------
  * The sets of <T> each collective is defined for.  The lists are those of
  * the OpenSHMEM 1.5 specification (Tables 5 and 10), as in the reference's
@@ -649,6 +742,63 @@ void shmem_sync_all(void);
                                      cmp_values)                               \
   OSC_GEN_SYNC(ivars, wait_until_some_vector)(ivars, nelems, indices, status,  \
                                               cmp, cmp_values)
+/* atomics: extended, standard and bitwise AMO tables */
+#define OSC_GEN_AMO_STD_INTS(_op)                                              \
+  int: shmem_int_##_op, long: shmem_long_##_op,                                \
+  long long: shmem_longlong_##_op, unsigned int: shmem_uint_##_op,             \
+  unsigned long: shmem_ulong_##_op, unsigned long long: shmem_ulonglong_##_op
+#define OSC_GEN_AMO_EXT(_p, _op)                                               \
+  _Generic(*(_p), OSC_GEN_AMO_STD_INTS(_op), float: shmem_float_##_op,         \
+           double: shmem_double_##_op)
+#define OSC_GEN_AMO_STD(_p, _op) _Generic(*(_p), OSC_GEN_AMO_STD_INTS(_op))
+#define OSC_GEN_AMO_BITWISE(_p, _op)                                           \
+  _Generic(*(_p), unsigned int: shmem_uint_##_op,                              \
+           unsigned long: shmem_ulong_##_op,                                   \
+           unsigned long long: shmem_ulonglong_##_op, int: shmem_int32_##_op,  \
+           long: shmem_int64_##_op)
+#define shmem_atomic_fetch(source, pe)                                         \
+  OSC_GEN_AMO_EXT(source, atomic_fetch)(source, pe)
+#define shmem_atomic_set(dest, value, pe)                                      \
+  OSC_GEN_AMO_EXT(dest, atomic_set)(dest, value, pe)
+#define shmem_atomic_swap(dest, value, pe)                                     \
+  OSC_GEN_AMO_EXT(dest, atomic_swap)(dest, value, pe)
+#define shmem_atomic_fetch_nbi(fetch, source, pe)                              \
+  OSC_GEN_AMO_EXT(source, atomic_fetch_nbi)(fetch, source, pe)
+#define shmem_atomic_swap_nbi(fetch, dest, value, pe)                          \
+  OSC_GEN_AMO_EXT(dest, atomic_swap_nbi)(fetch, dest, value, pe)
+#define shmem_atomic_compare_swap(dest, cond, value, pe)                       \
+  OSC_GEN_AMO_STD(dest, atomic_compare_swap)(dest, cond, value, pe)
+#define shmem_atomic_fetch_inc(dest, pe)                                       \
+  OSC_GEN_AMO_STD(dest, atomic_fetch_inc)(dest, pe)
+#define shmem_atomic_inc(dest, pe) OSC_GEN_AMO_STD(dest, atomic_inc)(dest, pe)
+#define shmem_atomic_fetch_add(dest, value, pe)                                \
+  OSC_GEN_AMO_STD(dest, atomic_fetch_add)(dest, value, pe)
+#define shmem_atomic_add(dest, value, pe)                                      \
+  OSC_GEN_AMO_STD(dest, atomic_add)(dest, value, pe)
+#define shmem_atomic_compare_swap_nbi(fetch, dest, cond, value, pe)            \
+  OSC_GEN_AMO_STD(dest, atomic_compare_swap_nbi)(fetch, dest, cond, value, pe)
+#define shmem_atomic_fetch_inc_nbi(fetch, dest, pe)                            \
+  OSC_GEN_AMO_STD(dest, atomic_fetch_inc_nbi)(fetch, dest, pe)
+#define shmem_atomic_fetch_add_nbi(fetch, dest, value, pe)                     \
+  OSC_GEN_AMO_STD(dest, atomic_fetch_add_nbi)(fetch, dest, value, pe)
+#define shmem_atomic_fetch_and(dest, value, pe)                                \
+  OSC_GEN_AMO_BITWISE(dest, atomic_fetch_and)(dest, value, pe)
+#define shmem_atomic_fetch_or(dest, value, pe)                                 \
+  OSC_GEN_AMO_BITWISE(dest, atomic_fetch_or)(dest, value, pe)
+#define shmem_atomic_fetch_xor(dest, value, pe)                                \
+  OSC_GEN_AMO_BITWISE(dest, atomic_fetch_xor)(dest, value, pe)
+#define shmem_atomic_and(dest, value, pe)                                      \
+  OSC_GEN_AMO_BITWISE(dest, atomic_and)(dest, value, pe)
+#define shmem_atomic_or(dest, value, pe)                                       \
+  OSC_GEN_AMO_BITWISE(dest, atomic_or)(dest, value, pe)
+#define shmem_atomic_xor(dest, value, pe)                                      \
+  OSC_GEN_AMO_BITWISE(dest, atomic_xor)(dest, value, pe)
+#define shmem_atomic_fetch_and_nbi(fetch, dest, value, pe)                     \
+  OSC_GEN_AMO_BITWISE(dest, atomic_fetch_and_nbi)(fetch, dest, value, pe)
+#define shmem_atomic_fetch_or_nbi(fetch, dest, value, pe)                      \
+  OSC_GEN_AMO_BITWISE(dest, atomic_fetch_or_nbi)(fetch, dest, value, pe)
+#define shmem_atomic_fetch_xor_nbi(fetch, dest, value, pe)                     \
+  OSC_GEN_AMO_BITWISE(dest, atomic_fetch_xor_nbi)(fetch, dest, value, pe)
 #endif /* C11 */
 
 #ifdef __cplusplus

@@ -113,13 +113,16 @@ int osc_init(int npes, size_t heap_bytes_per_pe, int device) {
     return osc_hip_error(rc, "hipStreamCreate");
   }
   rc = oscd_wait_init();
+  if (rc == OSC_OK) rc = oscd_amo_init();
   if (rc != OSC_OK) {
+    oscd_wait_fini();
     oscd_stream_destroy(osc_g.stream);
     oscd_free(base);
     return osc_hip_error(rc, "completion slots");
   }
   rc = osc_heap_init(heap_bytes_per_pe);
   if (rc != OSC_OK) {
+    oscd_amo_fini();
     oscd_wait_fini();
     oscd_stream_destroy(osc_g.stream);
     oscd_free(base);
@@ -148,6 +151,7 @@ int osc_finalize(void) {
   oscd_stream_sync(osc_g.stream);
   oscd_stream_destroy(osc_g.stream);
   oscd_wait_fini();
+  oscd_amo_fini();
   if (osc_g.aux_dev) oscd_free(osc_g.aux_dev);
   if (osc_g.tab_event) oscd_event_destroy(osc_g.tab_event);
   if (osc_g.tab_dev) oscd_free(osc_g.tab_dev);

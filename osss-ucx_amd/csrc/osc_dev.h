@@ -109,6 +109,24 @@ int oscd_copy_strided(void *dst, const void *src, uint64_t es,
 /* signal update after the preceding work on `stream`: *sig = value
  * (op 0, set) or *sig += value (op 1, add), as one 64-bit atomic. */
 int oscd_signal(uint64_t *sig, uint64_t value, int op, void *stream);
+/* Atomic memory operation on a 4- or 8-byte device word (dev_amo.hip).
+ * With old != NULL the call waits (bounded spin on a pinned-memory slot,
+ * then hipStreamSynchronize) and returns the value found; with old == NULL
+ * it is only enqueued.  CSWAP writes value where the word equals cond. */
+enum {
+  OSCD_AMO_FETCH = 0,
+  OSCD_AMO_SET,
+  OSCD_AMO_SWAP,
+  OSCD_AMO_CSWAP,
+  OSCD_AMO_ADD,
+  OSCD_AMO_AND,
+  OSCD_AMO_OR,
+  OSCD_AMO_XOR
+};
+int oscd_amo_init(void);
+void oscd_amo_fini(void);
+int oscd_amo(void *addr, int width, int op, uint64_t value, uint64_t cond,
+             uint64_t *old, void *stream, unsigned spin_us);
 
 /* ---- batched execution ------------------------------------------------- */
 /* One collective call of a batch as the device sees it (64 bytes). */

@@ -91,6 +91,36 @@ void uses(shmem_team_t t, int pe) {
     k += shmem_wait_until_some_vector(ld, 4, idx, st, SHMEM_CMP_GE, cv);
     (void)k;
   }
+  {
+    long lf = 0, *lp = 0;
+    float ff = 0, *fp = 0;
+    unsigned *up = 0;
+    long r = 0;
+    r += shmem_atomic_fetch(ls, pe);
+    shmem_atomic_set(fp, 1.0f, pe);
+    ff = shmem_atomic_swap(fp, ff, pe);
+    shmem_atomic_fetch_nbi(&ff, fp, pe);
+    shmem_atomic_swap_nbi(&lf, lp, 2L, pe);
+    r += shmem_atomic_compare_swap(lp, 1L, 2L, pe);
+    r += shmem_atomic_fetch_inc(lp, pe);
+    shmem_atomic_inc(lp, pe);
+    r += shmem_atomic_fetch_add(lp, 3L, pe);
+    shmem_atomic_add(lp, 3L, pe);
+    shmem_atomic_compare_swap_nbi(&lf, lp, 1L, 2L, pe);
+    shmem_atomic_fetch_inc_nbi(&lf, lp, pe);
+    shmem_atomic_fetch_add_nbi(&lf, lp, 4L, pe);
+    r += (long)shmem_atomic_fetch_and(up, 1u, pe);
+    r += (long)shmem_atomic_fetch_or(up, 1u, pe);
+    r += (long)shmem_atomic_fetch_xor(up, 1u, pe);
+    shmem_atomic_and(lp, 1L, pe);
+    shmem_atomic_or(up, 1u, pe);
+    shmem_atomic_xor(up, 1u, pe);
+    shmem_atomic_fetch_and_nbi((unsigned *)&lf, up, 1u, pe);
+    shmem_atomic_fetch_or_nbi(&lf, lp, 1L, pe);
+    shmem_atomic_fetch_xor_nbi(&lf, lp, 1L, pe);
+    (void)r;
+    (void)ff;
+  }
 }
 
 int main(void) {
@@ -149,6 +179,11 @@ int main(void) {
   SAME(OSC_GEN_SYNC(psz, wait_until_all), shmem_ulong_wait_until_all);
   SAME(OSC_GEN_SYNC1(p16, test), shmem_short_test);
   SAME(OSC_GEN_SYNC1(u16, wait), shmem_ushort_wait);
+  SAME(OSC_GEN_AMO_EXT(cf, atomic_fetch), shmem_float_atomic_fetch);
+  SAME(OSC_GEN_AMO_STD(p64, atomic_fetch_inc), shmem_long_atomic_fetch_inc);
+  SAME(OSC_GEN_AMO_STD(psz, atomic_add), shmem_ulong_atomic_add);
+  SAME(OSC_GEN_AMO_BITWISE(p32, atomic_fetch_xor), shmem_int32_atomic_fetch_xor);
+  SAME(OSC_GEN_AMO_BITWISE(u64, atomic_or), shmem_ulong_atomic_or);
   if (bad == 0) printf("generics ok\n");
   return bad != 0;
 }

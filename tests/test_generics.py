@@ -44,6 +44,9 @@ def test_generic_selection(tmp_path):
     "shmem_sum_reduce(t, (long *)0, (const int *)0, 4);",
     "shmem_test_any((short *)0, 4, 0, SHMEM_CMP_EQ, (short)0);",
     "shmem_wait_until((double *)0, SHMEM_CMP_EQ, 0.0);",
+    "shmem_atomic_fetch_add((float *)0, 1.0f, 0);",
+    "shmem_atomic_fetch_and((short *)0, (short)1, 0);",
+    "shmem_atomic_inc((double *)0, 0);",
 ])
 def test_types_outside_the_tables_do_not_compile(tmp_path, stmt):
     src = tmp_path / "bad.c"
