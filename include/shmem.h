@@ -359,6 +359,24 @@ OSC_AMO_DEPR_INT_TYPES(OSC_DECL_AMO_DEPR_INT)
 OSC_DECL_AMO_DEPR_EXT(float, float)
 OSC_DECL_AMO_DEPR_EXT(double, double)
 
+/* ---- distributed locks (reference include/shmem/api.h:3007-3069) --------
+ * lock is a symmetric long, zero when free; the lock word is PE 0's copy.
+ * shmem_test_lock returns 0 when it took the lock and 1 when another PE
+ * holds it; shmem_clear_lock completes the critical section's stores
+ * first. */
+void shmem_set_lock(long *lock);
+void shmem_clear_lock(long *lock);
+int shmem_test_lock(long *lock);
+
+/* ---- deprecated cache routines: no effect (reference
+ * include/shmem/cache.h) ---------------------------------------------------- */
+void shmem_clear_cache_inv(void);
+void shmem_set_cache_inv(void);
+void shmem_clear_cache_line_inv(void *dest);
+void shmem_set_cache_line_inv(void *dest);
+void shmem_udcflush(void);
+void shmem_udcflush_line(void *dest);
+
 /* ---- type tables -------------------------------------------------------
  * The sets of <T> each collective is defined for.  The lists are those of
  * the OpenSHMEM 1.5 specification (Tables 5 and 10), as in the reference's

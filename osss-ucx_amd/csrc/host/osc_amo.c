@@ -11,8 +11,10 @@
  * travel as bit patterns of their width (4 or 8 bytes): float and double
  * take part only in fetch, set and swap, where bits are all that matter.
  */
+#include <sched.h>
 #include <stdint.h>
 #include <string.h>
+#include <time.h>
 
 #include "../../../include/shmem.h"
 #include "osc_internal.h"
@@ -214,3 +216,45 @@ OSC_AMO_DEPR_INT_TYPES(OSC_DEF_AMO_DEPR_EXT)
 OSC_AMO_DEPR_INT_TYPES(OSC_DEF_AMO_DEPR_INT)
 OSC_DEF_AMO_DEPR_EXT(float, float)
 OSC_DEF_AMO_DEPR_EXT(double, double)
+
+/* ---- distributed locks ----------------------------------------------------
+ * The lock word is PE 0's copy of the symmetric long: 0 when free, else
+ * the holder's PE + 1 (compare_swap takes it, swap releases it). */
+int shmem_test_lock(long *lock) {
+  uint64_t old = 1;
+  const long me = (long)osc_my_pe_get() + 1;
+  if (amo(lock, sizeof(long), 0, OSCD_AMO_CSWAP, (uint64_t)me, 0, &old) != OSC_OK)
+    return 1;
+  return old != 0;
+}
+
+void shmem_set_lock(long *lock) {
+  const long me = (long)osc_my_pe_get() + 1;
+  for (;;) {
+    uint64_t old = 1;
+    if (amo(lock, sizeof(long), 0, OSCD_AMO_CSWAP, (uint64_t)me, 0, &old) != OSC_OK)
+      return; /* error recorded: a bad lock address never succeeds */
+    if (old == 0) return;
+    sched_yield();
+  }
+}
+
+void shmem_clear_lock(long *lock) {
+  /* stream order puts the release after every store of the critical
+   * section, blocking or not */
+  amo(lock, sizeof(long), 0, OSCD_AMO_SET, 0, 0, NULL);
+}
+
+/* ---- deprecated cache routines: no effect ------------------------------- */
+void shmem_clear_cache_inv(void) {}
+void shmem_set_cache_inv(void) {}
+void shmem_clear_cache_line_inv(void *dest) { (void)dest; }
+void shmem_set_cache_line_inv(void *dest) { (void)dest; }
+void shmem_udcflush(void) {}
+void shmem_udcflush_line(void *dest) { (void)dest; }
+
+double shmemx_wtime(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}

@@ -18,7 +18,7 @@ INCLUDE = os.path.join(ROOT, "include")
 def declared_functions():
     """Function names declared by include/*.h after macro expansion."""
     names = set()
-    for h in ("shmem.h", "osc.h"):
+    for h in sorted(f for f in os.listdir(INCLUDE) if f.endswith(".h")):
         out = subprocess.run(["gcc", "-E", "-P", "-I", INCLUDE,
                               os.path.join(INCLUDE, h)],
                              check=True, capture_output=True, text=True).stdout
@@ -141,3 +141,11 @@ def test_strerror_names_every_code():
     lib = osc.lib()
     for code in range(0, -11, -1):
         assert lib.osc_strerror(code) != b"unknown error"
+
+
+def test_wtime_is_monotonic_seconds():
+    lib = osc.lib()
+    lib.shmemx_wtime.restype = C.c_double
+    a = lib.shmemx_wtime()
+    b = lib.shmemx_wtime()
+    assert 0 < a <= b < a + 1.0

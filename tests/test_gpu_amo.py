@@ -294,3 +294,44 @@ def test_spmd_compare_swap_lock_guards_a_plain_update():
         return 0
     assert _spmd(npes, body) == 0
     assert all(v == npes * per for v in out.values()), out
+
+
+def test_spmd_locks_guard_a_plain_update_and_test_lock_reports_holders():
+    npes, per = 5, 15
+    L = osc.lib()
+    L.shmem_set_lock.restype, L.shmem_set_lock.argtypes = None, [VP]
+    L.shmem_clear_lock.restype, L.shmem_clear_lock.argtypes = None, [VP]
+    L.shmem_test_lock.restype, L.shmem_test_lock.argtypes = I, [VP]
+    L.shmem_long_g.restype, L.shmem_long_g.argtypes = C.c_long, [VP, I]
+    L.shmem_long_p.restype, L.shmem_long_p.argtypes = None, [VP, C.c_long, I]
+    out = {}
+
+    def body(pe):
+        lock = L.shmem_calloc(1, 8)
+        ctr = L.shmem_calloc(1, 8)
+        L.shmem_barrier_all()
+        for _ in range(per):
+            L.shmem_set_lock(lock)
+            v = L.shmem_long_g(ctr, 0)
+            L.shmem_long_p(ctr, v + 1, 0)
+            L.shmem_clear_lock(lock)
+        L.shmem_barrier_all()
+        held = None
+        if pe == 0:                        # PE 0 takes it; the others try
+            assert L.shmem_test_lock(lock) == 0
+        L.shmem_barrier_all()
+        if pe != 0:
+            held = L.shmem_test_lock(lock)
+        L.shmem_barrier_all()
+        if pe == 0:
+            L.shmem_clear_lock(lock)
+        L.shmem_barrier_all()
+        out[pe] = (L.shmem_long_g(ctr, 0), held)
+        L.shmem_barrier_all()
+        L.shmem_free(ctr)
+        L.shmem_free(lock)
+        return 0
+    assert _spmd(npes, body) == 0
+    for pe in range(npes):
+        assert out[pe][0] == npes * per
+        assert out[pe][1] == (None if pe == 0 else 1)
