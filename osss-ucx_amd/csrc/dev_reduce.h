@@ -582,12 +582,6 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
     }
   }
 
-  if constexpr (Pol::SPECIAL) {
-    if (__syncthreads_or(spec != 0))
-      special_block<T, Pol::OPC, VEC>(g, src_off, dest_off, ubase, spec, lg,
-                                      lds);
-  }
-
   if (any) {
     for (int j = 0; j < P; ++j) {
       char *dp = d0 + j * step;
@@ -599,6 +593,17 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
                                                 ((1u << VEC) - 1u)
                                           : 0u);
     }
+  }
+
+  // Special elements last: the stores above skip them, so their inputs
+  // are still intact (also when dest == source), and the block-wide vote
+  // sits after every wave's stores instead of between the read and the
+  // write phase, where it held each wave's stores back until the slowest
+  // wave of the workgroup had loaded all P slices.
+  if constexpr (Pol::SPECIAL) {
+    if (__syncthreads_or(spec != 0))
+      special_block<T, Pol::OPC, VEC>(g, src_off, dest_off, ubase, spec, lg,
+                                      lds);
   }
 }
 
