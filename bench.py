@@ -106,8 +106,8 @@ def accuracy(L, h, src, dest, windows=4, width=256, seed=9):
 def kernel_key(op, dtype, mode):
     """The kernel instance the library launches for one cell at the default
     shape (8 units x 2 slices in flight; FP32 accumulated in FP64 and the
-    FP64 double-double sum: 8 x 1; reference-order tree: 2 x 4), in the
-    short form the profiles/ summaries use."""
+    FP64 double-double sum: 8 x 1; reference-order tree: 2 x 4 for float,
+    1 x 8 for double), in the short form the profiles/ summaries use."""
     o = osc.OPS[op]
     if dtype in ("u64", "i64"):
         t = "long" if (dtype == "i64" and op in ("min", "max")) else "u64"
@@ -122,7 +122,9 @@ def kernel_key(op, dtype, mode):
             return "k_reduce<float, PolF32Acc<%d>, 4, 8, 1, 1>" % o
         return "k_reduce<double, PolF64Acc<%d>, 2, 8, %d, 1>" % (
             o, 1 if op == "sum" else 2)
-    return "k_reduce<%s, PolTree<%s, %d>, %d, 2, 4, 8>" % (t, t, o, vec)
+    if dtype == "f32":
+        return "k_reduce<float, PolTree<float, %d>, 4, 2, 4, 8>" % o
+    return "k_reduce<double, PolTree<double, %d>, 2, 1, 8, 8>" % o
 
 
 def measured_traffic(key):
