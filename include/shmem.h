@@ -42,6 +42,30 @@ typedef struct shmem_team_config {
 
 #define SHMEM_TEAM_NUM_CONTEXTS (1 << 0)
 
+/* ---- communication contexts (reference include/shmem/defs.h:112-126,
+ * api.h:3615-3700) ----------------------------------------------------------
+ * A context is an independent ordering of RMA and atomic operations: here
+ * its own device stream, so work on different contexts can overlap, and
+ * shmem_ctx_quiet completes only that context's operations.
+ * SHMEM_CTX_DEFAULT is the library stream that the routines without a
+ * context use.  On a context made by shmem_team_create_ctx, PE numbers
+ * are indices into that team.  The options are accepted and have no
+ * further effect. */
+typedef void *shmem_ctx_t;
+extern shmem_ctx_t SHMEM_CTX_DEFAULT;
+#define SHMEM_CTX_INVALID NULL
+enum {
+  SHMEM_CTX_SERIALIZED = 1 << 0,
+  SHMEM_CTX_PRIVATE = 1 << 1,
+  SHMEM_CTX_NOSTORE = 1 << 2
+};
+int shmem_ctx_create(long options, shmem_ctx_t *ctx);
+int shmem_team_create_ctx(shmem_team_t team, long options, shmem_ctx_t *ctx);
+void shmem_ctx_destroy(shmem_ctx_t ctx);
+int shmem_ctx_get_team(shmem_ctx_t ctx, shmem_team_t *team);
+void shmem_ctx_fence(shmem_ctx_t ctx);
+void shmem_ctx_quiet(shmem_ctx_t ctx);
+
 /* ---- library setup (reference src/api/init.c:100-211) -------------------
  * Thread levels, malloc hints and the name length take the reference's
  * values (include/shmem/defs.h:93-98, :136-139; configure.ac:18). */
@@ -184,6 +208,65 @@ void shmem_getmem_nbi(void *dest, const void *source, size_t nbytes, int pe);
                                      size_t nelems, uint64_t *sig_addr,        \
                                      uint64_t signal, int sig_op, int pe);
 #define OSC_SIZED_RMA(X) X(8) X(16) X(32) X(64) X(128)
+#define OSC_DECL_CTX_RMA(_type, _typename)                                     \
+  void shmem_ctx_##_typename##_put(shmem_ctx_t ctx, _type *dest,               \
+      const _type *src, size_t nelems, int pe);                                \
+  void shmem_ctx_##_typename##_put_nbi(shmem_ctx_t ctx, _type *dest,           \
+      const _type *src, size_t nelems, int pe);                                \
+  void shmem_ctx_##_typename##_get(shmem_ctx_t ctx, _type *dest,               \
+      const _type *src, size_t nelems, int pe);                                \
+  void shmem_ctx_##_typename##_get_nbi(shmem_ctx_t ctx, _type *dest,           \
+      const _type *src, size_t nelems, int pe);                                \
+  void shmem_ctx_##_typename##_p(shmem_ctx_t ctx, _type *dest, _type value,    \
+      int pe);                                                                 \
+  _type shmem_ctx_##_typename##_g(shmem_ctx_t ctx, const _type *src,           \
+      int pe);                                                                 \
+  void shmem_ctx_##_typename##_iput(shmem_ctx_t ctx, _type *dest,              \
+      const _type *src, ptrdiff_t tst, ptrdiff_t sst, size_t nelems,           \
+      int pe);                                                                 \
+  void shmem_ctx_##_typename##_iget(shmem_ctx_t ctx, _type *dest,              \
+      const _type *src, ptrdiff_t tst, ptrdiff_t sst, size_t nelems,           \
+      int pe);                                                                 \
+  void shmem_ctx_##_typename##_put_signal(shmem_ctx_t ctx, _type *dest,        \
+      const _type *src, size_t nelems, uint64_t *sig_addr, uint64_t signal,    \
+      int sig_op, int pe);                                                     \
+  void shmem_ctx_##_typename##_put_signal_nbi(shmem_ctx_t ctx, _type *dest,    \
+      const _type *src, size_t nelems, uint64_t *sig_addr, uint64_t signal,    \
+      int sig_op, int pe);
+#define OSC_DECL_CTX_SIZED_RMA(_size)                                          \
+  void shmem_ctx_put##_size(shmem_ctx_t ctx, void *dest, const void *src,      \
+      size_t nelems, int pe);                                                  \
+  void shmem_ctx_put##_size##_nbi(shmem_ctx_t ctx, void *dest,                 \
+      const void *src, size_t nelems, int pe);                                 \
+  void shmem_ctx_get##_size(shmem_ctx_t ctx, void *dest, const void *src,      \
+      size_t nelems, int pe);                                                  \
+  void shmem_ctx_get##_size##_nbi(shmem_ctx_t ctx, void *dest,                 \
+      const void *src, size_t nelems, int pe);                                 \
+  void shmem_ctx_iput##_size(shmem_ctx_t ctx, void *dest, const void *src,     \
+      ptrdiff_t tst, ptrdiff_t sst, size_t nelems, int pe);                    \
+  void shmem_ctx_iget##_size(shmem_ctx_t ctx, void *dest, const void *src,     \
+      ptrdiff_t tst, ptrdiff_t sst, size_t nelems, int pe);                    \
+  void shmem_ctx_put##_size##_signal(shmem_ctx_t ctx, void *dest,              \
+      const void *src, size_t nelems, uint64_t *sig_addr, uint64_t signal,     \
+      int sig_op, int pe);                                                     \
+  void shmem_ctx_put##_size##_signal_nbi(shmem_ctx_t ctx, void *dest,          \
+      const void *src, size_t nelems, uint64_t *sig_addr, uint64_t signal,     \
+      int sig_op, int pe);
+void shmem_ctx_putmem(shmem_ctx_t ctx, void *dest, const void *source,
+                      size_t nbytes, int pe);
+void shmem_ctx_getmem(shmem_ctx_t ctx, void *dest, const void *source,
+                      size_t nbytes, int pe);
+void shmem_ctx_putmem_nbi(shmem_ctx_t ctx, void *dest, const void *source,
+                          size_t nbytes, int pe);
+void shmem_ctx_getmem_nbi(shmem_ctx_t ctx, void *dest, const void *source,
+                          size_t nbytes, int pe);
+void shmem_ctx_putmem_signal(shmem_ctx_t ctx, void *dest, const void *source,
+                             size_t nbytes, uint64_t *sig_addr,
+                             uint64_t signal, int sig_op, int pe);
+void shmem_ctx_putmem_signal_nbi(shmem_ctx_t ctx, void *dest,
+                                 const void *source, size_t nbytes,
+                                 uint64_t *sig_addr, uint64_t signal,
+                                 int sig_op, int pe);
 
 void shmem_putmem_signal(void *dest, const void *source, size_t nbytes,
                          uint64_t *sig_addr, uint64_t signal, int sig_op,
@@ -287,7 +370,8 @@ OSC_P2P_SYNC_TYPES(OSC_DECL_P2P_SETS)
   X(uint64_t, uint64)                                                          \
   X(size_t, size)                                                              \
   X(ptrdiff_t, ptrdiff)
-#define OSC_AMO_EXT_TYPES(X) X(float, float) X(double, double) OSC_AMO_STD_TYPES(X)
+#define OSC_AMO_EXT_TYPES(X) X(float, float) X(double,                         \
+      double) OSC_AMO_STD_TYPES(X)
 #define OSC_AMO_BITWISE_TYPES(X)                                               \
   X(unsigned int, uint)                                                        \
   X(unsigned long, ulong)                                                      \
@@ -340,9 +424,62 @@ OSC_P2P_SYNC_TYPES(OSC_DECL_P2P_SETS)
 OSC_AMO_EXT_TYPES(OSC_DECL_AMO_EXT)
 OSC_AMO_STD_TYPES(OSC_DECL_AMO_STD)
 OSC_AMO_BITWISE_TYPES(OSC_DECL_AMO_BITWISE)
+#define OSC_DECL_CTX_AMO_EXT(_type, _typename)                                 \
+  _type shmem_ctx_##_typename##_atomic_fetch(shmem_ctx_t ctx,                  \
+      const _type *source, int pe);                                            \
+  void shmem_ctx_##_typename##_atomic_set(shmem_ctx_t ctx, _type *dest,        \
+      _type value, int pe);                                                    \
+  _type shmem_ctx_##_typename##_atomic_swap(shmem_ctx_t ctx, _type *dest,      \
+      _type value, int pe);                                                    \
+  void shmem_ctx_##_typename##_atomic_fetch_nbi(shmem_ctx_t ctx,               \
+      _type *fetch, const _type *source, int pe);                              \
+  void shmem_ctx_##_typename##_atomic_swap_nbi(shmem_ctx_t ctx,                \
+      _type *fetch, _type *dest, _type value, int pe);
+#define OSC_DECL_CTX_AMO_STD(_type, _typename)                                 \
+  _type shmem_ctx_##_typename##_atomic_compare_swap(shmem_ctx_t ctx,           \
+      _type *dest, _type cond, _type value, int pe);                           \
+  _type shmem_ctx_##_typename##_atomic_fetch_inc(shmem_ctx_t ctx,              \
+      _type *dest, int pe);                                                    \
+  void shmem_ctx_##_typename##_atomic_inc(shmem_ctx_t ctx, _type *dest,        \
+      int pe);                                                                 \
+  _type shmem_ctx_##_typename##_atomic_fetch_add(shmem_ctx_t ctx,              \
+      _type *dest, _type value, int pe);                                       \
+  void shmem_ctx_##_typename##_atomic_add(shmem_ctx_t ctx, _type *dest,        \
+      _type value, int pe);                                                    \
+  void shmem_ctx_##_typename##_atomic_compare_swap_nbi(shmem_ctx_t ctx,        \
+       _type *fetch, _type *dest, _type cond, _type value, int pe);            \
+  void shmem_ctx_##_typename##_atomic_fetch_inc_nbi(shmem_ctx_t ctx,           \
+      _type *fetch, _type *dest, int pe);                                      \
+  void shmem_ctx_##_typename##_atomic_fetch_add_nbi(shmem_ctx_t ctx,           \
+      _type *fetch, const _type *dest, _type value, int pe);
+#define OSC_DECL_CTX_AMO_BITWISE(_type, _typename)                             \
+  _type shmem_ctx_##_typename##_atomic_fetch_and(shmem_ctx_t ctx,              \
+      _type *dest, _type value, int pe);                                       \
+  _type shmem_ctx_##_typename##_atomic_fetch_or(shmem_ctx_t ctx,               \
+      _type *dest, _type value, int pe);                                       \
+  _type shmem_ctx_##_typename##_atomic_fetch_xor(shmem_ctx_t ctx,              \
+      _type *dest, _type value, int pe);                                       \
+  void shmem_ctx_##_typename##_atomic_and(shmem_ctx_t ctx, _type *dest,        \
+      _type value, int pe);                                                    \
+  void shmem_ctx_##_typename##_atomic_or(shmem_ctx_t ctx, _type *dest,         \
+      _type value, int pe);                                                    \
+  void shmem_ctx_##_typename##_atomic_xor(shmem_ctx_t ctx, _type *dest,        \
+      _type value, int pe);                                                    \
+  void shmem_ctx_##_typename##_atomic_fetch_and_nbi(shmem_ctx_t ctx,           \
+      _type *fetch, _type *dest, _type value, int pe);                         \
+  void shmem_ctx_##_typename##_atomic_fetch_or_nbi(shmem_ctx_t ctx,            \
+      _type *fetch, _type *dest, _type value, int pe);                         \
+  void shmem_ctx_##_typename##_atomic_fetch_xor_nbi(shmem_ctx_t ctx,           \
+      _type *fetch, _type *dest, _type value, int pe);                         \
+  void shmem_ctx_##_typename##_atomic_xor_nbi(shmem_ctx_t ctx, _type *fetch,   \
+      _type *dest, _type value, int pe);
+OSC_AMO_EXT_TYPES(OSC_DECL_CTX_AMO_EXT)
+OSC_AMO_STD_TYPES(OSC_DECL_CTX_AMO_STD)
+OSC_AMO_BITWISE_TYPES(OSC_DECL_CTX_AMO_BITWISE)
 /* deprecated names (OpenSHMEM 1.3): int, long, long long; float and double
  * for fetch/set/swap */
-#define OSC_AMO_DEPR_INT_TYPES(X) X(int, int) X(long, long) X(long long, longlong)
+#define OSC_AMO_DEPR_INT_TYPES(X) X(int, int) X(long, long) X(long long,       \
+      longlong)
 #define OSC_DECL_AMO_DEPR_EXT(_type, _typename)                                \
   _type shmem_##_typename##_fetch(const _type *target, int pe);                \
   void shmem_##_typename##_set(_type *target, _type value, int pe);            \
@@ -463,6 +600,8 @@ void shmem_udcflush_line(void *dest);
 
 OSC_RMA_TYPES(OSC_DECL_RMA)
 OSC_SIZED_RMA(OSC_DECL_SIZED_RMA)
+OSC_RMA_TYPES(OSC_DECL_CTX_RMA)
+OSC_SIZED_RMA(OSC_DECL_CTX_SIZED_RMA)
 
 /* ---- team reductions (reference include/shmem/api.h:3079-3260) ---------
  * long double is declared for link compatibility; on gfx950 (no 80-bit
@@ -601,17 +740,21 @@ void shmem_sync_all(void);
  * outside a routine's table does not compile.  There are no contexts in
  * this library, so there are no shmem_ctx_t forms. */
 #if defined(__STDC_VERSION__) && __STDC_VERSION__ >= 201112L && !defined(__cplusplus)
-#define OSC_GEN_INTS(_op)                                                      \
-  char: shmem_char_##_op, signed char: shmem_schar_##_op,                      \
-  short: shmem_short_##_op, int: shmem_int_##_op, long: shmem_long_##_op,      \
-  long long: shmem_longlong_##_op, unsigned char: shmem_uchar_##_op,           \
-  unsigned short: shmem_ushort_##_op, unsigned int: shmem_uint_##_op,          \
-  unsigned long: shmem_ulong_##_op, unsigned long long: shmem_ulonglong_##_op
-#define OSC_GEN_REALS(_op)                                                     \
-  float: shmem_float_##_op, double: shmem_double_##_op,                        \
-  long double: shmem_longdouble_##_op
+#define OSC_GEN_INTS_P(_pre, _op)                                              \
+  char: _pre##char_##_op, signed char: _pre##schar_##_op,                      \
+  short: _pre##short_##_op, int: _pre##int_##_op, long: _pre##long_##_op,      \
+  long long: _pre##longlong_##_op, unsigned char: _pre##uchar_##_op,           \
+  unsigned short: _pre##ushort_##_op, unsigned int: _pre##uint_##_op,          \
+  unsigned long: _pre##ulong_##_op, unsigned long long: _pre##ulonglong_##_op
+#define OSC_GEN_REALS_P(_pre, _op)                                             \
+  float: _pre##float_##_op, double: _pre##double_##_op,                        \
+  long double: _pre##longdouble_##_op
+#define OSC_GEN_INTS(_op) OSC_GEN_INTS_P(shmem_, _op)
+#define OSC_GEN_REALS(_op) OSC_GEN_REALS_P(shmem_, _op)
 /* RMA and data-movement table; min/max table */
-#define OSC_GEN_STD(_p, _op) _Generic(*(_p), OSC_GEN_INTS(_op), OSC_GEN_REALS(_op))
+#define OSC_GEN_STD_P(_pre, _p, _op)                                           \
+  _Generic(*(_p), OSC_GEN_INTS_P(_pre, _op), OSC_GEN_REALS_P(_pre, _op))
+#define OSC_GEN_STD(_p, _op) OSC_GEN_STD_P(shmem_, _p, _op)
 /* and/or/xor table: unsigned types, int8..int64 */
 #define OSC_GEN_BITWISE(_p, _op)                                               \
   _Generic(*(_p), unsigned char: shmem_uchar_##_op,                            \
@@ -629,7 +772,8 @@ void shmem_sync_all(void);
 #define OSC_GEN_TO_ALL_INTS(_op)                                               \
   short: shmem_short_##_op, int: shmem_int_##_op, long: shmem_long_##_op,      \
   long long: shmem_longlong_##_op
-#define OSC_GEN_TO_ALL_BITWISE(_p, _op) _Generic(*(_p), OSC_GEN_TO_ALL_INTS(_op))
+#define OSC_GEN_TO_ALL_BITWISE(_p, _op) _Generic(*(_p),                        \
+      OSC_GEN_TO_ALL_INTS(_op))
 #define OSC_GEN_TO_ALL_MINMAX(_p, _op)                                         \
   _Generic(*(_p), OSC_GEN_TO_ALL_INTS(_op), OSC_GEN_REALS(_op))
 #define OSC_GEN_TO_ALL_ARITH(_p, _op)                                          \
@@ -637,27 +781,83 @@ void shmem_sync_all(void);
            float _Complex: shmem_complexf_##_op,                               \
            double _Complex: shmem_complexd_##_op)
 
-#define shmem_put(dest, source, nelems, pe)                                    \
-  OSC_GEN_STD(dest, put)(dest, source, nelems, pe)
-#define shmem_get(dest, source, nelems, pe)                                    \
-  OSC_GEN_STD(dest, get)(dest, source, nelems, pe)
-#define shmem_put_nbi(dest, source, nelems, pe)                                \
-  OSC_GEN_STD(dest, put_nbi)(dest, source, nelems, pe)
-#define shmem_get_nbi(dest, source, nelems, pe)                                \
-  OSC_GEN_STD(dest, get_nbi)(dest, source, nelems, pe)
-#define shmem_p(dest, value, pe) OSC_GEN_STD(dest, p)(dest, value, pe)
-#define shmem_g(source, pe) OSC_GEN_STD(source, g)(source, pe)
-#define shmem_iput(dest, source, dst, sst, nelems, pe)                         \
-  OSC_GEN_STD(dest, iput)(dest, source, dst, sst, nelems, pe)
-#define shmem_iget(dest, source, dst, sst, nelems, pe)                         \
-  OSC_GEN_STD(dest, iget)(dest, source, dst, sst, nelems, pe)
-#define shmem_put_signal(dest, source, nelems, sig_addr, signal, sig_op, pe)  \
-  OSC_GEN_STD(dest, put_signal)(dest, source, nelems, sig_addr, signal,        \
-                                sig_op, pe)
-#define shmem_put_signal_nbi(dest, source, nelems, sig_addr, signal, sig_op,  \
-                             pe)                                               \
-  OSC_GEN_STD(dest, put_signal_nbi)(dest, source, nelems, sig_addr, signal,    \
-                                    sig_op, pe)
+/* RMA and atomics take an optional leading context, told apart by the
+ * number of arguments: OSC_SEL<k+1> picks the context form (k + 1
+ * arguments) or the plain one (k arguments). */
+#define OSC_SEL3(_1, _2, _3, N, ...) N
+#define OSC_SEL4(_1, _2, _3, _4, N, ...) N
+#define OSC_SEL5(_1, _2, _3, _4, _5, N, ...) N
+#define OSC_SEL6(_1, _2, _3, _4, _5, _6, N, ...) N
+#define OSC_SEL7(_1, _2, _3, _4, _5, _6, _7, N, ...) N
+#define OSC_SEL8(_1, _2, _3, _4, _5, _6, _7, _8, N, ...) N
+#define OSC_GP_put(dest, source, nelems, pe) \
+  OSC_GEN_STD_P(shmem_, dest, put)(dest, source, nelems, pe)
+#define OSC_GC_put(ctx, dest, source, nelems, pe) \
+  OSC_GEN_STD_P(shmem_ctx_, dest, put)(ctx, dest, source, nelems, pe)
+#define shmem_put(...) \
+  OSC_SEL5(__VA_ARGS__, OSC_GC_put, OSC_GP_put, 0)(__VA_ARGS__)
+#define OSC_GP_get(dest, source, nelems, pe) \
+  OSC_GEN_STD_P(shmem_, dest, get)(dest, source, nelems, pe)
+#define OSC_GC_get(ctx, dest, source, nelems, pe) \
+  OSC_GEN_STD_P(shmem_ctx_, dest, get)(ctx, dest, source, nelems, pe)
+#define shmem_get(...) \
+  OSC_SEL5(__VA_ARGS__, OSC_GC_get, OSC_GP_get, 0)(__VA_ARGS__)
+#define OSC_GP_put_nbi(dest, source, nelems, pe) \
+  OSC_GEN_STD_P(shmem_, dest, put_nbi)(dest, source, nelems, pe)
+#define OSC_GC_put_nbi(ctx, dest, source, nelems, pe) \
+  OSC_GEN_STD_P(shmem_ctx_, dest, put_nbi)(ctx, dest, source, nelems, pe)
+#define shmem_put_nbi(...) \
+  OSC_SEL5(__VA_ARGS__, OSC_GC_put_nbi, OSC_GP_put_nbi, 0)(__VA_ARGS__)
+#define OSC_GP_get_nbi(dest, source, nelems, pe) \
+  OSC_GEN_STD_P(shmem_, dest, get_nbi)(dest, source, nelems, pe)
+#define OSC_GC_get_nbi(ctx, dest, source, nelems, pe) \
+  OSC_GEN_STD_P(shmem_ctx_, dest, get_nbi)(ctx, dest, source, nelems, pe)
+#define shmem_get_nbi(...) \
+  OSC_SEL5(__VA_ARGS__, OSC_GC_get_nbi, OSC_GP_get_nbi, 0)(__VA_ARGS__)
+#define OSC_GP_p(dest, value, pe) \
+  OSC_GEN_STD_P(shmem_, dest, p)(dest, value, pe)
+#define OSC_GC_p(ctx, dest, value, pe) \
+  OSC_GEN_STD_P(shmem_ctx_, dest, p)(ctx, dest, value, pe)
+#define shmem_p(...) \
+  OSC_SEL4(__VA_ARGS__, OSC_GC_p, OSC_GP_p, 0)(__VA_ARGS__)
+#define OSC_GP_g(source, pe) \
+  OSC_GEN_STD_P(shmem_, source, g)(source, pe)
+#define OSC_GC_g(ctx, source, pe) \
+  OSC_GEN_STD_P(shmem_ctx_, source, g)(ctx, source, pe)
+#define shmem_g(...) \
+  OSC_SEL3(__VA_ARGS__, OSC_GC_g, OSC_GP_g, 0)(__VA_ARGS__)
+#define OSC_GP_iput(dest, source, dst, sst, nelems, pe) \
+  OSC_GEN_STD_P(shmem_, dest, iput)(dest, source, dst, sst, nelems, pe)
+#define OSC_GC_iput(ctx, dest, source, dst, sst, nelems, pe) \
+  OSC_GEN_STD_P(shmem_ctx_, dest, iput)(ctx, dest, source, dst, sst, nelems, pe)
+#define shmem_iput(...) \
+  OSC_SEL7(__VA_ARGS__, OSC_GC_iput, OSC_GP_iput, 0)(__VA_ARGS__)
+#define OSC_GP_iget(dest, source, dst, sst, nelems, pe) \
+  OSC_GEN_STD_P(shmem_, dest, iget)(dest, source, dst, sst, nelems, pe)
+#define OSC_GC_iget(ctx, dest, source, dst, sst, nelems, pe) \
+  OSC_GEN_STD_P(shmem_ctx_, dest, iget)(ctx, dest, source, dst, sst, nelems, pe)
+#define shmem_iget(...) \
+  OSC_SEL7(__VA_ARGS__, OSC_GC_iget, OSC_GP_iget, 0)(__VA_ARGS__)
+#define OSC_GP_put_signal(dest, source, nelems, sig_addr, signal, sig_op, pe) \
+  OSC_GEN_STD_P(shmem_, dest, put_signal)(dest, source, nelems, sig_addr,      \
+      signal, sig_op, pe)
+#define OSC_GC_put_signal(ctx, dest, source, nelems, sig_addr, signal,         \
+      sig_op, pe)                                                              \
+  OSC_GEN_STD_P(shmem_ctx_, dest, put_signal)(ctx, dest, source, nelems,       \
+      sig_addr, signal, sig_op, pe)
+#define shmem_put_signal(...) \
+  OSC_SEL8(__VA_ARGS__, OSC_GC_put_signal, OSC_GP_put_signal, 0)(__VA_ARGS__)
+#define OSC_GP_put_signal_nbi(dest, source, nelems, sig_addr, signal, sig_op,  \
+      pe)                                                                      \
+  OSC_GEN_STD_P(shmem_, dest, put_signal_nbi)(dest, source, nelems, sig_addr,  \
+      signal, sig_op, pe)
+#define OSC_GC_put_signal_nbi(ctx, dest, source, nelems, sig_addr, signal,     \
+      sig_op, pe)                                                              \
+  OSC_GEN_STD_P(shmem_ctx_, dest, put_signal_nbi)(ctx, dest, source, nelems,   \
+      sig_addr, signal, sig_op, pe)
+#define shmem_put_signal_nbi(...) \
+  OSC_SEL8(__VA_ARGS__, OSC_GC_put_signal_nbi, OSC_GP_put_signal_nbi,          \
+      0)(__VA_ARGS__)
 
 #define shmem_and_reduce(team, dest, source, nreduce)                          \
   OSC_GEN_BITWISE(dest, and_reduce)(team, dest, source, nreduce)
@@ -687,23 +887,28 @@ void shmem_sync_all(void);
 
 #define shmem_and_to_all(target, source, nreduce, PE_start, logPE_stride,     \
                          PE_size, pWrk, pSync)                                 \
-  OSC_GEN_TO_ALL_BITWISE(target, and_to_all)(target, source, nreduce, PE_start,  \
+  OSC_GEN_TO_ALL_BITWISE(target, and_to_all)(target, source, nreduce,          \
+      PE_start,                                                                \
       logPE_stride, PE_size, pWrk, pSync)
 #define shmem_or_to_all(target, source, nreduce, PE_start, logPE_stride,      \
                         PE_size, pWrk, pSync)                                  \
-  OSC_GEN_TO_ALL_BITWISE(target, or_to_all)(target, source, nreduce, PE_start,  \
+  OSC_GEN_TO_ALL_BITWISE(target, or_to_all)(target, source, nreduce,           \
+      PE_start,                                                                \
       logPE_stride, PE_size, pWrk, pSync)
 #define shmem_xor_to_all(target, source, nreduce, PE_start, logPE_stride,     \
                          PE_size, pWrk, pSync)                                 \
-  OSC_GEN_TO_ALL_BITWISE(target, xor_to_all)(target, source, nreduce, PE_start,  \
+  OSC_GEN_TO_ALL_BITWISE(target, xor_to_all)(target, source, nreduce,          \
+      PE_start,                                                                \
       logPE_stride, PE_size, pWrk, pSync)
 #define shmem_min_to_all(target, source, nreduce, PE_start, logPE_stride,     \
                          PE_size, pWrk, pSync)                                 \
-  OSC_GEN_TO_ALL_MINMAX(target, min_to_all)(target, source, nreduce, PE_start,  \
+  OSC_GEN_TO_ALL_MINMAX(target, min_to_all)(target, source, nreduce,           \
+      PE_start,                                                                \
       logPE_stride, PE_size, pWrk, pSync)
 #define shmem_max_to_all(target, source, nreduce, PE_start, logPE_stride,     \
                          PE_size, pWrk, pSync)                                 \
-  OSC_GEN_TO_ALL_MINMAX(target, max_to_all)(target, source, nreduce, PE_start,  \
+  OSC_GEN_TO_ALL_MINMAX(target, max_to_all)(target, source, nreduce,           \
+      PE_start,                                                                \
       logPE_stride, PE_size, pWrk, pSync)
 #define shmem_sum_to_all(target, source, nreduce, PE_start, logPE_stride,     \
                          PE_size, pWrk, pSync)                                 \
@@ -711,7 +916,8 @@ void shmem_sync_all(void);
       logPE_stride, PE_size, pWrk, pSync)
 #define shmem_prod_to_all(target, source, nreduce, PE_start, logPE_stride,    \
                           PE_size, pWrk, pSync)                                \
-  OSC_GEN_TO_ALL_ARITH(target, prod_to_all)(target, source, nreduce, PE_start,  \
+  OSC_GEN_TO_ALL_ARITH(target, prod_to_all)(target, source, nreduce,           \
+      PE_start,                                                                \
       logPE_stride, PE_size, pWrk, pSync)
 /* point-to-point synchronization: the AMO table; short and unsigned short
  * also for the deprecated single-variable forms */
@@ -761,62 +967,186 @@ void shmem_sync_all(void);
   OSC_GEN_SYNC(ivars, wait_until_some_vector)(ivars, nelems, indices, status,  \
                                               cmp, cmp_values)
 /* atomics: extended, standard and bitwise AMO tables */
-#define OSC_GEN_AMO_STD_INTS(_op)                                              \
-  int: shmem_int_##_op, long: shmem_long_##_op,                                \
-  long long: shmem_longlong_##_op, unsigned int: shmem_uint_##_op,             \
-  unsigned long: shmem_ulong_##_op, unsigned long long: shmem_ulonglong_##_op
-#define OSC_GEN_AMO_EXT(_p, _op)                                               \
-  _Generic(*(_p), OSC_GEN_AMO_STD_INTS(_op), float: shmem_float_##_op,         \
-           double: shmem_double_##_op)
-#define OSC_GEN_AMO_STD(_p, _op) _Generic(*(_p), OSC_GEN_AMO_STD_INTS(_op))
-#define OSC_GEN_AMO_BITWISE(_p, _op)                                           \
-  _Generic(*(_p), unsigned int: shmem_uint_##_op,                              \
-           unsigned long: shmem_ulong_##_op,                                   \
-           unsigned long long: shmem_ulonglong_##_op, int: shmem_int32_##_op,  \
-           long: shmem_int64_##_op)
-#define shmem_atomic_fetch(source, pe)                                         \
-  OSC_GEN_AMO_EXT(source, atomic_fetch)(source, pe)
-#define shmem_atomic_set(dest, value, pe)                                      \
-  OSC_GEN_AMO_EXT(dest, atomic_set)(dest, value, pe)
-#define shmem_atomic_swap(dest, value, pe)                                     \
-  OSC_GEN_AMO_EXT(dest, atomic_swap)(dest, value, pe)
-#define shmem_atomic_fetch_nbi(fetch, source, pe)                              \
-  OSC_GEN_AMO_EXT(source, atomic_fetch_nbi)(fetch, source, pe)
-#define shmem_atomic_swap_nbi(fetch, dest, value, pe)                          \
-  OSC_GEN_AMO_EXT(dest, atomic_swap_nbi)(fetch, dest, value, pe)
-#define shmem_atomic_compare_swap(dest, cond, value, pe)                       \
-  OSC_GEN_AMO_STD(dest, atomic_compare_swap)(dest, cond, value, pe)
-#define shmem_atomic_fetch_inc(dest, pe)                                       \
-  OSC_GEN_AMO_STD(dest, atomic_fetch_inc)(dest, pe)
-#define shmem_atomic_inc(dest, pe) OSC_GEN_AMO_STD(dest, atomic_inc)(dest, pe)
-#define shmem_atomic_fetch_add(dest, value, pe)                                \
-  OSC_GEN_AMO_STD(dest, atomic_fetch_add)(dest, value, pe)
-#define shmem_atomic_add(dest, value, pe)                                      \
-  OSC_GEN_AMO_STD(dest, atomic_add)(dest, value, pe)
-#define shmem_atomic_compare_swap_nbi(fetch, dest, cond, value, pe)            \
-  OSC_GEN_AMO_STD(dest, atomic_compare_swap_nbi)(fetch, dest, cond, value, pe)
-#define shmem_atomic_fetch_inc_nbi(fetch, dest, pe)                            \
-  OSC_GEN_AMO_STD(dest, atomic_fetch_inc_nbi)(fetch, dest, pe)
-#define shmem_atomic_fetch_add_nbi(fetch, dest, value, pe)                     \
-  OSC_GEN_AMO_STD(dest, atomic_fetch_add_nbi)(fetch, dest, value, pe)
-#define shmem_atomic_fetch_and(dest, value, pe)                                \
-  OSC_GEN_AMO_BITWISE(dest, atomic_fetch_and)(dest, value, pe)
-#define shmem_atomic_fetch_or(dest, value, pe)                                 \
-  OSC_GEN_AMO_BITWISE(dest, atomic_fetch_or)(dest, value, pe)
-#define shmem_atomic_fetch_xor(dest, value, pe)                                \
-  OSC_GEN_AMO_BITWISE(dest, atomic_fetch_xor)(dest, value, pe)
-#define shmem_atomic_and(dest, value, pe)                                      \
-  OSC_GEN_AMO_BITWISE(dest, atomic_and)(dest, value, pe)
-#define shmem_atomic_or(dest, value, pe)                                       \
-  OSC_GEN_AMO_BITWISE(dest, atomic_or)(dest, value, pe)
-#define shmem_atomic_xor(dest, value, pe)                                      \
-  OSC_GEN_AMO_BITWISE(dest, atomic_xor)(dest, value, pe)
-#define shmem_atomic_fetch_and_nbi(fetch, dest, value, pe)                     \
-  OSC_GEN_AMO_BITWISE(dest, atomic_fetch_and_nbi)(fetch, dest, value, pe)
-#define shmem_atomic_fetch_or_nbi(fetch, dest, value, pe)                      \
-  OSC_GEN_AMO_BITWISE(dest, atomic_fetch_or_nbi)(fetch, dest, value, pe)
-#define shmem_atomic_fetch_xor_nbi(fetch, dest, value, pe)                     \
-  OSC_GEN_AMO_BITWISE(dest, atomic_fetch_xor_nbi)(fetch, dest, value, pe)
+#define OSC_GEN_AMO_STD_INTS_P(_pre, _op)                                      \
+  int: _pre##int_##_op, long: _pre##long_##_op,                                \
+  long long: _pre##longlong_##_op, unsigned int: _pre##uint_##_op,             \
+  unsigned long: _pre##ulong_##_op,                                            \
+  unsigned long long: _pre##ulonglong_##_op
+#define OSC_GEN_AMO_EXT_P(_pre, _p, _op)                                       \
+  _Generic(*(_p), OSC_GEN_AMO_STD_INTS_P(_pre, _op),                           \
+           float: _pre##float_##_op, double: _pre##double_##_op)
+#define OSC_GEN_AMO_STD_P(_pre, _p, _op)                                       \
+  _Generic(*(_p), OSC_GEN_AMO_STD_INTS_P(_pre, _op))
+#define OSC_GEN_AMO_BITWISE_P(_pre, _p, _op)                                   \
+  _Generic(*(_p), unsigned int: _pre##uint_##_op,                              \
+           unsigned long: _pre##ulong_##_op,                                   \
+           unsigned long long: _pre##ulonglong_##_op,                          \
+           int: _pre##int32_##_op, long: _pre##int64_##_op)
+#define OSC_GEN_AMO_EXT(_p, _op) OSC_GEN_AMO_EXT_P(shmem_, _p, _op)
+#define OSC_GEN_AMO_STD(_p, _op) OSC_GEN_AMO_STD_P(shmem_, _p, _op)
+#define OSC_GEN_AMO_BITWISE(_p, _op) OSC_GEN_AMO_BITWISE_P(shmem_, _p, _op)
+#define OSC_GP_atomic_fetch(source, pe) \
+  OSC_GEN_AMO_EXT_P(shmem_, source, atomic_fetch)(source, pe)
+#define OSC_GC_atomic_fetch(ctx, source, pe) \
+  OSC_GEN_AMO_EXT_P(shmem_ctx_, source, atomic_fetch)(ctx, source, pe)
+#define shmem_atomic_fetch(...) \
+  OSC_SEL3(__VA_ARGS__, OSC_GC_atomic_fetch, OSC_GP_atomic_fetch,              \
+      0)(__VA_ARGS__)
+#define OSC_GP_atomic_set(dest, value, pe) \
+  OSC_GEN_AMO_EXT_P(shmem_, dest, atomic_set)(dest, value, pe)
+#define OSC_GC_atomic_set(ctx, dest, value, pe) \
+  OSC_GEN_AMO_EXT_P(shmem_ctx_, dest, atomic_set)(ctx, dest, value, pe)
+#define shmem_atomic_set(...) \
+  OSC_SEL4(__VA_ARGS__, OSC_GC_atomic_set, OSC_GP_atomic_set, 0)(__VA_ARGS__)
+#define OSC_GP_atomic_swap(dest, value, pe) \
+  OSC_GEN_AMO_EXT_P(shmem_, dest, atomic_swap)(dest, value, pe)
+#define OSC_GC_atomic_swap(ctx, dest, value, pe) \
+  OSC_GEN_AMO_EXT_P(shmem_ctx_, dest, atomic_swap)(ctx, dest, value, pe)
+#define shmem_atomic_swap(...) \
+  OSC_SEL4(__VA_ARGS__, OSC_GC_atomic_swap, OSC_GP_atomic_swap, 0)(__VA_ARGS__)
+#define OSC_GP_atomic_fetch_nbi(fetch, source, pe) \
+  OSC_GEN_AMO_EXT_P(shmem_, source, atomic_fetch_nbi)(fetch, source, pe)
+#define OSC_GC_atomic_fetch_nbi(ctx, fetch, source, pe) \
+  OSC_GEN_AMO_EXT_P(shmem_ctx_, source, atomic_fetch_nbi)(ctx, fetch, source,  \
+      pe)
+#define shmem_atomic_fetch_nbi(...) \
+  OSC_SEL4(__VA_ARGS__, OSC_GC_atomic_fetch_nbi, OSC_GP_atomic_fetch_nbi,      \
+      0)(__VA_ARGS__)
+#define OSC_GP_atomic_swap_nbi(fetch, dest, value, pe) \
+  OSC_GEN_AMO_EXT_P(shmem_, dest, atomic_swap_nbi)(fetch, dest, value, pe)
+#define OSC_GC_atomic_swap_nbi(ctx, fetch, dest, value, pe) \
+  OSC_GEN_AMO_EXT_P(shmem_ctx_, dest, atomic_swap_nbi)(ctx, fetch, dest,       \
+      value, pe)
+#define shmem_atomic_swap_nbi(...) \
+  OSC_SEL5(__VA_ARGS__, OSC_GC_atomic_swap_nbi, OSC_GP_atomic_swap_nbi,        \
+      0)(__VA_ARGS__)
+#define OSC_GP_atomic_compare_swap(dest, cond, value, pe) \
+  OSC_GEN_AMO_STD_P(shmem_, dest, atomic_compare_swap)(dest, cond, value, pe)
+#define OSC_GC_atomic_compare_swap(ctx, dest, cond, value, pe) \
+  OSC_GEN_AMO_STD_P(shmem_ctx_, dest, atomic_compare_swap)(ctx, dest, cond,    \
+      value, pe)
+#define shmem_atomic_compare_swap(...) \
+  OSC_SEL5(__VA_ARGS__, OSC_GC_atomic_compare_swap,                            \
+      OSC_GP_atomic_compare_swap, 0)(__VA_ARGS__)
+#define OSC_GP_atomic_fetch_inc(dest, pe) \
+  OSC_GEN_AMO_STD_P(shmem_, dest, atomic_fetch_inc)(dest, pe)
+#define OSC_GC_atomic_fetch_inc(ctx, dest, pe) \
+  OSC_GEN_AMO_STD_P(shmem_ctx_, dest, atomic_fetch_inc)(ctx, dest, pe)
+#define shmem_atomic_fetch_inc(...) \
+  OSC_SEL3(__VA_ARGS__, OSC_GC_atomic_fetch_inc, OSC_GP_atomic_fetch_inc,      \
+      0)(__VA_ARGS__)
+#define OSC_GP_atomic_inc(dest, pe) \
+  OSC_GEN_AMO_STD_P(shmem_, dest, atomic_inc)(dest, pe)
+#define OSC_GC_atomic_inc(ctx, dest, pe) \
+  OSC_GEN_AMO_STD_P(shmem_ctx_, dest, atomic_inc)(ctx, dest, pe)
+#define shmem_atomic_inc(...) \
+  OSC_SEL3(__VA_ARGS__, OSC_GC_atomic_inc, OSC_GP_atomic_inc, 0)(__VA_ARGS__)
+#define OSC_GP_atomic_fetch_add(dest, value, pe) \
+  OSC_GEN_AMO_STD_P(shmem_, dest, atomic_fetch_add)(dest, value, pe)
+#define OSC_GC_atomic_fetch_add(ctx, dest, value, pe) \
+  OSC_GEN_AMO_STD_P(shmem_ctx_, dest, atomic_fetch_add)(ctx, dest, value, pe)
+#define shmem_atomic_fetch_add(...) \
+  OSC_SEL4(__VA_ARGS__, OSC_GC_atomic_fetch_add, OSC_GP_atomic_fetch_add,      \
+      0)(__VA_ARGS__)
+#define OSC_GP_atomic_add(dest, value, pe) \
+  OSC_GEN_AMO_STD_P(shmem_, dest, atomic_add)(dest, value, pe)
+#define OSC_GC_atomic_add(ctx, dest, value, pe) \
+  OSC_GEN_AMO_STD_P(shmem_ctx_, dest, atomic_add)(ctx, dest, value, pe)
+#define shmem_atomic_add(...) \
+  OSC_SEL4(__VA_ARGS__, OSC_GC_atomic_add, OSC_GP_atomic_add, 0)(__VA_ARGS__)
+#define OSC_GP_atomic_compare_swap_nbi(fetch, dest, cond, value, pe) \
+  OSC_GEN_AMO_STD_P(shmem_, dest, atomic_compare_swap_nbi)(fetch, dest, cond,  \
+      value, pe)
+#define OSC_GC_atomic_compare_swap_nbi(ctx, fetch, dest, cond, value, pe) \
+  OSC_GEN_AMO_STD_P(shmem_ctx_, dest, atomic_compare_swap_nbi)(ctx, fetch,     \
+      dest, cond, value, pe)
+#define shmem_atomic_compare_swap_nbi(...) \
+  OSC_SEL6(__VA_ARGS__, OSC_GC_atomic_compare_swap_nbi,                        \
+      OSC_GP_atomic_compare_swap_nbi, 0)(__VA_ARGS__)
+#define OSC_GP_atomic_fetch_inc_nbi(fetch, dest, pe) \
+  OSC_GEN_AMO_STD_P(shmem_, dest, atomic_fetch_inc_nbi)(fetch, dest, pe)
+#define OSC_GC_atomic_fetch_inc_nbi(ctx, fetch, dest, pe) \
+  OSC_GEN_AMO_STD_P(shmem_ctx_, dest, atomic_fetch_inc_nbi)(ctx, fetch, dest,  \
+      pe)
+#define shmem_atomic_fetch_inc_nbi(...) \
+  OSC_SEL4(__VA_ARGS__, OSC_GC_atomic_fetch_inc_nbi,                           \
+      OSC_GP_atomic_fetch_inc_nbi, 0)(__VA_ARGS__)
+#define OSC_GP_atomic_fetch_add_nbi(fetch, dest, value, pe) \
+  OSC_GEN_AMO_STD_P(shmem_, dest, atomic_fetch_add_nbi)(fetch, dest, value, pe)
+#define OSC_GC_atomic_fetch_add_nbi(ctx, fetch, dest, value, pe) \
+  OSC_GEN_AMO_STD_P(shmem_ctx_, dest, atomic_fetch_add_nbi)(ctx, fetch, dest,  \
+      value, pe)
+#define shmem_atomic_fetch_add_nbi(...) \
+  OSC_SEL5(__VA_ARGS__, OSC_GC_atomic_fetch_add_nbi,                           \
+      OSC_GP_atomic_fetch_add_nbi, 0)(__VA_ARGS__)
+#define OSC_GP_atomic_fetch_and(dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_, dest, atomic_fetch_and)(dest, value, pe)
+#define OSC_GC_atomic_fetch_and(ctx, dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_ctx_, dest, atomic_fetch_and)(ctx, dest, value,  \
+      pe)
+#define shmem_atomic_fetch_and(...) \
+  OSC_SEL4(__VA_ARGS__, OSC_GC_atomic_fetch_and, OSC_GP_atomic_fetch_and,      \
+      0)(__VA_ARGS__)
+#define OSC_GP_atomic_and(dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_, dest, atomic_and)(dest, value, pe)
+#define OSC_GC_atomic_and(ctx, dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_ctx_, dest, atomic_and)(ctx, dest, value, pe)
+#define shmem_atomic_and(...) \
+  OSC_SEL4(__VA_ARGS__, OSC_GC_atomic_and, OSC_GP_atomic_and, 0)(__VA_ARGS__)
+#define OSC_GP_atomic_fetch_and_nbi(fetch, dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_, dest, atomic_fetch_and_nbi)(fetch, dest,       \
+      value, pe)
+#define OSC_GC_atomic_fetch_and_nbi(ctx, fetch, dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_ctx_, dest, atomic_fetch_and_nbi)(ctx, fetch,    \
+      dest, value, pe)
+#define shmem_atomic_fetch_and_nbi(...) \
+  OSC_SEL5(__VA_ARGS__, OSC_GC_atomic_fetch_and_nbi,                           \
+      OSC_GP_atomic_fetch_and_nbi, 0)(__VA_ARGS__)
+#define OSC_GP_atomic_fetch_or(dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_, dest, atomic_fetch_or)(dest, value, pe)
+#define OSC_GC_atomic_fetch_or(ctx, dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_ctx_, dest, atomic_fetch_or)(ctx, dest, value, pe)
+#define shmem_atomic_fetch_or(...) \
+  OSC_SEL4(__VA_ARGS__, OSC_GC_atomic_fetch_or, OSC_GP_atomic_fetch_or,        \
+      0)(__VA_ARGS__)
+#define OSC_GP_atomic_or(dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_, dest, atomic_or)(dest, value, pe)
+#define OSC_GC_atomic_or(ctx, dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_ctx_, dest, atomic_or)(ctx, dest, value, pe)
+#define shmem_atomic_or(...) \
+  OSC_SEL4(__VA_ARGS__, OSC_GC_atomic_or, OSC_GP_atomic_or, 0)(__VA_ARGS__)
+#define OSC_GP_atomic_fetch_or_nbi(fetch, dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_, dest, atomic_fetch_or_nbi)(fetch, dest,        \
+      value, pe)
+#define OSC_GC_atomic_fetch_or_nbi(ctx, fetch, dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_ctx_, dest, atomic_fetch_or_nbi)(ctx, fetch,     \
+      dest, value, pe)
+#define shmem_atomic_fetch_or_nbi(...) \
+  OSC_SEL5(__VA_ARGS__, OSC_GC_atomic_fetch_or_nbi,                            \
+      OSC_GP_atomic_fetch_or_nbi, 0)(__VA_ARGS__)
+#define OSC_GP_atomic_fetch_xor(dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_, dest, atomic_fetch_xor)(dest, value, pe)
+#define OSC_GC_atomic_fetch_xor(ctx, dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_ctx_, dest, atomic_fetch_xor)(ctx, dest, value,  \
+      pe)
+#define shmem_atomic_fetch_xor(...) \
+  OSC_SEL4(__VA_ARGS__, OSC_GC_atomic_fetch_xor, OSC_GP_atomic_fetch_xor,      \
+      0)(__VA_ARGS__)
+#define OSC_GP_atomic_xor(dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_, dest, atomic_xor)(dest, value, pe)
+#define OSC_GC_atomic_xor(ctx, dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_ctx_, dest, atomic_xor)(ctx, dest, value, pe)
+#define shmem_atomic_xor(...) \
+  OSC_SEL4(__VA_ARGS__, OSC_GC_atomic_xor, OSC_GP_atomic_xor, 0)(__VA_ARGS__)
+#define OSC_GP_atomic_fetch_xor_nbi(fetch, dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_, dest, atomic_fetch_xor_nbi)(fetch, dest,       \
+      value, pe)
+#define OSC_GC_atomic_fetch_xor_nbi(ctx, fetch, dest, value, pe) \
+  OSC_GEN_AMO_BITWISE_P(shmem_ctx_, dest, atomic_fetch_xor_nbi)(ctx, fetch,    \
+      dest, value, pe)
+#define shmem_atomic_fetch_xor_nbi(...) \
+  OSC_SEL5(__VA_ARGS__, OSC_GC_atomic_fetch_xor_nbi,                           \
+      OSC_GP_atomic_fetch_xor_nbi, 0)(__VA_ARGS__)
 #endif /* C11 */
 
 #ifdef __cplusplus

@@ -25,21 +25,25 @@ static uint64_t bits_of(const void *v, size_t w) {
   return u;
 }
 
-/* the operation on dest's copy on PE pe; old != NULL waits for the value */
-static int amo(const void *dest, size_t w, int pe, int op, uint64_t v,
-               uint64_t cond, uint64_t *old) {
+/* the operation on dest's copy on PE pe of context ctx (its stream; a
+ * team context's PE numbers are team indices); old != NULL waits for the
+ * value */
+static int amo_ctx(shmem_ctx_t ctx, const void *dest, size_t w, int pe, int op,
+                   uint64_t v, uint64_t cond, uint64_t *old) {
   size_t off;
   char *target;
+  void *st = NULL;
   int rc;
   if (old != NULL) *old = 0;
-  if (!osc_g.initialized) return osc_set_error(OSC_ERR_NOTINIT, "not initialised");
-  if (pe < 0 || pe >= osc_g.npes) return osc_set_error(OSC_ERR_ARG, "PE %d", pe);
+  pe = osc_ctx_target(ctx, pe, &st);
+  if (pe < 0) return osc_last_error_code();
+  if (pe >= osc_g.npes) return osc_set_error(OSC_ERR_ARG, "PE %d", pe);
   rc = osc_sym_range(dest, w, &off);
   if (rc != OSC_OK) return rc;
   target = osc_g.base + (size_t)pe * osc_g.slice + off;
   if (((uintptr_t)target % w) != 0)
     return osc_set_error(OSC_ERR_PTR, "atomic target %p is not %zu-byte aligned", dest, w);
-  rc = oscd_amo(target, (int)w, op, v, cond, old, osc_g.stream, osc_g.wait_spin_us);
+  rc = oscd_amo(target, (int)w, op, v, cond, old, st, osc_g.wait_spin_us);
   if (rc == OSC_ERR_HIP) return osc_hip_error(rc, "atomic operation");
   return rc == OSC_OK ? OSC_OK : osc_set_error(rc, "atomic operation");
 }
@@ -73,110 +77,223 @@ static void deliver(void *fetch, uint64_t value, size_t w) {
   }
 
 /* fetching operation returning _type */
-#define OSC_AMO_GET(_type, _typename, _dest, _pe, _op, _v, _cond)              \
+#define OSC_AMO_GET(_type, _typename, _ctx, _dest, _pe, _op, _v, _cond)        \
   do {                                                                         \
     uint64_t old_;                                                             \
-    amo(_dest, sizeof(_type), _pe, _op, _v, _cond, &old_);                     \
+    amo_ctx(_ctx, _dest, sizeof(_type), _pe, _op, _v, _cond, &old_);           \
     return from_##_typename(old_);                                             \
   } while (0)
 
 /* fetching operation delivered to *fetch */
-#define OSC_AMO_NBI(_type, _fetch, _dest, _pe, _op, _v, _cond)                 \
+#define OSC_AMO_NBI(_type, _ctx, _fetch, _dest, _pe, _op, _v, _cond)           \
   do {                                                                         \
     uint64_t old_;                                                             \
-    if (amo(_dest, sizeof(_type), _pe, _op, _v, _cond, &old_) == OSC_OK)       \
+    if (amo_ctx(_ctx, _dest, sizeof(_type), _pe, _op, _v, _cond, &old_) ==     \
+        OSC_OK)                                                                \
       deliver(_fetch, old_, sizeof(_type));                                    \
   } while (0)
 
 #define B(_x) bits_of(&(_x), sizeof(_x))
 
+/* Every routine is its context form on SHMEM_CTX_DEFAULT. */
 #define OSC_DEF_AMO_EXT(_type, _typename)                                      \
+  _type shmem_ctx_##_typename##_atomic_fetch(shmem_ctx_t ctx,                  \
+      const _type *source, int pe) {                                           \
+    OSC_AMO_GET(_type, _typename, ctx, source, pe, OSCD_AMO_FETCH, 0, 0);      \
+  }                                                                            \
   _type shmem_##_typename##_atomic_fetch(const _type *source, int pe) {        \
-    OSC_AMO_GET(_type, _typename, source, pe, OSCD_AMO_FETCH, 0, 0);           \
+    return shmem_ctx_##_typename##_atomic_fetch(SHMEM_CTX_DEFAULT, source,     \
+        pe);                                                                   \
+  }                                                                            \
+  void shmem_ctx_##_typename##_atomic_set(shmem_ctx_t ctx, _type *dest,        \
+      _type value, int pe) {                                                   \
+    amo_ctx(ctx, dest, sizeof(_type), pe, OSCD_AMO_SET, B(value), 0, NULL);    \
   }                                                                            \
   void shmem_##_typename##_atomic_set(_type *dest, _type value, int pe) {      \
-    amo(dest, sizeof(_type), pe, OSCD_AMO_SET, B(value), 0, NULL);             \
+    shmem_ctx_##_typename##_atomic_set(SHMEM_CTX_DEFAULT, dest, value, pe);    \
+  }                                                                            \
+  _type shmem_ctx_##_typename##_atomic_swap(shmem_ctx_t ctx, _type *dest,      \
+      _type value, int pe) {                                                   \
+    OSC_AMO_GET(_type, _typename, ctx, dest, pe, OSCD_AMO_SWAP, B(value), 0);  \
   }                                                                            \
   _type shmem_##_typename##_atomic_swap(_type *dest, _type value, int pe) {    \
-    OSC_AMO_GET(_type, _typename, dest, pe, OSCD_AMO_SWAP, B(value), 0);       \
+    return shmem_ctx_##_typename##_atomic_swap(SHMEM_CTX_DEFAULT, dest, value, \
+        pe);                                                                   \
+  }                                                                            \
+  void shmem_ctx_##_typename##_atomic_fetch_nbi(shmem_ctx_t ctx, _type *fetch, \
+      const _type *source, int pe) {                                           \
+    OSC_AMO_NBI(_type, ctx, fetch, source, pe, OSCD_AMO_FETCH, 0, 0);          \
   }                                                                            \
   void shmem_##_typename##_atomic_fetch_nbi(_type *fetch, const _type *source, \
-                                            int pe) {                          \
-    OSC_AMO_NBI(_type, fetch, source, pe, OSCD_AMO_FETCH, 0, 0);               \
+      int pe) {                                                                \
+    shmem_ctx_##_typename##_atomic_fetch_nbi(SHMEM_CTX_DEFAULT, fetch, source, \
+        pe);                                                                   \
+  }                                                                            \
+  void shmem_ctx_##_typename##_atomic_swap_nbi(shmem_ctx_t ctx, _type *fetch,  \
+      _type *dest, _type value, int pe) {                                      \
+    OSC_AMO_NBI(_type, ctx, fetch, dest, pe, OSCD_AMO_SWAP, B(value), 0);      \
   }                                                                            \
   void shmem_##_typename##_atomic_swap_nbi(_type *fetch, _type *dest,          \
-                                           _type value, int pe) {              \
-    OSC_AMO_NBI(_type, fetch, dest, pe, OSCD_AMO_SWAP, B(value), 0);           \
+      _type value, int pe) {                                                   \
+    shmem_ctx_##_typename##_atomic_swap_nbi(SHMEM_CTX_DEFAULT, fetch, dest,    \
+        value, pe);                                                            \
   }
 
 #define OSC_DEF_AMO_STD(_type, _typename)                                      \
+  _type shmem_ctx_##_typename##_atomic_compare_swap(shmem_ctx_t ctx,           \
+      _type *dest, _type cond, _type value, int pe) {                          \
+    OSC_AMO_GET(_type, _typename, ctx, dest, pe, OSCD_AMO_CSWAP, B(value),     \
+        B(cond));                                                              \
+  }                                                                            \
   _type shmem_##_typename##_atomic_compare_swap(_type *dest, _type cond,       \
-                                                _type value, int pe) {         \
-    OSC_AMO_GET(_type, _typename, dest, pe, OSCD_AMO_CSWAP, B(value), B(cond)); \
+      _type value, int pe) {                                                   \
+    return shmem_ctx_##_typename##_atomic_compare_swap(SHMEM_CTX_DEFAULT,      \
+        dest, cond, value, pe);                                                \
+  }                                                                            \
+  _type shmem_ctx_##_typename##_atomic_fetch_inc(shmem_ctx_t ctx, _type *dest, \
+      int pe) {                                                                \
+    OSC_AMO_GET(_type, _typename, ctx, dest, pe, OSCD_AMO_ADD, 1, 0);          \
   }                                                                            \
   _type shmem_##_typename##_atomic_fetch_inc(_type *dest, int pe) {            \
-    OSC_AMO_GET(_type, _typename, dest, pe, OSCD_AMO_ADD, 1, 0);               \
+    return shmem_ctx_##_typename##_atomic_fetch_inc(SHMEM_CTX_DEFAULT, dest,   \
+        pe);                                                                   \
+  }                                                                            \
+  void shmem_ctx_##_typename##_atomic_inc(shmem_ctx_t ctx, _type *dest,        \
+      int pe) {                                                                \
+    amo_ctx(ctx, dest, sizeof(_type), pe, OSCD_AMO_ADD, 1, 0, NULL);           \
   }                                                                            \
   void shmem_##_typename##_atomic_inc(_type *dest, int pe) {                   \
-    amo(dest, sizeof(_type), pe, OSCD_AMO_ADD, 1, 0, NULL);                    \
+    shmem_ctx_##_typename##_atomic_inc(SHMEM_CTX_DEFAULT, dest, pe);           \
+  }                                                                            \
+  _type shmem_ctx_##_typename##_atomic_fetch_add(shmem_ctx_t ctx, _type *dest, \
+      _type value, int pe) {                                                   \
+    OSC_AMO_GET(_type, _typename, ctx, dest, pe, OSCD_AMO_ADD, B(value), 0);   \
   }                                                                            \
   _type shmem_##_typename##_atomic_fetch_add(_type *dest, _type value,         \
-                                             int pe) {                         \
-    OSC_AMO_GET(_type, _typename, dest, pe, OSCD_AMO_ADD, B(value), 0);        \
+      int pe) {                                                                \
+    return shmem_ctx_##_typename##_atomic_fetch_add(SHMEM_CTX_DEFAULT, dest,   \
+        value, pe);                                                            \
+  }                                                                            \
+  void shmem_ctx_##_typename##_atomic_add(shmem_ctx_t ctx, _type *dest,        \
+      _type value, int pe) {                                                   \
+    amo_ctx(ctx, dest, sizeof(_type), pe, OSCD_AMO_ADD, B(value), 0, NULL);    \
   }                                                                            \
   void shmem_##_typename##_atomic_add(_type *dest, _type value, int pe) {      \
-    amo(dest, sizeof(_type), pe, OSCD_AMO_ADD, B(value), 0, NULL);             \
+    shmem_ctx_##_typename##_atomic_add(SHMEM_CTX_DEFAULT, dest, value, pe);    \
   }                                                                            \
-  void shmem_##_typename##_atomic_compare_swap_nbi(                            \
+  void shmem_ctx_##_typename##_atomic_compare_swap_nbi(shmem_ctx_t ctx,        \
       _type *fetch, _type *dest, _type cond, _type value, int pe) {            \
-    OSC_AMO_NBI(_type, fetch, dest, pe, OSCD_AMO_CSWAP, B(value), B(cond));    \
+    OSC_AMO_NBI(_type, ctx, fetch, dest, pe, OSCD_AMO_CSWAP, B(value),         \
+        B(cond));                                                              \
+  }                                                                            \
+  void shmem_##_typename##_atomic_compare_swap_nbi(_type *fetch, _type *dest,  \
+      _type cond, _type value, int pe) {                                       \
+    shmem_ctx_##_typename##_atomic_compare_swap_nbi(SHMEM_CTX_DEFAULT, fetch,  \
+        dest, cond, value, pe);                                                \
+  }                                                                            \
+  void shmem_ctx_##_typename##_atomic_fetch_inc_nbi(shmem_ctx_t ctx,           \
+      _type *fetch, _type *dest, int pe) {                                     \
+    OSC_AMO_NBI(_type, ctx, fetch, dest, pe, OSCD_AMO_ADD, 1, 0);              \
   }                                                                            \
   void shmem_##_typename##_atomic_fetch_inc_nbi(_type *fetch, _type *dest,     \
-                                                int pe) {                      \
-    OSC_AMO_NBI(_type, fetch, dest, pe, OSCD_AMO_ADD, 1, 0);                   \
+      int pe) {                                                                \
+    shmem_ctx_##_typename##_atomic_fetch_inc_nbi(SHMEM_CTX_DEFAULT, fetch,     \
+        dest, pe);                                                             \
   }                                                                            \
-  void shmem_##_typename##_atomic_fetch_add_nbi(                               \
+  void shmem_ctx_##_typename##_atomic_fetch_add_nbi(shmem_ctx_t ctx,           \
       _type *fetch, const _type *dest, _type value, int pe) {                  \
-    OSC_AMO_NBI(_type, fetch, dest, pe, OSCD_AMO_ADD, B(value), 0);            \
+    OSC_AMO_NBI(_type, ctx, fetch, dest, pe, OSCD_AMO_ADD, B(value), 0);       \
+  }                                                                            \
+  void shmem_##_typename##_atomic_fetch_add_nbi(_type *fetch,                  \
+      const _type *dest, _type value, int pe) {                                \
+    shmem_ctx_##_typename##_atomic_fetch_add_nbi(SHMEM_CTX_DEFAULT, fetch,     \
+        dest, value, pe);                                                      \
   }
 
 #define OSC_DEF_AMO_BITWISE(_type, _typename)                                  \
+  _type shmem_ctx_##_typename##_atomic_fetch_and(shmem_ctx_t ctx, _type *dest, \
+      _type value, int pe) {                                                   \
+    OSC_AMO_GET(_type, _typename, ctx, dest, pe, OSCD_AMO_AND, B(value), 0);   \
+  }                                                                            \
   _type shmem_##_typename##_atomic_fetch_and(_type *dest, _type value,         \
-                                             int pe) {                         \
-    OSC_AMO_GET(_type, _typename, dest, pe, OSCD_AMO_AND, B(value), 0);        \
+      int pe) {                                                                \
+    return shmem_ctx_##_typename##_atomic_fetch_and(SHMEM_CTX_DEFAULT, dest,   \
+        value, pe);                                                            \
   }                                                                            \
-  _type shmem_##_typename##_atomic_fetch_or(_type *dest, _type value,          \
-                                            int pe) {                          \
-    OSC_AMO_GET(_type, _typename, dest, pe, OSCD_AMO_OR, B(value), 0);         \
-  }                                                                            \
-  _type shmem_##_typename##_atomic_fetch_xor(_type *dest, _type value,         \
-                                             int pe) {                         \
-    OSC_AMO_GET(_type, _typename, dest, pe, OSCD_AMO_XOR, B(value), 0);        \
+  void shmem_ctx_##_typename##_atomic_and(shmem_ctx_t ctx, _type *dest,        \
+      _type value, int pe) {                                                   \
+    amo_ctx(ctx, dest, sizeof(_type), pe, OSCD_AMO_AND, B(value), 0, NULL);    \
   }                                                                            \
   void shmem_##_typename##_atomic_and(_type *dest, _type value, int pe) {      \
-    amo(dest, sizeof(_type), pe, OSCD_AMO_AND, B(value), 0, NULL);             \
+    shmem_ctx_##_typename##_atomic_and(SHMEM_CTX_DEFAULT, dest, value, pe);    \
   }                                                                            \
-  void shmem_##_typename##_atomic_or(_type *dest, _type value, int pe) {       \
-    amo(dest, sizeof(_type), pe, OSCD_AMO_OR, B(value), 0, NULL);              \
-  }                                                                            \
-  void shmem_##_typename##_atomic_xor(_type *dest, _type value, int pe) {      \
-    amo(dest, sizeof(_type), pe, OSCD_AMO_XOR, B(value), 0, NULL);             \
+  void shmem_ctx_##_typename##_atomic_fetch_and_nbi(shmem_ctx_t ctx,           \
+      _type *fetch, _type *dest, _type value, int pe) {                        \
+    OSC_AMO_NBI(_type, ctx, fetch, dest, pe, OSCD_AMO_AND, B(value), 0);       \
   }                                                                            \
   void shmem_##_typename##_atomic_fetch_and_nbi(_type *fetch, _type *dest,     \
-                                                _type value, int pe) {         \
-    OSC_AMO_NBI(_type, fetch, dest, pe, OSCD_AMO_AND, B(value), 0);            \
+      _type value, int pe) {                                                   \
+    shmem_ctx_##_typename##_atomic_fetch_and_nbi(SHMEM_CTX_DEFAULT, fetch,     \
+        dest, value, pe);                                                      \
+  }                                                                            \
+  _type shmem_ctx_##_typename##_atomic_fetch_or(shmem_ctx_t ctx, _type *dest,  \
+      _type value, int pe) {                                                   \
+    OSC_AMO_GET(_type, _typename, ctx, dest, pe, OSCD_AMO_OR, B(value), 0);    \
+  }                                                                            \
+  _type shmem_##_typename##_atomic_fetch_or(_type *dest, _type value,          \
+      int pe) {                                                                \
+    return shmem_ctx_##_typename##_atomic_fetch_or(SHMEM_CTX_DEFAULT, dest,    \
+        value, pe);                                                            \
+  }                                                                            \
+  void shmem_ctx_##_typename##_atomic_or(shmem_ctx_t ctx, _type *dest,         \
+      _type value, int pe) {                                                   \
+    amo_ctx(ctx, dest, sizeof(_type), pe, OSCD_AMO_OR, B(value), 0, NULL);     \
+  }                                                                            \
+  void shmem_##_typename##_atomic_or(_type *dest, _type value, int pe) {       \
+    shmem_ctx_##_typename##_atomic_or(SHMEM_CTX_DEFAULT, dest, value, pe);     \
+  }                                                                            \
+  void shmem_ctx_##_typename##_atomic_fetch_or_nbi(shmem_ctx_t ctx,            \
+      _type *fetch, _type *dest, _type value, int pe) {                        \
+    OSC_AMO_NBI(_type, ctx, fetch, dest, pe, OSCD_AMO_OR, B(value), 0);        \
   }                                                                            \
   void shmem_##_typename##_atomic_fetch_or_nbi(_type *fetch, _type *dest,      \
-                                               _type value, int pe) {          \
-    OSC_AMO_NBI(_type, fetch, dest, pe, OSCD_AMO_OR, B(value), 0);             \
+      _type value, int pe) {                                                   \
+    shmem_ctx_##_typename##_atomic_fetch_or_nbi(SHMEM_CTX_DEFAULT, fetch,      \
+        dest, value, pe);                                                      \
+  }                                                                            \
+  _type shmem_ctx_##_typename##_atomic_fetch_xor(shmem_ctx_t ctx, _type *dest, \
+      _type value, int pe) {                                                   \
+    OSC_AMO_GET(_type, _typename, ctx, dest, pe, OSCD_AMO_XOR, B(value), 0);   \
+  }                                                                            \
+  _type shmem_##_typename##_atomic_fetch_xor(_type *dest, _type value,         \
+      int pe) {                                                                \
+    return shmem_ctx_##_typename##_atomic_fetch_xor(SHMEM_CTX_DEFAULT, dest,   \
+        value, pe);                                                            \
+  }                                                                            \
+  void shmem_ctx_##_typename##_atomic_xor(shmem_ctx_t ctx, _type *dest,        \
+      _type value, int pe) {                                                   \
+    amo_ctx(ctx, dest, sizeof(_type), pe, OSCD_AMO_XOR, B(value), 0, NULL);    \
+  }                                                                            \
+  void shmem_##_typename##_atomic_xor(_type *dest, _type value, int pe) {      \
+    shmem_ctx_##_typename##_atomic_xor(SHMEM_CTX_DEFAULT, dest, value, pe);    \
+  }                                                                            \
+  void shmem_ctx_##_typename##_atomic_fetch_xor_nbi(shmem_ctx_t ctx,           \
+      _type *fetch, _type *dest, _type value, int pe) {                        \
+    OSC_AMO_NBI(_type, ctx, fetch, dest, pe, OSCD_AMO_XOR, B(value), 0);       \
   }                                                                            \
   void shmem_##_typename##_atomic_fetch_xor_nbi(_type *fetch, _type *dest,     \
-                                                _type value, int pe) {         \
-    OSC_AMO_NBI(_type, fetch, dest, pe, OSCD_AMO_XOR, B(value), 0);            \
+      _type value, int pe) {                                                   \
+    shmem_ctx_##_typename##_atomic_fetch_xor_nbi(SHMEM_CTX_DEFAULT, fetch,     \
+        dest, value, pe);                                                      \
+  }                                                                            \
+  void shmem_ctx_##_typename##_atomic_xor_nbi(shmem_ctx_t ctx, _type *fetch,   \
+      _type *dest, _type value, int pe) {                                      \
+    OSC_AMO_NBI(_type, ctx, fetch, dest, pe, OSCD_AMO_XOR, B(value), 0);       \
   }                                                                            \
   void shmem_##_typename##_atomic_xor_nbi(_type *fetch, _type *dest,           \
-                                          _type value, int pe) {               \
-    OSC_AMO_NBI(_type, fetch, dest, pe, OSCD_AMO_XOR, B(value), 0);            \
+      _type value, int pe) {                                                   \
+    shmem_ctx_##_typename##_atomic_xor_nbi(SHMEM_CTX_DEFAULT, fetch, dest,     \
+        value, pe);                                                            \
   }
 
 #define OSC_DEF_AMO_DEPR_EXT(_type, _typename)                                 \
@@ -223,7 +340,8 @@ OSC_DEF_AMO_DEPR_EXT(double, double)
 int shmem_test_lock(long *lock) {
   uint64_t old = 1;
   const long me = (long)osc_my_pe_get() + 1;
-  if (amo(lock, sizeof(long), 0, OSCD_AMO_CSWAP, (uint64_t)me, 0, &old) != OSC_OK)
+  if (amo_ctx(SHMEM_CTX_DEFAULT, lock, sizeof(long), 0, OSCD_AMO_CSWAP,
+              (uint64_t)me, 0, &old) != OSC_OK)
     return 1;
   return old != 0;
 }
@@ -232,7 +350,8 @@ void shmem_set_lock(long *lock) {
   const long me = (long)osc_my_pe_get() + 1;
   for (;;) {
     uint64_t old = 1;
-    if (amo(lock, sizeof(long), 0, OSCD_AMO_CSWAP, (uint64_t)me, 0, &old) != OSC_OK)
+    if (amo_ctx(SHMEM_CTX_DEFAULT, lock, sizeof(long), 0, OSCD_AMO_CSWAP,
+              (uint64_t)me, 0, &old) != OSC_OK)
       return; /* error recorded: a bad lock address never succeeds */
     if (old == 0) return;
     sched_yield();
@@ -242,7 +361,7 @@ void shmem_set_lock(long *lock) {
 void shmem_clear_lock(long *lock) {
   /* stream order puts the release after every store of the critical
    * section, blocking or not */
-  amo(lock, sizeof(long), 0, OSCD_AMO_SET, 0, 0, NULL);
+  amo_ctx(SHMEM_CTX_DEFAULT, lock, sizeof(long), 0, OSCD_AMO_SET, 0, 0, NULL);
 }
 
 /* ---- deprecated cache routines: no effect ------------------------------- */

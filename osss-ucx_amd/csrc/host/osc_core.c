@@ -148,6 +148,7 @@ int osc_init(int npes, size_t heap_bytes_per_pe, int device) {
 
 int osc_finalize(void) {
   if (!osc_g.initialized) return OSC_OK;
+  osc_ctx_release_all();
   oscd_stream_sync(osc_g.stream);
   oscd_stream_destroy(osc_g.stream);
   oscd_wait_fini();

@@ -66,6 +66,11 @@ extern osc_team osc_team_shared;
 
 /* completion of everything enqueued on the library stream so far */
 int osc_wait(void);
+/* destroys the contexts the program did not (shmem_api.c) */
+void osc_ctx_release_all(void);
+/* a context's stream and the global PE of its PE number pe; -1 with the
+ * error recorded if the context or the PE is invalid (shmem_api.c) */
+int osc_ctx_target(void *ctx, int pe, void **stream);
 
 /* errors */
 int osc_set_error(int code, const char *fmt, ...);

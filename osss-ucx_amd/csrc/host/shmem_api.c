@@ -187,8 +187,8 @@ int osc_heap_call_alloc(size_t size, size_t align, int zero, size_t *off) {
 
 int osc_heap_call_free(size_t off) {
   int rc;
-  /* outstanding kernels may still use the object */
-  oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
+  /* outstanding work of any context may still use the object */
+  oscd_device_sync();
   pthread_mutex_lock(&osc_g.lock);
   rc = osc_heap_free(off);
   pthread_mutex_unlock(&osc_g.lock);
@@ -303,37 +303,202 @@ static char *remote(const void *sym, ptrdiff_t lo, size_t bytes, int pe) {
   return osc_g.base + (size_t)pe * osc_g.slice + off - lo;
 }
 
-static void xfer(void *dst, const void *src, size_t nbytes, int blocking) {
+/* ---- communication contexts --------------------------------------------------
+ * A context is a device stream (and, from shmem_team_create_ctx, a team
+ * whose indices its PE numbers are).  SHMEM_CTX_DEFAULT stands for the
+ * library stream.  Contexts not destroyed by the program are released by
+ * osc_finalize (osc_ctx_release_all). */
+#define OSC_CTX_MAGIC 0x6f736378u
+typedef struct osc_ctx {
+  uint32_t magic;
+  void *stream;
+  shmem_team_t team; /* NULL: PE numbers are global */
+  long options;
+  struct osc_ctx *next;
+} osc_ctx;
+static osc_ctx g_ctx_default = {OSC_CTX_MAGIC, NULL, NULL, 0, NULL};
+shmem_ctx_t SHMEM_CTX_DEFAULT = &g_ctx_default;
+static pthread_mutex_t g_ctx_mu = PTHREAD_MUTEX_INITIALIZER;
+static osc_ctx *g_ctxs; /* created and not destroyed */
+
+static const osc_ctx *ctx_get(shmem_ctx_t ctx) {
+  const osc_ctx *c = (const osc_ctx *)ctx;
+  if (c == NULL || c->magic != OSC_CTX_MAGIC) {
+    osc_set_error(OSC_ERR_ARG, "invalid context");
+    return NULL;
+  }
+  if (!osc_g.initialized) {
+    osc_set_error(OSC_ERR_NOTINIT, "not initialised");
+    return NULL;
+  }
+  return c;
+}
+
+/* The context's stream and the global PE of its PE number pe; -1 (error
+ * recorded) if either is invalid. */
+int osc_ctx_target(void *ctx, int pe, void **stream) {
+  const osc_ctx *c = ctx_get(ctx);
+  const osc_team *t;
+  if (c == NULL) return -1;
+  *stream = c == &g_ctx_default ? osc_g.stream : c->stream;
+  if (c->team == NULL) return pe;
+  t = osc_team_get(c->team);
+  if (t == NULL) return osc_set_error(OSC_ERR_TEAM, "the context's team is gone"), -1;
+  if (pe < 0 || pe >= t->npes)
+    return osc_set_error(OSC_ERR_ARG, "PE %d outside the context's team of %d", pe,
+                         t->npes), -1;
+  return t->start + pe * t->stride;
+}
+
+static int ctx_new(shmem_team_t team, long options, shmem_ctx_t *out) {
+  osc_ctx *c;
+  int rc;
+  if (out == NULL) return osc_set_error(OSC_ERR_ARG, "NULL context address"), -1;
+  *out = SHMEM_CTX_INVALID;
+  if (!osc_g.initialized) return osc_set_error(OSC_ERR_NOTINIT, "not initialised"), -1;
+  c = (osc_ctx *)calloc(1, sizeof(*c));
+  if (c == NULL) return osc_set_error(OSC_ERR_NOMEM, "context"), -1;
+  rc = oscd_stream_create(&c->stream);
+  if (rc != OSC_OK) {
+    free(c);
+    return osc_hip_error(rc, "context stream"), -1;
+  }
+  c->magic = OSC_CTX_MAGIC;
+  c->team = team;
+  c->options = options;
+  pthread_mutex_lock(&g_ctx_mu);
+  c->next = g_ctxs;
+  g_ctxs = c;
+  pthread_mutex_unlock(&g_ctx_mu);
+  *out = c;
+  return 0;
+}
+
+int shmem_ctx_create(long options, shmem_ctx_t *ctx) {
+  return ctx_new(NULL, options, ctx);
+}
+
+int shmem_team_create_ctx(shmem_team_t team, long options, shmem_ctx_t *ctx) {
+  if (ctx != NULL) *ctx = SHMEM_CTX_INVALID;
+  if (osc_team_get(team) == NULL) return osc_set_error(OSC_ERR_TEAM, "invalid team"), -1;
+  return ctx_new(team, options, ctx);
+}
+
+static void ctx_free(osc_ctx *c) {
+  oscd_stream_sync(c->stream);
+  oscd_stream_destroy(c->stream);
+  c->magic = 0;
+  free(c);
+}
+
+void shmem_ctx_destroy(shmem_ctx_t ctx) {
+  osc_ctx *c = (osc_ctx *)ctx, **pp;
+  if (ctx_get(ctx) == NULL) return;
+  if (c == &g_ctx_default) {
+    osc_set_error(OSC_ERR_ARG, "the default context cannot be destroyed");
+    return;
+  }
+  pthread_mutex_lock(&g_ctx_mu);
+  for (pp = &g_ctxs; *pp != NULL; pp = &(*pp)->next)
+    if (*pp == c) {
+      *pp = c->next;
+      break;
+    }
+  pthread_mutex_unlock(&g_ctx_mu);
+  ctx_free(c);
+}
+
+void osc_ctx_release_all(void) {
+  pthread_mutex_lock(&g_ctx_mu);
+  while (g_ctxs != NULL) {
+    osc_ctx *c = g_ctxs;
+    g_ctxs = c->next;
+    ctx_free(c);
+  }
+  pthread_mutex_unlock(&g_ctx_mu);
+}
+
+int shmem_ctx_get_team(shmem_ctx_t ctx, shmem_team_t *team) {
+  const osc_ctx *c;
+  if (team == NULL) return osc_set_error(OSC_ERR_ARG, "NULL team address"), -1;
+  *team = SHMEM_TEAM_INVALID;
+  c = ctx_get(ctx);
+  if (c == NULL) return -1;
+  *team = c->team != NULL ? c->team : SHMEM_TEAM_WORLD;
+  return 0;
+}
+
+/* operations of one context are ordered by its stream already */
+void shmem_ctx_fence(shmem_ctx_t ctx) { (void)ctx_get(ctx); }
+
+void shmem_ctx_quiet(shmem_ctx_t ctx) {
+  const osc_ctx *c = ctx_get(ctx);
+  int rc;
+  if (c == NULL) return;
+  rc = oscd_stream_wait(c == &g_ctx_default ? osc_g.stream : c->stream,
+                        osc_g.wait_spin_us);
+  if (rc != OSC_OK) osc_hip_error(rc, "context quiet");
+}
+
+/* A copy on stream st; the host side is copied once the stream's earlier
+ * work is done. */
+static void xfer(void *st, void *dst, const void *src, size_t nbytes, int blocking) {
   int rc;
   if (nbytes == 0) return;
   if (is_sym(dst) && is_sym(src)) {
-    rc = oscd_copy(dst, src, nbytes, osc_g.stream);
-    if (rc == OSC_OK && blocking) rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
+    rc = oscd_copy(dst, src, nbytes, st);
+    if (rc == OSC_OK && blocking) rc = oscd_stream_wait(st, osc_g.wait_spin_us);
   } else if (is_sym(dst)) {
-    rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
+    rc = oscd_stream_wait(st, osc_g.wait_spin_us);
     if (rc == OSC_OK) rc = oscd_memcpy_h2d(dst, src, nbytes);
   } else {
-    rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
+    rc = oscd_stream_wait(st, osc_g.wait_spin_us);
     if (rc == OSC_OK) rc = oscd_memcpy_d2h(dst, src, nbytes);
   }
   if (rc != OSC_OK) osc_hip_error(rc, "RMA copy");
 }
 
+static void rma_mem(shmem_ctx_t ctx, void *dest, const void *source,
+                    size_t nbytes, int pe, int is_put, int blocking) {
+  void *st = NULL;
+  const int g = osc_ctx_target(ctx, pe, &st);
+  if (g < 0) return;
+  if (is_put) {
+    void *d = remote(dest, 0, nbytes, g);
+    if (d != NULL) xfer(st, d, source, nbytes, blocking);
+  } else {
+    const void *s = remote(source, 0, nbytes, g);
+    if (s != NULL) xfer(st, dest, s, nbytes, blocking);
+  }
+}
+
+void shmem_ctx_putmem(shmem_ctx_t ctx, void *dest, const void *source,
+                      size_t nbytes, int pe) {
+  rma_mem(ctx, dest, source, nbytes, pe, 1, 1);
+}
+void shmem_ctx_putmem_nbi(shmem_ctx_t ctx, void *dest, const void *source,
+                          size_t nbytes, int pe) {
+  rma_mem(ctx, dest, source, nbytes, pe, 1, 0);
+}
+void shmem_ctx_getmem(shmem_ctx_t ctx, void *dest, const void *source,
+                      size_t nbytes, int pe) {
+  rma_mem(ctx, dest, source, nbytes, pe, 0, 1);
+}
+void shmem_ctx_getmem_nbi(shmem_ctx_t ctx, void *dest, const void *source,
+                          size_t nbytes, int pe) {
+  rma_mem(ctx, dest, source, nbytes, pe, 0, 0);
+}
 void shmem_putmem(void *dest, const void *source, size_t nbytes, int pe) {
-  void *d = remote(dest, 0, nbytes, pe);
-  if (d != NULL) xfer(d, source, nbytes, 1);
+  rma_mem(SHMEM_CTX_DEFAULT, dest, source, nbytes, pe, 1, 1);
 }
 void shmem_putmem_nbi(void *dest, const void *source, size_t nbytes, int pe) {
-  void *d = remote(dest, 0, nbytes, pe);
-  if (d != NULL) xfer(d, source, nbytes, 0);
+  rma_mem(SHMEM_CTX_DEFAULT, dest, source, nbytes, pe, 1, 0);
 }
 void shmem_getmem(void *dest, const void *source, size_t nbytes, int pe) {
-  const void *s = remote(source, 0, nbytes, pe);
-  if (s != NULL) xfer(dest, s, nbytes, 1);
+  rma_mem(SHMEM_CTX_DEFAULT, dest, source, nbytes, pe, 0, 1);
 }
 void shmem_getmem_nbi(void *dest, const void *source, size_t nbytes, int pe) {
-  const void *s = remote(source, 0, nbytes, pe);
-  if (s != NULL) xfer(dest, s, nbytes, 0);
+  rma_mem(SHMEM_CTX_DEFAULT, dest, source, nbytes, pe, 0, 0);
 }
 
 /* Strided put (is_put) or get of n elements of es bytes; strides in
@@ -341,16 +506,19 @@ void shmem_getmem_nbi(void *dest, const void *source, size_t nbytes, int pe) {
  * element in index order): with a zero stride on the written side only
  * the last element survives, so only it is moved.  A host-side array is
  * packed into (or unpacked from) a contiguous device staging buffer. */
-static void strided(void *dest, const void *src, size_t es, ptrdiff_t tst,
-                    ptrdiff_t sst, size_t n, int pe, int is_put) {
+static void strided(shmem_ctx_t ctx, void *dest, const void *src, size_t es,
+                    ptrdiff_t tst, ptrdiff_t sst, size_t n, int pe, int is_put) {
   const ptrdiff_t rst = is_put ? tst : sst; /* remote stride */
   const ptrdiff_t lst = is_put ? sst : tst; /* local stride  */
   void *rsym = is_put ? dest : (void *)src;
   void *lp = is_put ? (void *)src : dest;
   ptrdiff_t lo, hi;
   char *r, *stage = NULL;
+  void *st = NULL;
   int rc = OSC_OK;
   if (n == 0) return;
+  pe = osc_ctx_target(ctx, pe, &st);
+  if (pe < 0) return;
   {
     /* every byte offset below, (n-1)*|stride|*es + es, must fit ptrdiff_t */
     const size_t at = (size_t)(rst < 0 ? -rst : rst), al = (size_t)(lst < 0 ? -lst : lst);
@@ -374,11 +542,11 @@ static void strided(void *dest, const void *src, size_t es, ptrdiff_t tst,
   if (is_sym(lp)) {
     rc = is_put ? oscd_copy_strided(r, lp, es, (uint64_t)(rst * (ptrdiff_t)es),
                                     (uint64_t)(lst * (ptrdiff_t)es), n,
-                                    osc_g.stream)
+                                    st)
                 : oscd_copy_strided(lp, r, es, (uint64_t)(lst * (ptrdiff_t)es),
                                     (uint64_t)(rst * (ptrdiff_t)es), n,
-                                    osc_g.stream);
-    if (rc == OSC_OK) rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
+                                    st);
+    if (rc == OSC_OK) rc = oscd_stream_wait(st, osc_g.wait_spin_us);
     if (rc != OSC_OK) osc_hip_error(rc, "strided RMA");
     return;
   }
@@ -393,16 +561,16 @@ static void strided(void *dest, const void *src, size_t es, ptrdiff_t tst,
     if (rc == OSC_OK && is_put) {
       for (t = 0; t < n; ++t)
         memcpy(host + t * es, (char *)lp + (ptrdiff_t)t * lst * (ptrdiff_t)es, es);
-      rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
+      rc = oscd_stream_wait(st, osc_g.wait_spin_us);
       if (rc == OSC_OK) rc = oscd_memcpy_h2d(stage, host, n * es);
       if (rc == OSC_OK)
         rc = oscd_copy_strided(r, stage, es, (uint64_t)(rst * (ptrdiff_t)es), es,
-                               n, osc_g.stream);
-      if (rc == OSC_OK) rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
+                               n, st);
+      if (rc == OSC_OK) rc = oscd_stream_wait(st, osc_g.wait_spin_us);
     } else if (rc == OSC_OK) {
       rc = oscd_copy_strided(stage, r, es, es, (uint64_t)(rst * (ptrdiff_t)es), n,
-                             osc_g.stream);
-      if (rc == OSC_OK) rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
+                             st);
+      if (rc == OSC_OK) rc = oscd_stream_wait(st, osc_g.wait_spin_us);
       if (rc == OSC_OK) rc = oscd_memcpy_d2h(host, stage, n * es);
       if (rc == OSC_OK)
         for (t = 0; t < n; ++t)
@@ -416,12 +584,16 @@ static void strided(void *dest, const void *src, size_t es, ptrdiff_t tst,
 
 /* put then signal update on the same stream, so the data is in place
  * before the signal changes (put_signal.h:28-50, SHMEM_SIGNAL_SET / ADD) */
-static void put_signal(void *dest, const void *src, size_t nbytes,
-                       uint64_t *sig_addr, uint64_t signal, int sig_op, int pe,
-                       int blocking) {
-  char *d = remote(dest, 0, nbytes, pe);
+static void put_signal(shmem_ctx_t ctx, void *dest, const void *src,
+                       size_t nbytes, uint64_t *sig_addr, uint64_t signal,
+                       int sig_op, int pe, int blocking) {
+  void *st = NULL;
+  char *d;
   uint64_t *s;
   int rc;
+  pe = osc_ctx_target(ctx, pe, &st);
+  if (pe < 0) return;
+  d = remote(dest, 0, nbytes, pe);
   if (d == NULL) return;
   s = (uint64_t *)remote(sig_addr, 0, sizeof(uint64_t), pe);
   if (s == NULL) return;
@@ -429,21 +601,32 @@ static void put_signal(void *dest, const void *src, size_t nbytes,
     osc_set_error(OSC_ERR_ARG, "signal op %d", sig_op);
     return;
   }
-  xfer(d, src, nbytes, 0);
-  rc = oscd_signal(s, signal, sig_op, osc_g.stream);
-  if (rc == OSC_OK && blocking) rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
+  xfer(st, d, src, nbytes, 0);
+  rc = oscd_signal(s, signal, sig_op, st);
+  if (rc == OSC_OK && blocking) rc = oscd_stream_wait(st, osc_g.wait_spin_us);
   if (rc != OSC_OK) osc_hip_error(rc, "signal update");
 }
 
 void shmem_putmem_signal(void *dest, const void *source, size_t nbytes,
                          uint64_t *sig_addr, uint64_t signal, int sig_op,
                          int pe) {
-  put_signal(dest, source, nbytes, sig_addr, signal, sig_op, pe, 1);
+  put_signal(SHMEM_CTX_DEFAULT, dest, source, nbytes, sig_addr, signal, sig_op, pe, 1);
 }
 void shmem_putmem_signal_nbi(void *dest, const void *source, size_t nbytes,
                              uint64_t *sig_addr, uint64_t signal, int sig_op,
                              int pe) {
-  put_signal(dest, source, nbytes, sig_addr, signal, sig_op, pe, 0);
+  put_signal(SHMEM_CTX_DEFAULT, dest, source, nbytes, sig_addr, signal, sig_op, pe, 0);
+}
+void shmem_ctx_putmem_signal(shmem_ctx_t ctx, void *dest, const void *source,
+                             size_t nbytes, uint64_t *sig_addr,
+                             uint64_t signal, int sig_op, int pe) {
+  put_signal(ctx, dest, source, nbytes, sig_addr, signal, sig_op, pe, 1);
+}
+void shmem_ctx_putmem_signal_nbi(shmem_ctx_t ctx, void *dest,
+                                 const void *source, size_t nbytes,
+                                 uint64_t *sig_addr, uint64_t signal,
+                                 int sig_op, int pe) {
+  put_signal(ctx, dest, source, nbytes, sig_addr, signal, sig_op, pe, 0);
 }
 
 static int read_signal(const uint64_t *sig_addr, uint64_t *v) {
@@ -494,89 +677,179 @@ uint64_t shmem_signal_wait_until(uint64_t *sig_addr, int cmp,
   }
 }
 
+/* Every routine is its context form on SHMEM_CTX_DEFAULT. */
 #define OSC_DEF_RMA(_type, _typename)                                          \
+  void shmem_ctx_##_typename##_put(shmem_ctx_t ctx, _type *dest,               \
+                                   const _type *src, size_t nelems, int pe) {  \
+    rma_mem(ctx, dest, src, nelems * sizeof(_type), pe, 1, 1);                 \
+  }                                                                            \
+  void shmem_ctx_##_typename##_put_nbi(shmem_ctx_t ctx, _type *dest,           \
+                                       const _type *src, size_t nelems,        \
+                                       int pe) {                               \
+    rma_mem(ctx, dest, src, nelems * sizeof(_type), pe, 1, 0);                 \
+  }                                                                            \
+  void shmem_ctx_##_typename##_get(shmem_ctx_t ctx, _type *dest,               \
+                                   const _type *src, size_t nelems, int pe) {  \
+    rma_mem(ctx, dest, src, nelems * sizeof(_type), pe, 0, 1);                 \
+  }                                                                            \
+  void shmem_ctx_##_typename##_get_nbi(shmem_ctx_t ctx, _type *dest,           \
+                                       const _type *src, size_t nelems,        \
+                                       int pe) {                               \
+    rma_mem(ctx, dest, src, nelems * sizeof(_type), pe, 0, 0);                 \
+  }                                                                            \
+  void shmem_ctx_##_typename##_p(shmem_ctx_t ctx, _type *dest, _type value,    \
+                                 int pe) {                                     \
+    rma_mem(ctx, dest, &value, sizeof(_type), pe, 1, 1);                       \
+  }                                                                            \
+  _type shmem_ctx_##_typename##_g(shmem_ctx_t ctx, const _type *src, int pe) { \
+    _type v;                                                                   \
+    memset(&v, 0, sizeof(v));                                                  \
+    rma_mem(ctx, &v, src, sizeof(_type), pe, 0, 1);                            \
+    return v;                                                                  \
+  }                                                                            \
+  void shmem_ctx_##_typename##_iput(shmem_ctx_t ctx, _type *dest,              \
+                                    const _type *src, ptrdiff_t tst,           \
+                                    ptrdiff_t sst, size_t nelems, int pe) {    \
+    strided(ctx, dest, src, sizeof(_type), tst, sst, nelems, pe, 1);           \
+  }                                                                            \
+  void shmem_ctx_##_typename##_iget(shmem_ctx_t ctx, _type *dest,              \
+                                    const _type *src, ptrdiff_t tst,           \
+                                    ptrdiff_t sst, size_t nelems, int pe) {    \
+    strided(ctx, dest, src, sizeof(_type), tst, sst, nelems, pe, 0);           \
+  }                                                                            \
+  void shmem_ctx_##_typename##_put_signal(                                     \
+      shmem_ctx_t ctx, _type *dest, const _type *src, size_t nelems,           \
+      uint64_t *sig_addr, uint64_t signal, int sig_op, int pe) {               \
+    put_signal(ctx, dest, src, nelems * sizeof(_type), sig_addr, signal,       \
+               sig_op, pe, 1);                                                 \
+  }                                                                            \
+  void shmem_ctx_##_typename##_put_signal_nbi(                                 \
+      shmem_ctx_t ctx, _type *dest, const _type *src, size_t nelems,           \
+      uint64_t *sig_addr, uint64_t signal, int sig_op, int pe) {               \
+    put_signal(ctx, dest, src, nelems * sizeof(_type), sig_addr, signal,       \
+               sig_op, pe, 0);                                                 \
+  }                                                                            \
   void shmem_##_typename##_put(_type *dest, const _type *src, size_t nelems,   \
                                int pe) {                                       \
-    shmem_putmem(dest, src, nelems * sizeof(_type), pe);                       \
+    shmem_ctx_##_typename##_put(SHMEM_CTX_DEFAULT, dest, src, nelems, pe);     \
   }                                                                            \
   void shmem_##_typename##_put_nbi(_type *dest, const _type *src,              \
                                    size_t nelems, int pe) {                    \
-    shmem_putmem_nbi(dest, src, nelems * sizeof(_type), pe);                   \
+    shmem_ctx_##_typename##_put_nbi(SHMEM_CTX_DEFAULT, dest, src, nelems, pe); \
   }                                                                            \
   void shmem_##_typename##_get(_type *dest, const _type *src, size_t nelems,   \
                                int pe) {                                       \
-    shmem_getmem(dest, src, nelems * sizeof(_type), pe);                       \
+    shmem_ctx_##_typename##_get(SHMEM_CTX_DEFAULT, dest, src, nelems, pe);     \
   }                                                                            \
   void shmem_##_typename##_get_nbi(_type *dest, const _type *src,              \
                                    size_t nelems, int pe) {                    \
-    shmem_getmem_nbi(dest, src, nelems * sizeof(_type), pe);                   \
+    shmem_ctx_##_typename##_get_nbi(SHMEM_CTX_DEFAULT, dest, src, nelems, pe); \
   }                                                                            \
   void shmem_##_typename##_p(_type *dest, _type value, int pe) {               \
-    shmem_putmem(dest, &value, sizeof(_type), pe);                             \
+    shmem_ctx_##_typename##_p(SHMEM_CTX_DEFAULT, dest, value, pe);             \
   }                                                                            \
   _type shmem_##_typename##_g(const _type *src, int pe) {                      \
-    _type v;                                                                   \
-    memset(&v, 0, sizeof(v));                                                  \
-    shmem_getmem(&v, src, sizeof(_type), pe);                                  \
-    return v;                                                                  \
+    return shmem_ctx_##_typename##_g(SHMEM_CTX_DEFAULT, src, pe);              \
   }                                                                            \
   void shmem_##_typename##_iput(_type *dest, const _type *src, ptrdiff_t tst,  \
                                 ptrdiff_t sst, size_t nelems, int pe) {        \
-    strided(dest, src, sizeof(_type), tst, sst, nelems, pe, 1);                \
+    strided(SHMEM_CTX_DEFAULT, dest, src, sizeof(_type), tst, sst, nelems, pe, \
+            1);                                                                \
   }                                                                            \
   void shmem_##_typename##_iget(_type *dest, const _type *src, ptrdiff_t tst,  \
                                 ptrdiff_t sst, size_t nelems, int pe) {        \
-    strided(dest, src, sizeof(_type), tst, sst, nelems, pe, 0);                \
+    strided(SHMEM_CTX_DEFAULT, dest, src, sizeof(_type), tst, sst, nelems, pe, \
+            0);                                                                \
   }                                                                            \
   void shmem_##_typename##_put_signal(_type *dest, const _type *src,           \
                                       size_t nelems, uint64_t *sig_addr,       \
                                       uint64_t signal, int sig_op, int pe) {   \
-    put_signal(dest, src, nelems * sizeof(_type), sig_addr, signal, sig_op,    \
-               pe, 1);                                                         \
+    put_signal(SHMEM_CTX_DEFAULT, dest, src, nelems * sizeof(_type), sig_addr, \
+               signal, sig_op, pe, 1);                                         \
   }                                                                            \
   void shmem_##_typename##_put_signal_nbi(_type *dest, const _type *src,       \
                                           size_t nelems, uint64_t *sig_addr,   \
                                           uint64_t signal, int sig_op,         \
                                           int pe) {                            \
-    put_signal(dest, src, nelems * sizeof(_type), sig_addr, signal, sig_op,    \
-               pe, 0);                                                         \
+    put_signal(SHMEM_CTX_DEFAULT, dest, src, nelems * sizeof(_type), sig_addr, \
+               signal, sig_op, pe, 0);                                         \
   }
 OSC_RMA_TYPES(OSC_DEF_RMA)
 
 #define OSC_DEF_SIZED_RMA(_size)                                               \
+  void shmem_ctx_put##_size(shmem_ctx_t ctx, void *dest, const void *src,      \
+                            size_t nelems, int pe) {                           \
+    rma_mem(ctx, dest, src, nelems * ((_size) / 8), pe, 1, 1);                 \
+  }                                                                            \
+  void shmem_ctx_put##_size##_nbi(shmem_ctx_t ctx, void *dest,                 \
+                                  const void *src, size_t nelems, int pe) {    \
+    rma_mem(ctx, dest, src, nelems * ((_size) / 8), pe, 1, 0);                 \
+  }                                                                            \
+  void shmem_ctx_get##_size(shmem_ctx_t ctx, void *dest, const void *src,      \
+                            size_t nelems, int pe) {                           \
+    rma_mem(ctx, dest, src, nelems * ((_size) / 8), pe, 0, 1);                 \
+  }                                                                            \
+  void shmem_ctx_get##_size##_nbi(shmem_ctx_t ctx, void *dest,                 \
+                                  const void *src, size_t nelems, int pe) {    \
+    rma_mem(ctx, dest, src, nelems * ((_size) / 8), pe, 0, 0);                 \
+  }                                                                            \
+  void shmem_ctx_iput##_size(shmem_ctx_t ctx, void *dest, const void *src,     \
+                             ptrdiff_t tst, ptrdiff_t sst, size_t nelems,      \
+                             int pe) {                                         \
+    strided(ctx, dest, src, (_size) / 8, tst, sst, nelems, pe, 1);             \
+  }                                                                            \
+  void shmem_ctx_iget##_size(shmem_ctx_t ctx, void *dest, const void *src,     \
+                             ptrdiff_t tst, ptrdiff_t sst, size_t nelems,      \
+                             int pe) {                                         \
+    strided(ctx, dest, src, (_size) / 8, tst, sst, nelems, pe, 0);             \
+  }                                                                            \
+  void shmem_ctx_put##_size##_signal(shmem_ctx_t ctx, void *dest,              \
+                                     const void *src, size_t nelems,           \
+                                     uint64_t *sig_addr, uint64_t signal,      \
+                                     int sig_op, int pe) {                     \
+    put_signal(ctx, dest, src, nelems * ((_size) / 8), sig_addr, signal,       \
+               sig_op, pe, 1);                                                 \
+  }                                                                            \
+  void shmem_ctx_put##_size##_signal_nbi(shmem_ctx_t ctx, void *dest,          \
+                                         const void *src, size_t nelems,       \
+                                         uint64_t *sig_addr, uint64_t signal,  \
+                                         int sig_op, int pe) {                 \
+    put_signal(ctx, dest, src, nelems * ((_size) / 8), sig_addr, signal,       \
+               sig_op, pe, 0);                                                 \
+  }                                                                            \
   void shmem_put##_size(void *dest, const void *src, size_t nelems, int pe) {  \
-    shmem_putmem(dest, src, nelems * ((_size) / 8), pe);                       \
+    shmem_ctx_put##_size(SHMEM_CTX_DEFAULT, dest, src, nelems, pe);            \
   }                                                                            \
   void shmem_put##_size##_nbi(void *dest, const void *src, size_t nelems,      \
                               int pe) {                                        \
-    shmem_putmem_nbi(dest, src, nelems * ((_size) / 8), pe);                   \
+    shmem_ctx_put##_size##_nbi(SHMEM_CTX_DEFAULT, dest, src, nelems, pe);      \
   }                                                                            \
   void shmem_get##_size(void *dest, const void *src, size_t nelems, int pe) {  \
-    shmem_getmem(dest, src, nelems * ((_size) / 8), pe);                       \
+    shmem_ctx_get##_size(SHMEM_CTX_DEFAULT, dest, src, nelems, pe);            \
   }                                                                            \
   void shmem_get##_size##_nbi(void *dest, const void *src, size_t nelems,      \
                               int pe) {                                        \
-    shmem_getmem_nbi(dest, src, nelems * ((_size) / 8), pe);                   \
+    shmem_ctx_get##_size##_nbi(SHMEM_CTX_DEFAULT, dest, src, nelems, pe);      \
   }                                                                            \
   void shmem_iput##_size(void *dest, const void *src, ptrdiff_t tst,           \
                          ptrdiff_t sst, size_t nelems, int pe) {               \
-    strided(dest, src, (_size) / 8, tst, sst, nelems, pe, 1);                  \
+    shmem_ctx_iput##_size(SHMEM_CTX_DEFAULT, dest, src, tst, sst, nelems, pe); \
   }                                                                            \
   void shmem_iget##_size(void *dest, const void *src, ptrdiff_t tst,           \
                          ptrdiff_t sst, size_t nelems, int pe) {               \
-    strided(dest, src, (_size) / 8, tst, sst, nelems, pe, 0);                  \
+    shmem_ctx_iget##_size(SHMEM_CTX_DEFAULT, dest, src, tst, sst, nelems, pe); \
   }                                                                            \
   void shmem_put##_size##_signal(void *dest, const void *src, size_t nelems,   \
                                  uint64_t *sig_addr, uint64_t signal,          \
                                  int sig_op, int pe) {                         \
-    put_signal(dest, src, nelems * ((_size) / 8), sig_addr, signal, sig_op,    \
-               pe, 1);                                                         \
+    shmem_ctx_put##_size##_signal(SHMEM_CTX_DEFAULT, dest, src, nelems,        \
+                                  sig_addr, signal, sig_op, pe);               \
   }                                                                            \
   void shmem_put##_size##_signal_nbi(void *dest, const void *src,              \
                                      size_t nelems, uint64_t *sig_addr,        \
                                      uint64_t signal, int sig_op, int pe) {    \
-    put_signal(dest, src, nelems * ((_size) / 8), sig_addr, signal, sig_op,    \
-               pe, 0);                                                         \
+    shmem_ctx_put##_size##_signal_nbi(SHMEM_CTX_DEFAULT, dest, src, nelems,    \
+                                      sig_addr, signal, sig_op, pe);           \
   }
 OSC_SIZED_RMA(OSC_DEF_SIZED_RMA)
 

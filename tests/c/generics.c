@@ -121,6 +121,28 @@ void uses(shmem_team_t t, int pe) {
     (void)r;
     (void)ff;
   }
+  { /* the context forms: one more leading argument */
+    shmem_ctx_t ctx = SHMEM_CTX_DEFAULT;
+    long *lp = 0, lf = 0;
+    uint64_t *sig = 0;
+    shmem_put(ctx, lp, ls, 4, pe);
+    shmem_get(ctx, lp, ls, 4, pe);
+    shmem_put_nbi(ctx, lp, ls, 4, pe);
+    shmem_get_nbi(ctx, lp, ls, 4, pe);
+    shmem_p(ctx, lp, 3L, pe);
+    lf = shmem_g(ctx, ls, pe);
+    shmem_iput(ctx, lp, ls, 2, 1, 4, pe);
+    shmem_iget(ctx, lp, ls, 1, 2, 4, pe);
+    shmem_put_signal(ctx, lp, ls, 4, sig, 1, SHMEM_SIGNAL_SET, pe);
+    shmem_put_signal_nbi(ctx, lp, ls, 4, sig, 1, SHMEM_SIGNAL_ADD, pe);
+    lf += shmem_atomic_fetch(ctx, ls, pe);
+    lf += shmem_atomic_fetch_inc(ctx, lp, pe);
+    shmem_atomic_add(ctx, lp, 2L, pe);
+    lf += shmem_atomic_compare_swap(ctx, lp, 1L, 2L, pe);
+    shmem_atomic_fetch_add_nbi(ctx, &lf, lp, 2L, pe);
+    shmem_atomic_fetch_xor_nbi(ctx, &lf, lp, 2L, pe);
+    (void)lf;
+  }
 }
 
 int main(void) {
@@ -184,6 +206,11 @@ int main(void) {
   SAME(OSC_GEN_AMO_STD(psz, atomic_add), shmem_ulong_atomic_add);
   SAME(OSC_GEN_AMO_BITWISE(p32, atomic_fetch_xor), shmem_int32_atomic_fetch_xor);
   SAME(OSC_GEN_AMO_BITWISE(u64, atomic_or), shmem_ulong_atomic_or);
+  SAME(OSC_GEN_STD_P(shmem_ctx_, p64, put), shmem_ctx_long_put);
+  SAME(OSC_GEN_AMO_STD_P(shmem_ctx_, psz, atomic_fetch_inc),
+       shmem_ctx_ulong_atomic_fetch_inc);
+  SAME(OSC_GEN_AMO_EXT_P(shmem_ctx_, cf, atomic_fetch),
+       shmem_ctx_float_atomic_fetch);
   if (bad == 0) printf("generics ok\n");
   return bad != 0;
 }
