@@ -93,3 +93,72 @@ def test_init_thread_query_and_pointers(monkeypatch):
     finally:
         L.shmem_finalize()
     assert L.shmem_initialized() == 0 and L.shmem_finalized() == 1
+
+
+def test_heap_names_and_interoperability_without_a_device():
+    """Names get indices in first-lookup order (the table is process-wide,
+    so the test works relative to what earlier lookups took); only index 0
+    allocates; no interoperability is offered."""
+    L = osc.lib()
+    L.shmemx_name_to_index.restype = C.c_int
+    L.shmemx_name_to_index.argtypes = [C.c_char_p]
+    L.shmemx_index_to_name.restype = C.c_char_p
+    L.shmemx_index_to_name.argtypes = [C.c_int]
+    L.shmemx_malloc_by_index.restype = C.c_void_p
+    L.shmemx_malloc_by_index.argtypes = [C.c_int, C.c_size_t]
+    a = L.shmemx_name_to_index(b"osc-test-heap-a")
+    b = L.shmemx_name_to_index(b"osc-test-heap-b")
+    assert a >= 0 and b == a + 1
+    assert L.shmemx_name_to_index(b"osc-test-heap-a") == a
+    assert L.shmemx_index_to_name(b) == b"osc-test-heap-b"
+    assert L.shmemx_index_to_name(10 ** 6) is None
+    L.osc_clear_error()
+    assert L.shmemx_malloc_by_index(1, 64) is None      # one heap: index 0
+    assert L.osc_last_error_code() != 0
+    L.osc_clear_error()
+    assert all(L.shmemx_query_interoperability(p) == 0 for p in range(4))
+
+
+@pytest.mark.gpu
+def test_heap_index_zero_and_completion_tests(tmp_path):
+    """A C11 program: shmemx_malloc with index 0 and with the name that
+    owns index 0, the C11 generic forms, quiet/fence tests after a put."""
+    src = tmp_path / "xheap.c"
+    src.write_text(r'''
+#include <shmemx.h>
+#include <stdio.h>
+int main(void) {
+  long v[4] = {1, 2, 3, 4}, w[4] = {0, 0, 0, 0};
+  int bad = 0;
+  shmem_init();
+  const char *first = shmemx_index_to_name(0);
+  shmemx_heap_index_t i0 = first ? 0 : shmemx_name_to_index("main");
+  long *a = (long *)shmemx_malloc(i0, 4 * sizeof(long));
+  long *b = (long *)shmemx_malloc(shmemx_index_to_name(0), 4 * sizeof(long));
+  long *c = (long *)shmemx_calloc_by_index(0, 4, sizeof(long));
+  if (!a || !b || !c) bad |= 1;
+  shmem_putmem(a, v, sizeof v, 1);
+  if (!shmemx_fence_test()) bad |= 2;
+  shmem_quiet();
+  if (!shmemx_quiet_test()) bad |= 4;
+  shmem_getmem(w, a, sizeof w, 1);
+  if (w[3] != 4) bad |= 8;
+  if (shmemx_lookup_remote_addr(a, 1) != shmem_ptr(a, 1)) bad |= 16;
+  if (shmemx_malloc_by_index(1, 8) != NULL) bad |= 32;
+  shmemx_free(i0, c);
+  shmemx_free(shmemx_index_to_name(0), b);
+  shmemx_free_by_index(0, a);
+  shmem_finalize();
+  printf("bad=%d\n", bad);
+  return bad;
+}
+''')
+    exe = str(tmp_path / "xheap")
+    r = subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", INC,
+                        str(src), "-o", exe, "-L", LIBDIR, "-losc",
+                        "-Wl,-rpath," + LIBDIR], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    env = dict(os.environ, OSC_NPES="2", SHMEM_SYMMETRIC_SIZE="4M")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120,
+                       env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
