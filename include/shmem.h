@@ -99,6 +99,12 @@ void shmem_quiet(void);
 void shmem_fence(void);
 void shmem_info_get_version(int *major, int *minor);
 void shmem_info_get_name(char *name);
+/* proposed additions the reference also has (api.h:318-331): the spec
+ * version as major * 100 + minor; this library's own version 1.0.0, as a
+ * number major * 10000 + minor * 100 + patch */
+void shmem_info_get_version_number(int *version);
+void shmem_info_get_vendor_version(int *major, int *minor, int *patch);
+void shmem_info_get_vendor_version_number(int *version);
 void shmem_pcontrol(const int level, ...); /* profiling control: no effect */
 
 /* ---- symmetric heap (reference src/api/shmalloc.c:47-195) --------------- */
@@ -1147,6 +1153,14 @@ void shmem_sync_all(void);
 #define shmem_atomic_fetch_xor_nbi(...) \
   OSC_SEL5(__VA_ARGS__, OSC_GC_atomic_fetch_xor_nbi,                           \
       OSC_GP_atomic_fetch_xor_nbi, 0)(__VA_ARGS__)
+#define OSC_GP_atomic_xor_nbi(fetch, dest, value, pe)                          \
+  OSC_GEN_AMO_BITWISE_P(shmem_, dest, atomic_xor_nbi)(fetch, dest, value, pe)
+#define OSC_GC_atomic_xor_nbi(ctx, fetch, dest, value, pe)                     \
+  OSC_GEN_AMO_BITWISE_P(shmem_ctx_, dest, atomic_xor_nbi)(ctx, fetch, dest,    \
+                                                           value, pe)
+#define shmem_atomic_xor_nbi(...)                                              \
+  OSC_SEL5(__VA_ARGS__, OSC_GC_atomic_xor_nbi, OSC_GP_atomic_xor_nbi,          \
+           0)(__VA_ARGS__)
 #endif /* C11 */
 
 #ifdef __cplusplus

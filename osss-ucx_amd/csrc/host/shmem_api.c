@@ -158,6 +158,17 @@ void shmem_info_get_version(int *major, int *minor) {
   if (major) *major = 1;
   if (minor) *minor = 5;
 }
+void shmem_info_get_version_number(int *version) {
+  if (version) *version = 105;
+}
+void shmem_info_get_vendor_version(int *major, int *minor, int *patch) {
+  if (major) *major = 1;
+  if (minor) *minor = 0;
+  if (patch) *patch = 0;
+}
+void shmem_info_get_vendor_version_number(int *version) {
+  if (version) *version = 10000;
+}
 void shmem_info_get_name(char *name) {
   if (name) strcpy(name, "osc-mi355x (OpenSHMEM collectives on gfx950)");
 }

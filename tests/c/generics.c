@@ -141,6 +141,8 @@ void uses(shmem_team_t t, int pe) {
     lf += shmem_atomic_compare_swap(ctx, lp, 1L, 2L, pe);
     shmem_atomic_fetch_add_nbi(ctx, &lf, lp, 2L, pe);
     shmem_atomic_fetch_xor_nbi(ctx, &lf, lp, 2L, pe);
+    shmem_atomic_xor_nbi(&lf, lp, 2L, pe);
+    shmem_atomic_xor_nbi(ctx, (uint64_t *)&lf, (uint64_t *)lp, (uint64_t)2, pe);
     (void)lf;
   }
 }

@@ -149,3 +149,16 @@ def test_wtime_is_monotonic_seconds():
     a = lib.shmemx_wtime()
     b = lib.shmemx_wtime()
     assert 0 < a <= b < a + 1.0
+
+
+def test_version_queries():
+    lib = osc.lib()
+    v = C.c_int()
+    lib.shmem_info_get_version_number(C.byref(v))
+    assert v.value == 105
+    a, b, c = C.c_int(), C.c_int(), C.c_int()
+    lib.shmem_info_get_version(C.byref(a), C.byref(b))
+    assert (a.value, b.value) == (1, 5)
+    lib.shmem_info_get_vendor_version(C.byref(a), C.byref(b), C.byref(c))
+    lib.shmem_info_get_vendor_version_number(C.byref(v))
+    assert v.value == a.value * 10000 + b.value * 100 + c.value
