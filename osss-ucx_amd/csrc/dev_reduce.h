@@ -178,7 +178,12 @@ struct PolF32Acc {
   __device__ static float out(Acc a) { return (float)a; }
 };
 
-// FP64 SUM/PROD accumulated in double-double, rounded once.
+// FP64 SUM/PROD accumulated in twice the working precision, rounded once.
+// SUM is Sum2 of Ogita, Rump and Oishi ("Accurate sum and dot product",
+// SIAM J. Sci. Comput. 26(6), 2005): TwoSum keeps the running sum's exact
+// rounding error, which collects in a second accumulator added once at the
+// end; the result is as accurate as a sum in double-double rounded to
+// double.  PROD multiplies in double-double.
 template <int OP>
 struct PolF64Acc {
   typedef dd Acc;
@@ -192,10 +197,17 @@ struct PolF64Acc {
   static constexpr int OPC = OP;
   __device__ static Acc lift(double x) { return {x, 0.0}; }
   __device__ static Acc seq(Acc a, double x) {
-    return OP == OSC_OP_SUM ? dd_add_d(a, x) : dd_mul_d(a, x);
+    if constexpr (OP == OSC_OP_SUM) {
+      const dd s = two_sum(a.hi, x);
+      return {s.hi, a.lo + s.lo};
+    } else {
+      return dd_mul_d(a, x);
+    }
   }
   __device__ static Acc comb(Acc a, Acc b) { return dd_add(a, b); }
-  __device__ static double out(Acc a) { return a.hi; }
+  __device__ static double out(Acc a) {
+    return OP == OSC_OP_SUM ? a.hi + a.lo : a.hi;
+  }
 };
 
 // complex float SUM/PROD in FP64 components.
