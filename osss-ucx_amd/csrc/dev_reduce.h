@@ -188,7 +188,7 @@ template <int OP>
 struct PolF64Acc {
   typedef dd Acc;
   static constexpr bool TREE = false;
-  // The double-double sum keeps 2 doubles per element: at 8x2 it takes 172
+  // The accurate sum keeps 2 doubles per element: at 8x2 it takes 180
   // VGPRs (2 waves per SIMD).  8x1 held to 4 waves per SIMD measured 3%
   // faster at 64 PEs x 4M.
   static constexpr int SHAPE = OP == OSC_OP_SUM ? 1 : 0;  // default shape
