@@ -33,6 +33,12 @@
 // byte is read once and every destination byte written once, so HBM
 // traffic = 2*P*n*s.  (UPT, INFL) is a per-call "shape" chosen from a
 // tuning table.
+//
+// Small integer calls (16+ members, at most 2048 elements) take a second
+// kernel instead, k_reduce_lanes: a group of lanes shares one element,
+// reads its members in parallel and combines them with a cross-lane
+// butterfly (DPP, ds_swizzle, ds_bpermute), so the call costs one HBM round
+// trip where the slice walk takes one per 8 members.
 #include <type_traits>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
