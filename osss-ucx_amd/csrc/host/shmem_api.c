@@ -352,7 +352,11 @@ int osc_ctx_target(void *ctx, int pe, void **stream) {
   const osc_team *t;
   if (c == NULL) return -1;
   *stream = c == &g_ctx_default ? osc_g.stream : c->stream;
-  if (c->team == NULL) return pe;
+  if (c->team == NULL) {
+    if (pe < 0 || pe >= osc_g.npes)
+      return osc_set_error(OSC_ERR_ARG, "PE %d outside 0..%d", pe, osc_g.npes - 1), -1;
+    return pe;
+  }
   t = osc_team_get(c->team);
   if (t == NULL) return osc_set_error(OSC_ERR_TEAM, "the context's team is gone"), -1;
   if (pe < 0 || pe >= t->npes)

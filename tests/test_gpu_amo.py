@@ -223,6 +223,36 @@ def test_errors_leave_words_alone(heap):
     assert F["fetch"](w, 0) == 5
 
 
+def test_negative_pe_is_an_error(heap):
+    """A negative PE number records an error and moves nothing, for the
+    atomics and for every RMA form on the default context."""
+    h = heap
+    F = amo_fns("long", C.c_long)
+    L = osc.lib()
+    w, src, sig = h.malloc(64), h.malloc(64), h.malloc(8)
+    F["set"](w, 5, 1)
+    h.put(src, np.full(8, 9, np.int64), 0)
+    calls = [
+        lambda: F["finc"](w, -1),
+        lambda: F["add"](w, 1, -1),
+        lambda: F["fetch_nbi"](w, src, -1),
+        lambda: fn("shmem_putmem", None, [VP, VP, C.c_size_t, I])(w, src, 8, -1),
+        lambda: fn("shmem_getmem", None, [VP, VP, C.c_size_t, I])(src, w, 8, -1),
+        lambda: fn("shmem_long_p", None, [VP, C.c_long, I])(w, 3, -1),
+        lambda: fn("shmem_putmem_signal", None,
+                   [VP, VP, C.c_size_t, VP, C.c_uint64, I, I])(w, src, 8, sig, 1, 0, -1),
+        lambda: fn("shmem_long_iput", None,
+                   [VP, VP, C.c_ssize_t, C.c_ssize_t, C.c_size_t, I])(w, src, 1, 1, 2, -1),
+    ]
+    for k, call in enumerate(calls):
+        L.osc_clear_error()
+        call()
+        assert L.osc_last_error_code() != 0, k
+    L.osc_clear_error()
+    assert F["fetch"](w, 1) == 5
+    assert h.get(src, 8, np.int64, 0).tolist() == [9] * 8
+
+
 PE_FN = C.CFUNCTYPE(C.c_int, C.c_int, C.c_void_p)
 
 
