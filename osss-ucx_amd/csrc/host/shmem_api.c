@@ -288,6 +288,13 @@ int shmem_addr_accessible(const void *addr, int pe) {
   return osc_pe_ptr(addr, pe) != NULL;
 }
 
+/* *out = nelems * es; OSC_ERR_ARG (recorded) if that overflows */
+static int sized_bytes(size_t nelems, size_t es, size_t *out) {
+  if (nelems > ((size_t)-1) / es) return osc_set_error(OSC_ERR_ARG, "size overflow");
+  *out = nelems * es;
+  return OSC_OK;
+}
+
 /* ---- RMA between emulated PEs ------------------------------------------------ */
 /* The local side may be a symmetric (device) address or host memory; the
  * remote side is the symmetric object on PE pe (reference putget.h:403-600:
@@ -473,10 +480,15 @@ static void xfer(void *st, void *dst, const void *src, size_t nbytes, int blocki
   if (rc != OSC_OK) osc_hip_error(rc, "RMA copy");
 }
 
+/* nelems elements of es bytes (a byte count overflowing size_t is an
+ * error, not a shorter copy) */
 static void rma_mem(shmem_ctx_t ctx, void *dest, const void *source,
-                    size_t nbytes, int pe, int is_put, int blocking) {
+                    size_t nelems, size_t es, int pe, int is_put, int blocking) {
   void *st = NULL;
-  const int g = osc_ctx_target(ctx, pe, &st);
+  size_t nbytes = 0;
+  int g;
+  if (sized_bytes(nelems, es, &nbytes) != OSC_OK) return;
+  g = osc_ctx_target(ctx, pe, &st);
   if (g < 0) return;
   if (is_put) {
     void *d = remote(dest, 0, nbytes, g);
@@ -489,31 +501,31 @@ static void rma_mem(shmem_ctx_t ctx, void *dest, const void *source,
 
 void shmem_ctx_putmem(shmem_ctx_t ctx, void *dest, const void *source,
                       size_t nbytes, int pe) {
-  rma_mem(ctx, dest, source, nbytes, pe, 1, 1);
+  rma_mem(ctx, dest, source, nbytes, 1, pe, 1, 1);
 }
 void shmem_ctx_putmem_nbi(shmem_ctx_t ctx, void *dest, const void *source,
                           size_t nbytes, int pe) {
-  rma_mem(ctx, dest, source, nbytes, pe, 1, 0);
+  rma_mem(ctx, dest, source, nbytes, 1, pe, 1, 0);
 }
 void shmem_ctx_getmem(shmem_ctx_t ctx, void *dest, const void *source,
                       size_t nbytes, int pe) {
-  rma_mem(ctx, dest, source, nbytes, pe, 0, 1);
+  rma_mem(ctx, dest, source, nbytes, 1, pe, 0, 1);
 }
 void shmem_ctx_getmem_nbi(shmem_ctx_t ctx, void *dest, const void *source,
                           size_t nbytes, int pe) {
-  rma_mem(ctx, dest, source, nbytes, pe, 0, 0);
+  rma_mem(ctx, dest, source, nbytes, 1, pe, 0, 0);
 }
 void shmem_putmem(void *dest, const void *source, size_t nbytes, int pe) {
-  rma_mem(SHMEM_CTX_DEFAULT, dest, source, nbytes, pe, 1, 1);
+  rma_mem(SHMEM_CTX_DEFAULT, dest, source, nbytes, 1, pe, 1, 1);
 }
 void shmem_putmem_nbi(void *dest, const void *source, size_t nbytes, int pe) {
-  rma_mem(SHMEM_CTX_DEFAULT, dest, source, nbytes, pe, 1, 0);
+  rma_mem(SHMEM_CTX_DEFAULT, dest, source, nbytes, 1, pe, 1, 0);
 }
 void shmem_getmem(void *dest, const void *source, size_t nbytes, int pe) {
-  rma_mem(SHMEM_CTX_DEFAULT, dest, source, nbytes, pe, 0, 1);
+  rma_mem(SHMEM_CTX_DEFAULT, dest, source, nbytes, 1, pe, 0, 1);
 }
 void shmem_getmem_nbi(void *dest, const void *source, size_t nbytes, int pe) {
-  rma_mem(SHMEM_CTX_DEFAULT, dest, source, nbytes, pe, 0, 0);
+  rma_mem(SHMEM_CTX_DEFAULT, dest, source, nbytes, 1, pe, 0, 0);
 }
 
 /* Strided put (is_put) or get of n elements of es bytes; strides in
@@ -600,12 +612,14 @@ static void strided(shmem_ctx_t ctx, void *dest, const void *src, size_t es,
 /* put then signal update on the same stream, so the data is in place
  * before the signal changes (put_signal.h:28-50, SHMEM_SIGNAL_SET / ADD) */
 static void put_signal(shmem_ctx_t ctx, void *dest, const void *src,
-                       size_t nbytes, uint64_t *sig_addr, uint64_t signal,
-                       int sig_op, int pe, int blocking) {
+                       size_t nelems, size_t es, uint64_t *sig_addr,
+                       uint64_t signal, int sig_op, int pe, int blocking) {
   void *st = NULL;
   char *d;
   uint64_t *s;
+  size_t nbytes = 0;
   int rc;
+  if (sized_bytes(nelems, es, &nbytes) != OSC_OK) return;
   pe = osc_ctx_target(ctx, pe, &st);
   if (pe < 0) return;
   d = remote(dest, 0, nbytes, pe);
@@ -625,23 +639,25 @@ static void put_signal(shmem_ctx_t ctx, void *dest, const void *src,
 void shmem_putmem_signal(void *dest, const void *source, size_t nbytes,
                          uint64_t *sig_addr, uint64_t signal, int sig_op,
                          int pe) {
-  put_signal(SHMEM_CTX_DEFAULT, dest, source, nbytes, sig_addr, signal, sig_op, pe, 1);
+  put_signal(SHMEM_CTX_DEFAULT, dest, source, nbytes, 1, sig_addr, signal,
+             sig_op, pe, 1);
 }
 void shmem_putmem_signal_nbi(void *dest, const void *source, size_t nbytes,
                              uint64_t *sig_addr, uint64_t signal, int sig_op,
                              int pe) {
-  put_signal(SHMEM_CTX_DEFAULT, dest, source, nbytes, sig_addr, signal, sig_op, pe, 0);
+  put_signal(SHMEM_CTX_DEFAULT, dest, source, nbytes, 1, sig_addr, signal,
+             sig_op, pe, 0);
 }
 void shmem_ctx_putmem_signal(shmem_ctx_t ctx, void *dest, const void *source,
                              size_t nbytes, uint64_t *sig_addr,
                              uint64_t signal, int sig_op, int pe) {
-  put_signal(ctx, dest, source, nbytes, sig_addr, signal, sig_op, pe, 1);
+  put_signal(ctx, dest, source, nbytes, 1, sig_addr, signal, sig_op, pe, 1);
 }
 void shmem_ctx_putmem_signal_nbi(shmem_ctx_t ctx, void *dest,
                                  const void *source, size_t nbytes,
                                  uint64_t *sig_addr, uint64_t signal,
                                  int sig_op, int pe) {
-  put_signal(ctx, dest, source, nbytes, sig_addr, signal, sig_op, pe, 0);
+  put_signal(ctx, dest, source, nbytes, 1, sig_addr, signal, sig_op, pe, 0);
 }
 
 static int read_signal(const uint64_t *sig_addr, uint64_t *v) {
@@ -696,30 +712,30 @@ uint64_t shmem_signal_wait_until(uint64_t *sig_addr, int cmp,
 #define OSC_DEF_RMA(_type, _typename)                                          \
   void shmem_ctx_##_typename##_put(shmem_ctx_t ctx, _type *dest,               \
                                    const _type *src, size_t nelems, int pe) {  \
-    rma_mem(ctx, dest, src, nelems * sizeof(_type), pe, 1, 1);                 \
+    rma_mem(ctx, dest, src, nelems, sizeof(_type), pe, 1, 1);                  \
   }                                                                            \
   void shmem_ctx_##_typename##_put_nbi(shmem_ctx_t ctx, _type *dest,           \
                                        const _type *src, size_t nelems,        \
                                        int pe) {                               \
-    rma_mem(ctx, dest, src, nelems * sizeof(_type), pe, 1, 0);                 \
+    rma_mem(ctx, dest, src, nelems, sizeof(_type), pe, 1, 0);                  \
   }                                                                            \
   void shmem_ctx_##_typename##_get(shmem_ctx_t ctx, _type *dest,               \
                                    const _type *src, size_t nelems, int pe) {  \
-    rma_mem(ctx, dest, src, nelems * sizeof(_type), pe, 0, 1);                 \
+    rma_mem(ctx, dest, src, nelems, sizeof(_type), pe, 0, 1);                  \
   }                                                                            \
   void shmem_ctx_##_typename##_get_nbi(shmem_ctx_t ctx, _type *dest,           \
                                        const _type *src, size_t nelems,        \
                                        int pe) {                               \
-    rma_mem(ctx, dest, src, nelems * sizeof(_type), pe, 0, 0);                 \
+    rma_mem(ctx, dest, src, nelems, sizeof(_type), pe, 0, 0);                  \
   }                                                                            \
   void shmem_ctx_##_typename##_p(shmem_ctx_t ctx, _type *dest, _type value,    \
                                  int pe) {                                     \
-    rma_mem(ctx, dest, &value, sizeof(_type), pe, 1, 1);                       \
+    rma_mem(ctx, dest, &value, 1, sizeof(_type), pe, 1, 1);                    \
   }                                                                            \
   _type shmem_ctx_##_typename##_g(shmem_ctx_t ctx, const _type *src, int pe) { \
     _type v;                                                                   \
     memset(&v, 0, sizeof(v));                                                  \
-    rma_mem(ctx, &v, src, sizeof(_type), pe, 0, 1);                            \
+    rma_mem(ctx, &v, src, 1, sizeof(_type), pe, 0, 1);                         \
     return v;                                                                  \
   }                                                                            \
   void shmem_ctx_##_typename##_iput(shmem_ctx_t ctx, _type *dest,              \
@@ -735,13 +751,13 @@ uint64_t shmem_signal_wait_until(uint64_t *sig_addr, int cmp,
   void shmem_ctx_##_typename##_put_signal(                                     \
       shmem_ctx_t ctx, _type *dest, const _type *src, size_t nelems,           \
       uint64_t *sig_addr, uint64_t signal, int sig_op, int pe) {               \
-    put_signal(ctx, dest, src, nelems * sizeof(_type), sig_addr, signal,       \
+    put_signal(ctx, dest, src, nelems, sizeof(_type), sig_addr, signal,        \
                sig_op, pe, 1);                                                 \
   }                                                                            \
   void shmem_ctx_##_typename##_put_signal_nbi(                                 \
       shmem_ctx_t ctx, _type *dest, const _type *src, size_t nelems,           \
       uint64_t *sig_addr, uint64_t signal, int sig_op, int pe) {               \
-    put_signal(ctx, dest, src, nelems * sizeof(_type), sig_addr, signal,       \
+    put_signal(ctx, dest, src, nelems, sizeof(_type), sig_addr, signal,        \
                sig_op, pe, 0);                                                 \
   }                                                                            \
   void shmem_##_typename##_put(_type *dest, const _type *src, size_t nelems,   \
@@ -779,14 +795,14 @@ uint64_t shmem_signal_wait_until(uint64_t *sig_addr, int cmp,
   void shmem_##_typename##_put_signal(_type *dest, const _type *src,           \
                                       size_t nelems, uint64_t *sig_addr,       \
                                       uint64_t signal, int sig_op, int pe) {   \
-    put_signal(SHMEM_CTX_DEFAULT, dest, src, nelems * sizeof(_type), sig_addr, \
+    put_signal(SHMEM_CTX_DEFAULT, dest, src, nelems, sizeof(_type), sig_addr,  \
                signal, sig_op, pe, 1);                                         \
   }                                                                            \
   void shmem_##_typename##_put_signal_nbi(_type *dest, const _type *src,       \
                                           size_t nelems, uint64_t *sig_addr,   \
                                           uint64_t signal, int sig_op,         \
                                           int pe) {                            \
-    put_signal(SHMEM_CTX_DEFAULT, dest, src, nelems * sizeof(_type), sig_addr, \
+    put_signal(SHMEM_CTX_DEFAULT, dest, src, nelems, sizeof(_type), sig_addr,  \
                signal, sig_op, pe, 0);                                         \
   }
 OSC_RMA_TYPES(OSC_DEF_RMA)
@@ -794,19 +810,19 @@ OSC_RMA_TYPES(OSC_DEF_RMA)
 #define OSC_DEF_SIZED_RMA(_size)                                               \
   void shmem_ctx_put##_size(shmem_ctx_t ctx, void *dest, const void *src,      \
                             size_t nelems, int pe) {                           \
-    rma_mem(ctx, dest, src, nelems * ((_size) / 8), pe, 1, 1);                 \
+    rma_mem(ctx, dest, src, nelems, (_size) / 8, pe, 1, 1);                    \
   }                                                                            \
   void shmem_ctx_put##_size##_nbi(shmem_ctx_t ctx, void *dest,                 \
                                   const void *src, size_t nelems, int pe) {    \
-    rma_mem(ctx, dest, src, nelems * ((_size) / 8), pe, 1, 0);                 \
+    rma_mem(ctx, dest, src, nelems, (_size) / 8, pe, 1, 0);                    \
   }                                                                            \
   void shmem_ctx_get##_size(shmem_ctx_t ctx, void *dest, const void *src,      \
                             size_t nelems, int pe) {                           \
-    rma_mem(ctx, dest, src, nelems * ((_size) / 8), pe, 0, 1);                 \
+    rma_mem(ctx, dest, src, nelems, (_size) / 8, pe, 0, 1);                    \
   }                                                                            \
   void shmem_ctx_get##_size##_nbi(shmem_ctx_t ctx, void *dest,                 \
                                   const void *src, size_t nelems, int pe) {    \
-    rma_mem(ctx, dest, src, nelems * ((_size) / 8), pe, 0, 0);                 \
+    rma_mem(ctx, dest, src, nelems, (_size) / 8, pe, 0, 0);                    \
   }                                                                            \
   void shmem_ctx_iput##_size(shmem_ctx_t ctx, void *dest, const void *src,     \
                              ptrdiff_t tst, ptrdiff_t sst, size_t nelems,      \
@@ -822,14 +838,14 @@ OSC_RMA_TYPES(OSC_DEF_RMA)
                                      const void *src, size_t nelems,           \
                                      uint64_t *sig_addr, uint64_t signal,      \
                                      int sig_op, int pe) {                     \
-    put_signal(ctx, dest, src, nelems * ((_size) / 8), sig_addr, signal,       \
+    put_signal(ctx, dest, src, nelems, (_size) / 8, sig_addr, signal,          \
                sig_op, pe, 1);                                                 \
   }                                                                            \
   void shmem_ctx_put##_size##_signal_nbi(shmem_ctx_t ctx, void *dest,          \
                                          const void *src, size_t nelems,       \
                                          uint64_t *sig_addr, uint64_t signal,  \
                                          int sig_op, int pe) {                 \
-    put_signal(ctx, dest, src, nelems * ((_size) / 8), sig_addr, signal,       \
+    put_signal(ctx, dest, src, nelems, (_size) / 8, sig_addr, signal,          \
                sig_op, pe, 0);                                                 \
   }                                                                            \
   void shmem_put##_size(void *dest, const void *src, size_t nelems, int pe) {  \
@@ -933,17 +949,22 @@ OSC_ARITH_TYPES(OSC_DEF_PROD)
 #define OSC_DEF_MOVE(_type, _typename)                                         \
   int shmem_##_typename##_fcollect(shmem_team_t team, _type *dest,             \
                                    const _type *source, size_t nelems) {       \
-    return do_fcollect(team, dest, source, nelems * sizeof(_type));            \
+    size_t nb = 0;                                                             \
+    if (sized_bytes(nelems, sizeof(_type), &nb) != OSC_OK) return OSC_ERR_ARG; \
+    return do_fcollect(team, dest, source, nb);                                \
   }                                                                            \
   int shmem_##_typename##_alltoall(shmem_team_t team, _type *dest,             \
                                    const _type *source, size_t nelems) {       \
-    return do_alltoall(team, dest, source, nelems * sizeof(_type));            \
+    size_t nb = 0;                                                             \
+    if (sized_bytes(nelems, sizeof(_type), &nb) != OSC_OK) return OSC_ERR_ARG; \
+    return do_alltoall(team, dest, source, nb);                                \
   }                                                                            \
   int shmem_##_typename##_broadcast(shmem_team_t team, _type *dest,            \
                                     const _type *source, size_t nelems,        \
                                     int PE_root) {                             \
-    return do_broadcast(team, dest, source, nelems * sizeof(_type), PE_root,   \
-                        0);                                                    \
+    size_t nb = 0;                                                             \
+    if (sized_bytes(nelems, sizeof(_type), &nb) != OSC_OK) return OSC_ERR_ARG; \
+    return do_broadcast(team, dest, source, nb, PE_root, 0);                   \
   }
 OSC_RMA_TYPES(OSC_DEF_MOVE)
 
@@ -1056,13 +1077,6 @@ OSC_TO_ALL_MINMAX_TYPES(OSC_DEF_MIN_TO_ALL)
 OSC_TO_ALL_MINMAX_TYPES(OSC_DEF_MAX_TO_ALL)
 OSC_TO_ALL_ARITH_TYPES(OSC_DEF_SUM_TO_ALL)
 OSC_TO_ALL_ARITH_TYPES(OSC_DEF_PROD_TO_ALL)
-
-/* *out = nelems * es; OSC_ERR_ARG (recorded) if that overflows */
-static int sized_bytes(size_t nelems, size_t es, size_t *out) {
-  if (nelems > ((size_t)-1) / es) return osc_set_error(OSC_ERR_ARG, "size overflow");
-  *out = nelems * es;
-  return OSC_OK;
-}
 
 #define OSC_DEF_SIZED_COLL(_size)                                              \
   void shmem_fcollect##_size(void *target, const void *source, size_t nelems, \

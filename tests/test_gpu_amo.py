@@ -253,6 +253,32 @@ def test_negative_pe_is_an_error(heap):
     assert h.get(src, 8, np.int64, 0).tolist() == [9] * 8
 
 
+def test_size_overflow_is_an_error(heap):
+    """An element count whose byte count overflows size_t (2^61 + 1 longs
+    would wrap to 8 bytes) records an error and moves nothing."""
+    h = heap
+    L = osc.lib()
+    w, src, sig = h.malloc(64), h.malloc(64), h.malloc(8)
+    for pe in range(h.npes):
+        h.put(w, np.full(8, 4, np.int64), pe)
+    h.put(src, np.full(8, 9, np.int64), 0)
+    big = (1 << 61) + 1
+    calls = [
+        lambda: osc.typed("shmem_long_put")(w, src, big, 1),
+        lambda: osc.typed("shmem_long_get")(w, src, big, 1),
+        lambda: fn("shmem_put64_nbi", None, [VP, VP, C.c_size_t, I])(w, src, big, 1),
+        lambda: osc.typed("shmem_long_put_signal")(w, src, big, sig, 1, 0, 1),
+        lambda: osc.typed("shmem_long_fcollect")(h.world, w, src, big),
+    ]
+    for k, call in enumerate(calls):
+        L.osc_clear_error()
+        call()
+        assert L.osc_last_error_code() != 0, k
+    L.osc_clear_error()
+    for pe in range(h.npes):
+        assert h.get(w, 8, np.int64, pe).tolist() == [4] * 8, pe
+
+
 PE_FN = C.CFUNCTYPE(C.c_int, C.c_int, C.c_void_p)
 
 
