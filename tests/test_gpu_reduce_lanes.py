@@ -126,3 +126,20 @@ def test_lanes_element_offset(heap, dtype, op):
     got = run_reduce(heap, op, dtype, src, team=team, misalign=1)
     heap.L.shmem_team_destroy(team)
     assert bit_equal(got, O.rec_dbl(op, dtype, src), dtype)
+
+
+def test_lanes_large_team():
+    """P = 1000 (not a power of two; at G = 64 a lane has 15 or 16 members:
+    its first load, then one chunk of up to 16) and P = 1100 (lanes 0..11
+    have 18 members, so a second, partial chunk); n = 4096 takes the slice
+    walk for comparison."""
+    rng = np.random.default_rng(59)
+    for P in (1000, 1100):
+        with osc.Heap(npes=P, heap_bytes=1 << 20) as h:
+            for dtype, op in (("i64", "sum"), ("u8", "xor"), ("i32", "min"),
+                              ("u16", "prod")):
+                for n in (1, 777, 2048, 4096):
+                    src = inputs(rng, op, dtype, P, n)
+                    got = run_reduce(h, op, dtype, src)
+                    want = O.rec_dbl(op, dtype, src)
+                    assert bit_equal(got, want, dtype), (P, dtype, op, n)
