@@ -353,6 +353,13 @@ int osc_serve_run(osc_request *reqs, int nreq, const osc_serve_cfg *cfg,
         osc_call tmp = *c;
         tmp.dest_off = cfg->dest_base;
         vrc = osc_call_validate(&tmp, next);
+        /* a source inside the destination arena could be overwritten by
+         * another request in flight */
+        if (vrc == OSC_OK && osc_call_src_bytes(c) > 0 &&
+            c->src_off < cfg->dest_base + cfg->dest_bytes &&
+            cfg->dest_base < c->src_off + osc_call_src_bytes(c))
+          vrc = osc_set_error(OSC_ERR_ARG, "request %d: source inside the destination arena",
+                              next);
         /* the arena holds dest_bytes / page whole pages */
         if (vrc == OSC_OK && (db + page - 1) / page > cfg->dest_bytes / page)
           vrc = OSC_ERR_BUDGET;

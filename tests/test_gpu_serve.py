@@ -203,3 +203,16 @@ def test_request_needing_more_whole_pages_than_the_arena_is_rejected(sw):
     assert -(-16 * c.count // cfg.page_bytes) > cfg.dest_bytes // cfg.page_bytes
     out, st = osc.serve(reqs, cfg)
     assert st.rejected == 1 and out[0].status == -9
+
+
+def test_request_reading_from_the_destination_arena_is_rejected(sw):
+    """The server hands out the arena's pages to requests in flight, so a
+    request whose source lies in the arena could read another request's
+    partial results: it is rejected at arrival; the others complete."""
+    reqs = requests(sw, 10, 12, 1e5)
+    bad = reqs[5].call
+    bad.src_off = sw.arena + ARENA - 256   # its last 256 bytes, or more
+    out, st = osc.serve(reqs, sw.cfg(osc.SERVE_BATCHED))
+    assert st.rejected == 1 and out[5].status == -1
+    assert st.completed == len(reqs) - 1
+    assert all(r.status == 0 for k, r in enumerate(out) if k != 5)
