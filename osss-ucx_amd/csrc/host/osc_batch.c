@@ -53,15 +53,23 @@ int osc_batch_plan_world(osc_call *calls, int ncalls, int world_npes,
   if (top == NULL) return osc_set_error(OSC_ERR_NOMEM, "planner");
   for (i = 0; i < ncalls; ++i) {
     osc_call *c = &calls[i];
-    const uint64_t need = align_up(osc_call_dest_bytes(c), 256);
-    const int lo = c->team_start;
-    const int hi = c->team_start + (c->team_npes - 1) * c->team_stride;
-    uint64_t off = 0;
-    int p;
-    if (c->team_npes < 1 || c->team_stride < 1 || lo < 0 || hi >= world_npes) {
+    uint64_t need, off = 0;
+    int lo, hi, p;
+    if (!osc_call_team_fits(c, world_npes)) {
       free(top);
       return osc_set_error(OSC_ERR_TEAM, "call %d: team outside the world", i);
     }
+    lo = c->team_start;
+    hi = c->team_start + (c->team_npes - 1) * c->team_stride;
+    /* a destination holds at least count bytes: this bound keeps the
+     * byte count below from wrapping */
+    if (c->count > dest_bytes_per_pe) {
+      free(top);
+      return osc_set_error(OSC_ERR_BUDGET, "call %d: count %llu > region %llu", i,
+                           (unsigned long long)c->count,
+                           (unsigned long long)dest_bytes_per_pe);
+    }
+    need = align_up(osc_call_dest_bytes(c), 256);
     if (need > dest_bytes_per_pe) {
       free(top);
       return osc_set_error(OSC_ERR_BUDGET, "call %d needs %llu bytes per PE > region %llu",

@@ -59,12 +59,23 @@ static int call_class(const osc_call *c) {
   return 4;
 }
 
+int osc_call_team_fits(const osc_call *c, int world_npes) {
+  /* in 64 bits: a large stride must not wrap the last member's PE */
+  const long long hi =
+      (long long)c->team_start + (long long)(c->team_npes - 1) * (long long)c->team_stride;
+  return c->team_npes >= 1 && c->team_stride >= 1 && c->team_start >= 0 && hi < world_npes;
+}
+
 int osc_call_validate(const osc_call *c, int i) {
-  const int hi = c->team_start + (c->team_npes - 1) * c->team_stride;
   if (c->kind < OSC_CALL_REDUCE || c->kind > OSC_CALL_BROADCAST)
     return osc_set_error(OSC_ERR_ARG, "call %d: kind %d", i, c->kind);
-  if (c->team_npes < 1 || c->team_stride < 1 || c->team_start < 0 || hi >= osc_g.npes)
+  if (!osc_call_team_fits(c, osc_g.npes))
     return osc_set_error(OSC_ERR_TEAM, "call %d: team outside the world", i);
+  /* every kind's destination holds at least count bytes; bounding count
+   * first keeps the byte counts below (count x size, count x P) exact */
+  if (c->count > osc_g.heap_bytes)
+    return osc_set_error(OSC_ERR_PTR, "call %d: count %llu larger than the heap", i,
+                         (unsigned long long)c->count);
   if (c->src_off > osc_g.heap_bytes || osc_call_src_bytes(c) > osc_g.heap_bytes - c->src_off ||
       c->dest_off > osc_g.heap_bytes || osc_call_dest_bytes(c) > osc_g.heap_bytes - c->dest_off)
     return osc_set_error(OSC_ERR_PTR, "call %d: buffers outside the heap", i);

@@ -34,6 +34,9 @@ typedef struct {
 uint64_t osc_call_dest_bytes(const osc_call *c);
 uint64_t osc_call_src_bytes(const osc_call *c);
 double osc_call_alg_bytes(const osc_call *c);
+/* 1 if the call's team (start, stride, size) lies inside a world of
+ * world_npes PEs */
+int osc_call_team_fits(const osc_call *c, int world_npes);
 int osc_call_validate(const osc_call *c, int i);
 void osc_call_to_device(const osc_call *c, oscd_bcall *d);
 /* upper bound of the tiles osc_plan_build makes for one call */

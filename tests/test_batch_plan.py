@@ -92,6 +92,30 @@ def test_team_outside_world_is_rejected():
         osc.plan([c], 0, 1 << 20, world_npes=10)
 
 
+def test_stride_that_wraps_32_bits_is_rejected():
+    """start 0, stride 2^30, 3 members: the last member's PE, 2^31, does
+    not fit an int; it is outside the world, not a negative PE."""
+    c = osc.Call()
+    c.kind = 1
+    c.team_start, c.team_stride, c.team_npes = 0, 1 << 30, 3
+    c.count = 16
+    with pytest.raises(osc.OscError) as e:
+        osc.plan([c], 0, 1 << 20, world_npes=10)
+    assert e.value.code == -3  # OSC_ERR_TEAM
+
+
+def test_count_whose_byte_count_wraps_is_rejected():
+    """fcollect of 2^62 bytes per PE on 4 PEs: 4 x 2^62 wraps to 0 in 64
+    bits; the call is rejected, not planned as an empty one."""
+    c = osc.Call()
+    c.kind = 1
+    c.team_start, c.team_stride, c.team_npes = 0, 1, 4
+    c.count = 1 << 62
+    with pytest.raises(osc.OscError) as e:
+        osc.plan([c], 0, 1 << 20, world_npes=4)
+    assert e.value.code == -9  # OSC_ERR_BUDGET
+
+
 def test_tuning_table_roundtrip(tmp_path):
     import ctypes as C
     L = osc.lib()

@@ -211,3 +211,21 @@ def test_batch_rejects_bad_calls(world):
     c.src_off += 4  # misaligned for the batched reduce
     with pytest.raises(osc.OscError):
         osc.Batch([c])
+
+
+def test_batch_rejects_wrapping_team_and_count(world):
+    """A stride whose last member wraps 32 bits and a count whose byte
+    count wraps 64 bits are errors in batch creation and in a request run
+    alone; nothing is launched for them."""
+    c = osc.Call()
+    c.kind, c.op, c.dtype = osc.CALL_FCOLLECT, 0, 0
+    c.team_start, c.team_stride, c.team_npes = 0, 1 << 30, 3
+    c.count, c.src_off, c.dest_off = 16, world.type_off["f32"], world.dest_off
+    for run in (lambda: osc.Batch([c]), lambda: osc.request_alone(c, world.dest_off)):
+        with pytest.raises(osc.OscError) as e:
+            run()
+        assert e.value.code == -3  # OSC_ERR_TEAM
+    c.team_stride, c.team_npes, c.count = 1, 4, 1 << 62
+    for run in (lambda: osc.Batch([c]), lambda: osc.request_alone(c, world.dest_off)):
+        with pytest.raises(osc.OscError):
+            run()
