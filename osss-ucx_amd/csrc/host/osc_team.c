@@ -81,7 +81,9 @@ static osc_team *new_team(const osc_team *parent, int start, int stride,
   if (t == NULL) return NULL;
   t->magic = OSC_TEAM_MAGIC;
   t->start = parent->start + start * parent->stride;
-  t->stride = stride * parent->stride;
+  /* a one-member team accepts any stride (its last member is its first);
+   * the stride then means nothing, and 1 keeps the product below in range */
+  t->stride = size == 1 ? 1 : stride * parent->stride;
   t->npes = size;
   t->parent = (osc_team *)parent;
   t->num_contexts =

@@ -178,6 +178,28 @@ def test_strided_team(heap):
     heap.L.shmem_team_destroy(team)
 
 
+def test_one_member_team_with_any_stride(heap):
+    """A one-member split accepts any stride: from a parent with stride 3,
+    start 2 and stride 2^30 give global PE 1 + 2*3 = 7, and a reduce over
+    it copies that PE's source."""
+    parent = heap.split(1, 3, 10)              # global PEs 1, 4, ..., 28
+    child = heap.split(2, 1 << 30, 1, parent=parent)
+    L = heap.L
+    assert L.shmem_team_n_pes(child) == 1
+    assert L.shmem_team_translate_pe(child, 0, heap.world) == 7
+    assert L.shmem_team_translate_pe(heap.world, 7, child) == 0
+    n = 64
+    x = np.arange(n, dtype=np.int64) * 3 - 50
+    s, d = heap.malloc(n * 8), heap.malloc(n * 8)
+    heap.put(s, x, 7)
+    assert osc.typed("shmem_long_sum_reduce")(child, d, s, n) == 0
+    assert np.array_equal(heap.get(d, n, np.int64, 7), x)
+    heap.free(s)
+    heap.free(d)
+    L.shmem_team_destroy(child)
+    L.shmem_team_destroy(parent)
+
+
 def test_zero_elements_and_errors(heap):
     L = heap.L
     s = heap.malloc(1024)
