@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kMoveBlock) void k_icopy(
        u < units; u += gs) {
     const uint64_t t = u / upe;
     const uint64_t w = u - t * upe;
-    st<W>(dst + t * dstep + w * W, ld<W>(src + t * sstep + w * W));
+    st<W>(dst + t * dstep + w * W, ld_nt<W>(src + t * sstep + w * W));
   }
 }
 
@@ -197,6 +197,9 @@ __global__ void k_signal(uint64_t *sig, const uint64_t value, const int op) {
 // Alltoall, large blocks: block b covers chunk (b % cpb) of the pair
 // (b / cpb); pair p = i*P + j moves src_i[j*nbytes ..] -> dest_j[i*nbytes ..].
 // Each thread moves UPT units, all loads issued before the stores.
+// Loads stay cached: repeated alltoalls over one 256 MiB working set are
+// partly served by the Infinity Cache, and non-temporal loads measured 21%
+// slower at 64 PEs x 64 KiB.
 template <int W, int UPT>
 __global__ __launch_bounds__(kMoveBlock) void k_a2a_big(
     const oscd_geom g, const uint64_t doff, const uint64_t soff,
@@ -299,6 +302,8 @@ __global__ __launch_bounds__(kMoveBlock) void k_a2a_mid(
 }
 
 // Plain copy, 16-byte units with a byte tail (alignment checked by host).
+// Non-temporal loads and stores: a 1 GiB copy measured 6.54 TB/s against
+// 6.34 with cached loads on MI355X.
 template <int W>
 __global__ __launch_bounds__(kMoveBlock) void k_copy(char *dst,
                                                      const char *src,
@@ -307,7 +312,7 @@ __global__ __launch_bounds__(kMoveBlock) void k_copy(char *dst,
   const uint64_t gs = (uint64_t)gridDim.x * kMoveBlock;
   for (uint64_t u = (uint64_t)blockIdx.x * kMoveBlock + threadIdx.x;
        u < nunits; u += gs)
-    st_nt<W>(dst + u * W, ld<W>(src + u * W));
+    st_nt<W>(dst + u * W, ld_nt<W>(src + u * W));
 }
 
 static int unit_width(uint64_t a) {
