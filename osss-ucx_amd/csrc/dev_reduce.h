@@ -38,7 +38,11 @@
 // kernel instead, k_reduce_lanes: a group of lanes shares one element,
 // reads its members in parallel and combines them with a cross-lane
 // butterfly (DPP, ds_swizzle, ds_bpermute), so the call costs one HBM round
-// trip where the slice walk takes one per 8 members.
+// trip where the slice walk takes one per 8 members.  Small float and
+// complex sums and products (16..1024 members, at most 16 KiB per member)
+// take k_reduce_gather: the lanes read the members in parallel into LDS and
+// every lane folds them in team order, the walk's own order, so the result
+// has the walk's bits.
 #include <type_traits>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -841,6 +845,86 @@ static int launch_lanes(const oscd_geom *g, uint64_t doff, uint64_t soff,
   return hipGetLastError() == hipSuccess ? OSC_OK : OSC_ERR_HIP;
 }
 
+// ---- small float calls: lanes gather, one fold in team order ---------------
+// Floating-point sums and products keep the slice walk's order (team index
+// 0, 1, ..., P-1), so a call gives the same bits on every path (alone,
+// batched, served).  Here the 64 lanes of a wave read one element's P
+// members in parallel into LDS (independent loads, in flight together), and
+// every lane then folds the P values in team order with the policy's own
+// lift/seq/out: the walk's arithmetic in the walk's order, without its
+// dependent passes over the slices.
+template <class T, class Pol>
+constexpr bool gather_ok() {
+  return !std::is_integral<T>::value && !Pol::TREE && !Pol::SPECIAL;
+}
+// LDS holds 4 waves x P values: at most 64 KiB for 16-byte elements
+constexpr int kGatherMaxPes = 1024;
+constexpr int kGatherChunk = 8;  // loads in flight per lane
+
+template <class T, class Pol>
+__global__ __launch_bounds__(kBlock) void k_reduce_gather(
+    const oscd_geom g, const uint64_t dest_off, const uint64_t src_off,
+    const uint64_t n) {
+  // values travel as raw words of the element's size (a complex struct in
+  // a local array would live in scratch memory)
+  typedef typename Raw<(int)sizeof(T)>::type R;
+  extern __shared__ unsigned char k_reduce_gather_lds[];
+  const int lane = (int)(threadIdx.x & 63u);
+  const int w = (int)(threadIdx.x >> 6);
+  const uint64_t e = (uint64_t)blockIdx.x * (kBlock / 64) + (uint64_t)w;
+  const int P = g.npes;
+  R *x = (R *)k_reduce_gather_lds + (size_t)w * (size_t)P;
+  const bool act = e < n;  // the same for the whole wave
+  const uint64_t eb = e * sizeof(T);
+  if (act) {
+    for (int j0 = lane; j0 < P; j0 += 64 * kGatherChunk) {
+      R v[kGatherChunk];
+#pragma unroll
+      for (int k = 0; k < kGatherChunk; ++k)
+        if (j0 + 64 * k < P)
+          v[k] = ld<(int)sizeof(T)>(member_ptr(g, j0 + 64 * k, src_off) + eb);
+#pragma unroll
+      for (int k = 0; k < kGatherChunk; ++k)
+        if (j0 + 64 * k < P) x[j0 + 64 * k] = v[k];
+    }
+  }
+  // every wave reaches the barrier; all loads of an element precede it and
+  // all its stores follow it, so dest == source is safe
+  __syncthreads();
+  if (act) {
+    auto value = [&](int j) {
+      const R raw = x[j];
+      T t;
+      __builtin_memcpy(&t, &raw, sizeof(T));
+      return t;
+    };
+    typename Pol::Acc acc = Pol::lift(value(0));
+    for (int j = 1; j < P; ++j) acc = Pol::seq(acc, value(j));
+    const T r = Pol::out(acc);
+    R rr;
+    __builtin_memcpy(&rr, &r, sizeof(T));
+    for (int j = lane; j < P; j += 64)
+      st<(int)sizeof(T)>(member_ptr(g, j, dest_off) + eb, rr);
+  }
+}
+
+template <class T, class Pol>
+static int launch_gather(const oscd_geom *g, uint64_t doff, uint64_t soff,
+                         uint64_t n, hipStream_t s) {
+  static_assert(gather_ok<T, Pol>(), "float and complex accumulations only");
+  const int P = g->npes;
+  const uint64_t al = sizeof(T);
+  if (P < 16 || P > kGatherMaxPes || n == 0 || n > (1ull << 24) ||
+      doff % al != 0 || soff % al != 0 || g->slice % al != 0 ||
+      (uintptr_t)g->base % al != 0)
+    return OSC_ERR_ARG;
+  const unsigned grid = (unsigned)((n + (kBlock / 64) - 1) / (kBlock / 64));
+  const size_t lds = (size_t)(kBlock / 64) * (size_t)P * sizeof(T);
+  hipLaunchKernelGGL((k_reduce_gather<T, Pol>), dim3(grid), dim3(kBlock), lds,
+                     s, *g, doff, soff, n);
+  return hipGetLastError() == hipSuccess ? OSC_OK : OSC_ERR_HIP;
+}
+
 // When the default launch takes the lanes kernel: teams of 16 members or
 // more and at most 2048 elements.  Measured on MI355X (one launch, 64-bit
 // and 32-bit sums, P = 16..256): the lanes kernel stays at the launch floor
@@ -856,6 +940,21 @@ static bool lanes_pays(const oscd_geom *g, uint64_t doff, uint64_t soff,
   return g->npes >= kLanesMinPes && n <= kLanesMaxElems && doff % al == 0 &&
          soff % al == 0 && g->slice % al == 0 &&
          (uintptr_t)g->base % al == 0;
+}
+
+// The gather kernel's cut is in bytes per member: measured (one launch,
+// f32/f64 sums and products, complex sums and products, P = 16..1024) it
+// wins up to 2048 doubles at 64+ members (P = 256, n = 1: 41 vs 9 us;
+// P = 1024: 153 vs 25 us), while 2048 complex doubles (32 KiB) and 8192
+// doubles already run faster through the walk's coalesced loads.
+constexpr uint64_t kGatherMaxBytes = 16384;
+template <class T>
+static bool gather_pays(const oscd_geom *g, uint64_t doff, uint64_t soff,
+                        uint64_t n) {
+  const uint64_t al = sizeof(T);
+  return g->npes >= kLanesMinPes && g->npes <= kGatherMaxPes &&
+         n <= kGatherMaxBytes / sizeof(T) && doff % al == 0 && soff % al == 0 &&
+         g->slice % al == 0 && (uintptr_t)g->base % al == 0;
 }
 
 // ---- host-side launch ------------------------------------------------------
@@ -922,6 +1021,9 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
     if constexpr (lanes_ok<T, Pol>()) {
       if (lanes_pays<T>(g, doff, soff, n))
         return launch_lanes<T, Pol>(g, doff, soff, n, s);
+    } else if constexpr (gather_ok<T, Pol>()) {
+      if (gather_pays<T>(g, doff, soff, n))
+        return launch_gather<T, Pol>(g, doff, soff, n, s);
     }
     // Small calls: at the default shape they would fill fewer workgroups
     // than the chip has CUs; one unit per thread with 8 slices in flight
