@@ -201,8 +201,9 @@ def bench_movement():
     blocks on 64 PEs (2*P^2*b bytes) and shmem_int32_fcollect of 1 MiB per
     PE on 256 PEs (P*m + P^2*m bytes), inputs resident, HIP-event medians.
     Also the latency of the small plain reduce: shmem_int_sum_reduce, 8 PEs x 65,536 int32,
-    kernel time and the blocking call's wall time, and of shmem_long_sum_reduce on
-    16 elements at 64 and 128 PEs (the cross-lane small-call kernel)."""
+    kernel time and the blocking call's wall time, and of shmem_long_sum_reduce and
+    shmem_double_sum_reduce on 16 elements at 64 and 128 PEs (the small-call
+    kernels)."""
     import numpy as np
     L = osc.lib()
     out = {}
@@ -247,26 +248,29 @@ def bench_movement():
         out["int_sum_8pe_64K"] = {"kernel_us": round(kern * 1e3, 2),
                                      "call_p50_us": round(walls[100], 2),
                                      "call_p99_us": round(walls[198], 2)}
-    # small calls on large teams (the cross-lane kernel): 16 int64 per PE
+    # small calls on large teams, 16 elements per PE: int64 (the cross-lane
+    # butterfly kernel) and double (the gather kernel, team-order fold)
     n = 16
     for P in (64, 128):
         with osc.Heap(npes=P, heap_bytes=1 << 20) as h:
             s, d = h.malloc(n * 8), h.malloc(n * 8)
-            h.fill(None, s, n, "i64", "int_range", -1000, 1000, seed=2)
             st = L.osc_default_stream()
-            kern = _median_event_ms(L, st, lambda: L.osc_reduce(
-                h.world, osc.OPS["sum"], osc.DTYPES["i64"], d, s, n, st),
-                reps=50)
-            fn = osc.typed("shmem_long_sum_reduce")
-            walls = []
-            for _ in range(200):
-                t0 = time.perf_counter()
-                fn(h.world, d, s, n)
-                walls.append((time.perf_counter() - t0) * 1e6)
-            walls.sort()
-            out[f"long_sum_{P}pe_16"] = {"kernel_us": round(kern * 1e3, 2),
-                                         "call_p50_us": round(walls[100], 2),
-                                         "call_p99_us": round(walls[198], 2)}
+            for tn, dt, dist, a, b in (("long", "i64", "int_range", -1000, 1000),
+                                       ("double", "f64", "normal", 0.0, 1.0)):
+                h.fill(None, s, n, dt, dist, a, b, seed=2)
+                kern = _median_event_ms(L, st, lambda: L.osc_reduce(
+                    h.world, osc.OPS["sum"], osc.DTYPES[dt], d, s, n, st),
+                    reps=50)
+                fn = osc.typed(f"shmem_{tn}_sum_reduce")
+                walls = []
+                for _ in range(200):
+                    t0 = time.perf_counter()
+                    fn(h.world, d, s, n)
+                    walls.append((time.perf_counter() - t0) * 1e6)
+                walls.sort()
+                out[f"{tn}_sum_{P}pe_16"] = {"kernel_us": round(kern * 1e3, 2),
+                                             "call_p50_us": round(walls[100], 2),
+                                             "call_p99_us": round(walls[198], 2)}
     return out
 
 
