@@ -39,10 +39,11 @@
 // reads its members in parallel and combines them with a cross-lane
 // butterfly (DPP, ds_swizzle, ds_bpermute), so the call costs one HBM round
 // trip where the slice walk takes one per 8 members.  Small float and
-// complex sums and products (16..1024 members, at most 16 KiB per member)
-// take k_reduce_gather: the lanes read the members in parallel into LDS and
+// complex reductions (16..1024 members, at most 16 KiB per member) take
+// k_reduce_gather: the lanes read the members in parallel into LDS and
 // every lane folds them in team order, the walk's own order, so the result
-// has the walk's bits.
+// has the walk's bits (float MIN/MAX elements with a NaN or a signed-zero
+// extremum get per-member results there too).
 #include <type_traits>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -855,7 +856,7 @@ static int launch_lanes(const oscd_geom *g, uint64_t doff, uint64_t soff,
 // dependent passes over the slices.
 template <class T, class Pol>
 constexpr bool gather_ok() {
-  return !std::is_integral<T>::value && !Pol::TREE && !Pol::SPECIAL;
+  return !std::is_integral<T>::value && !Pol::TREE;
 }
 // LDS holds 4 waves x P values: at most 64 KiB for 16-byte elements
 constexpr int kGatherMaxPes = 1024;
@@ -864,7 +865,7 @@ constexpr int kGatherChunk = 8;  // loads in flight per lane
 template <class T, class Pol>
 __global__ __launch_bounds__(kBlock) void k_reduce_gather(
     const oscd_geom g, const uint64_t dest_off, const uint64_t src_off,
-    const uint64_t n) {
+    const uint64_t n, const int lg) {
   // values travel as raw words of the element's size (a complex struct in
   // a local array would live in scratch memory)
   typedef typename Raw<(int)sizeof(T)>::type R;
@@ -899,19 +900,58 @@ __global__ __launch_bounds__(kBlock) void k_reduce_gather(
       return t;
     };
     typename Pol::Acc acc = Pol::lift(value(0));
-    for (int j = 1; j < P; ++j) acc = Pol::seq(acc, value(j));
+    bool nan = is_nan(value(0));
+    for (int j = 1; j < P; ++j) {
+      const T v = value(j);
+      if (Pol::SPECIAL) nan |= is_nan(v);
+      acc = Pol::seq(acc, v);
+    }
     const T r = Pol::out(acc);
-    R rr;
-    __builtin_memcpy(&rr, &r, sizeof(T));
-    for (int j = lane; j < P; j += 64)
-      st<(int)sizeof(T)>(member_ptr(g, j, dest_off) + eb, rr);
+    bool special = false;
+    if constexpr (Pol::SPECIAL) special = nan || is_nan_or_zero(r);
+    if constexpr (Pol::SPECIAL) {
+      if (special) {
+      // float MIN/MAX with a NaN or a signed-zero extremum: each member
+      // gets its own recursive-doubling result, evaluated over the LDS copy
+      // as the slice walk's special path does (the element is the same
+      // for the whole wave, so this branch is uniform)
+      const int p2s = 1 << lg;
+      for (int k = lane; k < p2s; k += 64) {
+        T stk[16];
+        for (int j = 0; j < p2s; ++j) {
+          const int kk = k ^ j;
+          const int m = member_of(kk, P, lg);
+          const int m1 = member_of(kk + 1, P, lg);
+          T leaf = (m1 - m == 2) ? Bin<Pol::OPC>::f(value(m + 1), value(m))
+                                 : value(m);
+          int L = 0;
+          while ((j >> L) & 1) {
+            leaf = Bin<Pol::OPC>::f(stk[L], leaf);
+            ++L;
+          }
+          stk[L] = leaf;
+        }
+        const T rk = stk[lg];
+        const int mk = member_of(k, P, lg);
+        const int mk1 = member_of(k + 1, P, lg);
+        *(T *)(member_ptr(g, mk, dest_off) + eb) = rk;
+        if (mk1 - mk == 2) *(T *)(member_ptr(g, mk + 1, dest_off) + eb) = rk;
+      }
+      }
+    }
+    if (!special) {
+      R rr;
+      __builtin_memcpy(&rr, &r, sizeof(T));
+      for (int j = lane; j < P; j += 64)
+        st<(int)sizeof(T)>(member_ptr(g, j, dest_off) + eb, rr);
+    }
   }
 }
 
 template <class T, class Pol>
-static int launch_gather(const oscd_geom *g, uint64_t doff, uint64_t soff,
-                         uint64_t n, hipStream_t s) {
-  static_assert(gather_ok<T, Pol>(), "float and complex accumulations only");
+static int launch_gather(const oscd_geom *g, int lg, uint64_t doff,
+                         uint64_t soff, uint64_t n, hipStream_t s) {
+  static_assert(gather_ok<T, Pol>(), "float and complex types only");
   const int P = g->npes;
   const uint64_t al = sizeof(T);
   if (P < 16 || P > kGatherMaxPes || n == 0 || n > (1ull << 24) ||
@@ -921,7 +961,7 @@ static int launch_gather(const oscd_geom *g, uint64_t doff, uint64_t soff,
   const unsigned grid = (unsigned)((n + (kBlock / 64) - 1) / (kBlock / 64));
   const size_t lds = (size_t)(kBlock / 64) * (size_t)P * sizeof(T);
   hipLaunchKernelGGL((k_reduce_gather<T, Pol>), dim3(grid), dim3(kBlock), lds,
-                     s, *g, doff, soff, n);
+                     s, *g, doff, soff, n, lg);
   return hipGetLastError() == hipSuccess ? OSC_OK : OSC_ERR_HIP;
 }
 
@@ -1023,7 +1063,7 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
         return launch_lanes<T, Pol>(g, doff, soff, n, s);
     } else if constexpr (gather_ok<T, Pol>()) {
       if (gather_pays<T>(g, doff, soff, n))
-        return launch_gather<T, Pol>(g, doff, soff, n, s);
+        return launch_gather<T, Pol>(g, lg, doff, soff, n, s);
     }
     // Small calls: at the default shape they would fill fewer workgroups
     // than the chip has CUs; one unit per thread with 8 slices in flight

@@ -1,7 +1,7 @@
 """GPU parity of the small-call float kernel (lanes gather the members into
 LDS, then every lane folds them in team order), which the default launch
-takes for float and complex sums and products on teams of 16 to 1024
-members with at most 16 KiB per member.
+takes for float and complex sums, products, minima and maxima on teams of
+16 to 1024 members with at most 16 KiB per member.
 
 The fold uses the slice walk's own arithmetic in the walk's order, so every
 result must be bit-identical to the slice walk on the same inputs (forced
@@ -14,7 +14,7 @@ import pytest
 
 import osc
 import oracle as O
-from gpu_util import inputs, max_rel_err, run_reduce
+from gpu_util import bit_equal, inputs, max_rel_err, run_reduce
 
 pytestmark = pytest.mark.gpu
 
@@ -123,3 +123,25 @@ def test_gather_team_limits():
                     got = run_reduce(h, op, dtype, src, team=team)
                     want = walk(h, op, dtype, src, team)
                     assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), (P, dtype, n)
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+@pytest.mark.parametrize("op", ["min", "max"])
+def test_gather_min_max_special_values(heap, dtype, op):
+    """Float MIN/MAX through the gather kernel: elements holding a NaN or
+    whose extremum is a signed zero give every member its own recursive-
+    doubling result (bit-exact against the oracle, per member); the others
+    give every member the same exact extremum.  Both equal the walk."""
+    rng = np.random.default_rng(73 + (op == "max") + 2 * (dtype == "f64"))
+    specials = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1.5, -2.0])
+    for P in (16, 17, 64, 100, 256):
+        team = team_of(heap, P)
+        for n in (1, 64, 301):
+            src = rng.choice(specials, size=(P, n)).astype(O.NP_DTYPE[dtype])
+            src[:, ::3] = rng.standard_normal((P, len(range(0, n, 3))))
+            got = run_reduce(heap, op, dtype, src, team=team)
+            want = O.rec_dbl(op, dtype, src)
+            assert bit_equal(got, want, dtype), (P, n)
+            assert bit_equal(got, walk(heap, op, dtype, src, team), dtype), (P, n)
+        if team != heap.world:
+            heap.L.shmem_team_destroy(team)
