@@ -43,7 +43,8 @@
 // k_reduce_gather: the lanes read the members in parallel into LDS and
 // every lane folds them in team order, the walk's own order, so the result
 // has the walk's bits (float MIN/MAX elements with a NaN or a signed-zero
-// extremum get per-member results there too).
+// extremum get per-member results there too, and reference mode folds the
+// reference's tree).
 #include <type_traits>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -848,15 +849,15 @@ static int launch_lanes(const oscd_geom *g, uint64_t doff, uint64_t soff,
 
 // ---- small float calls: lanes gather, one fold in team order ---------------
 // Floating-point sums and products keep the slice walk's order (team index
-// 0, 1, ..., P-1), so a call gives the same bits on every path (alone,
-// batched, served).  Here the 64 lanes of a wave read one element's P
-// members in parallel into LDS (independent loads, in flight together), and
-// every lane then folds the P values in team order with the policy's own
-// lift/seq/out: the walk's arithmetic in the walk's order, without its
-// dependent passes over the slices.
+// 0, 1, ..., P-1, or the reference's tree in reference mode), so a call
+// gives the same bits on every path (alone, batched, served).  Here the 64
+// lanes of a wave read one element's P members in parallel into LDS
+// (independent loads, in flight together), and every lane then folds the P
+// values in the walk's order with the policy's own operations, without the
+// walk's dependent passes over the slices.
 template <class T, class Pol>
 constexpr bool gather_ok() {
-  return !std::is_integral<T>::value && !Pol::TREE;
+  return !std::is_integral<T>::value;
 }
 // LDS holds 4 waves x P values: at most 64 KiB for 16-byte elements
 constexpr int kGatherMaxPes = 1024;
@@ -899,12 +900,43 @@ __global__ __launch_bounds__(kBlock) void k_reduce_gather(
       __builtin_memcpy(&t, &raw, sizeof(T));
       return t;
     };
-    typename Pol::Acc acc = Pol::lift(value(0));
-    bool nan = is_nan(value(0));
-    for (int j = 1; j < P; ++j) {
-      const T v = value(j);
-      if (Pol::SPECIAL) nan |= is_nan(v);
-      acc = Pol::seq(acc, v);
+    typename Pol::Acc acc;
+    bool nan = false;
+    if constexpr (Pol::TREE) {
+      // the reference's order: member 0's recursive-doubling tree (leaves
+      // in team order, a folded pair as op(x[m+1], x[m]), merged by a
+      // binary counter), which every member's tree equals for + and *
+      // (constant stack indices and unconditional selects, as in the
+      // walk's tree, keep the stack in registers; lg <= 10 here)
+      constexpr int kLv = 11;
+      const int p2s = 1 << lg;
+      typename Pol::Acc stk[kLv];
+      for (int j = 0; j < p2s; ++j) {
+        const int m = member_of(j, P, lg);
+        const int m1 = member_of(j + 1, P, lg);
+        typename Pol::Acc leaf =
+            (m1 - m == 2) ? Pol::comb(Pol::lift(value(m + 1)), Pol::lift(value(m)))
+                          : Pol::lift(value(m));
+        bool carry = true;
+#pragma unroll
+        for (int L = 0; L < kLv; ++L) {
+          const bool occupied = ((j >> L) & 1) != 0;
+          if (carry && occupied) leaf = Pol::comb(stk[L], leaf);
+          stk[L] = pick(carry && !occupied, leaf, stk[L]);
+          carry = carry && occupied;
+        }
+      }
+#pragma unroll
+      for (int L = 0; L < kLv; ++L)
+        if (L == lg) acc = stk[L];
+    } else {
+      acc = Pol::lift(value(0));
+      nan = is_nan(value(0));
+      for (int j = 1; j < P; ++j) {
+        const T v = value(j);
+        if (Pol::SPECIAL) nan |= is_nan(v);
+        acc = Pol::seq(acc, v);
+      }
     }
     const T r = Pol::out(acc);
     bool special = false;

@@ -145,3 +145,39 @@ def test_gather_min_max_special_values(heap, dtype, op):
             assert bit_equal(got, walk(heap, op, dtype, src, team), dtype), (P, n)
         if team != heap.world:
             heap.L.shmem_team_destroy(team)
+
+
+def walk_mode(h, op, dtype, src, team, mode):
+    osc.check(h.L.osc_set_reduce_shape(3), "shape")
+    try:
+        return run_reduce(h, op, dtype, src, team=team, mode=mode)
+    finally:
+        h.L.osc_set_reduce_shape(-1)
+
+
+@pytest.mark.parametrize("dtype,op", CASES)
+def test_gather_reference_order(heap, dtype, op):
+    """Reference mode (rec_dbl): the gather kernel evaluates member 0's
+    recursive-doubling tree over the LDS copy; every member's result must
+    be bit-identical to the oracle's step-by-step recursive doubling and to
+    the walk's tree."""
+    rng = np.random.default_rng(3000 + CASES.index((dtype, op)))
+    for P in (16, 17, 64, 100, 256):
+        team = team_of(heap, P)
+        for n in (1, 3, 257):
+            src = inputs(rng, op, dtype, P, n)
+            got = run_reduce(heap, op, dtype, src, team=team, mode=1)
+            assert bit_equal(got, O.rec_dbl(op, dtype, src), dtype), (P, n)
+            assert bit_equal(got, walk_mode(heap, op, dtype, src, team, 1), dtype), (P, n)
+        if team != heap.world:
+            heap.L.shmem_team_destroy(team)
+
+
+def test_gather_reference_order_large_team():
+    """P = 1000: p2s = 512 leaves, 488 of them folded pairs."""
+    rng = np.random.default_rng(79)
+    with osc.Heap(npes=1000, heap_bytes=1 << 20) as h:
+        for dtype, op in (("f32", "sum"), ("f64", "prod"), ("c64", "sum")):
+            src = inputs(rng, op, dtype, 1000, 33)
+            got = run_reduce(h, op, dtype, src, mode=1)
+            assert bit_equal(got, O.rec_dbl(op, dtype, src), dtype), (dtype, op)
