@@ -1,12 +1,14 @@
 // dev_amo.hip - atomic memory operations on symmetric variables.
 //
 // One lane of a one-wave kernel performs the operation on the target's
-// device address (32 or 64 bits).  A fetching operation stores the old
-// value, then a ticket with a system-scope release, into its own slot of
-// coherent pinned host memory; the host spins on the ticket for a bounded
-// time and then falls back to hipStreamSynchronize, as the completion
-// wait of blocking calls does.  A non-fetching operation is only enqueued:
-// stream order puts it before every later operation of the library.
+// device address (32 or 64 bits).  A fetching operation clears its slot's
+// ticket, stores the old value, then the ticket with a system-scope release,
+// into its own slot of coherent pinned host memory (a sequence lock: the
+// host re-reads the ticket after the value); the host spins on the ticket
+// for a bounded time and then falls back to hipStreamSynchronize, as the
+// completion wait of blocking calls does.  A non-fetching operation is only
+// enqueued: stream order puts it before every later operation of the
+// library.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -55,6 +57,12 @@ __global__ void k_amo(W *addr, const int op, const W v, const W cond,
       break;
   }
   if (slot != nullptr) {
+    // Sequence lock: the slot's ticket is cleared before the value is
+    // replaced, so a host reader that finds the same ticket before and
+    // after reading the value knows the value is this operation's.
+    __hip_atomic_store(slot + 1, (uint64_t)0, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
     __hip_atomic_store(slot, (uint64_t)old, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(slot + 1, ticket, __ATOMIC_RELEASE,
@@ -122,5 +130,8 @@ extern "C" int oscd_amo(void *addr, int width, int op, uint64_t value,
     }
   }
   *old = __atomic_load_n(slot, __ATOMIC_RELAXED);
+  // the slot was reused (1024 later operations) while it was being read
+  __atomic_thread_fence(__ATOMIC_ACQUIRE);
+  if (__atomic_load_n(slot + 1, __ATOMIC_RELAXED) != ticket) return OSC_ERR_STATE;
   return OSC_OK;
 }

@@ -63,12 +63,20 @@ int osc_heap_alloc(size_t size, size_t align, size_t *off) {
   if (align < OSC_MIN_ALIGN) align = OSC_MIN_ALIGN;
   if ((align & (align - 1)) != 0) return OSC_ERR_ARG;
   if (size == 0) size = 1;
+  /* Every offset and block end lies in [0, heap_bytes]: a request larger
+   * than the heap can never fit, and rejecting it here keeps the rounding
+   * and the fit test below from wrapping (the reference's dlmalloc returns
+   * NULL for such sizes as well). */
+  if (size > osc_g.heap_bytes || size > ((size_t)-1) - OSC_MIN_ALIGN ||
+      align > osc_g.heap_bytes)
+    return OSC_ERR_NOMEM;
   size = round_up(size, OSC_MIN_ALIGN);
   for (b = osc_g.blocks; b != NULL; b = b->next) {
-    size_t start;
+    size_t start, end;
     if (b->used) continue;
     start = round_up(b->off, align);
-    if (start + size > b->off + b->size) continue;
+    end = b->off + b->size;
+    if (start < b->off || start > end || size > end - start) continue;
     if (split_at(b, start) != OSC_OK) return OSC_ERR_NOMEM;
     if (start != b->off) b = b->next; /* the aligned tail */
     if (split_at(b, start + size) != OSC_OK) return OSC_ERR_NOMEM;

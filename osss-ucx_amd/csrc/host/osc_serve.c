@@ -29,6 +29,7 @@
 #define _GNU_SOURCE
 #include <sched.h>
 #include <stdio.h>
+#include <limits.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -231,9 +232,14 @@ int osc_serve_run(osc_request *reqs, int nreq, const osc_serve_cfg *cfg,
       !(cfg->age_bytes_per_s >= 0.0 && cfg->age_bytes_per_s <= 1e30))
     return osc_set_error(OSC_ERR_ARG, "serve order %d / age weight %g", cfg->order,
                          cfg->age_bytes_per_s);
-  if (cfg->dest_base + cfg->dest_bytes > osc_g.heap_bytes || page % 256 != 0 ||
+  /* written so that no sum wraps: base, then size within what is left */
+  if (cfg->dest_base > osc_g.heap_bytes ||
+      cfg->dest_bytes > osc_g.heap_bytes - cfg->dest_base || page % 256 != 0 ||
       cfg->dest_base % 256 != 0)
     return osc_set_error(OSC_ERR_ARG, "destination arena outside the heap or misaligned");
+  if (cfg->dest_bytes / page < 1 || cfg->dest_bytes / page > (uint64_t)INT_MAX)
+    return osc_set_error(OSC_ERR_ARG, "destination arena of %llu pages",
+                         (unsigned long long)(cfg->dest_bytes / page));
   for (i = 1; i < nreq; ++i)
     if (reqs[i].arrival_s < reqs[i - 1].arrival_s)
       return osc_set_error(OSC_ERR_ARG, "arrivals must be non-decreasing (request %d)", i);

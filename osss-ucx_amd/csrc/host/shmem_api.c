@@ -465,8 +465,14 @@ void shmem_ctx_quiet(shmem_ctx_t ctx) {
 /* A copy on stream st; the host side is copied once the stream's earlier
  * work is done. */
 static void xfer(void *st, void *dst, const void *src, size_t nbytes, int blocking) {
+  size_t off;
   int rc;
   if (nbytes == 0) return;
+  /* a symmetric local side must lie inside one PE's heap as a whole, or
+   * the copy would run into the next PE's slice (or past the arena) */
+  if ((is_sym(dst) && osc_sym_range(dst, nbytes, &off) != OSC_OK) ||
+      (is_sym(src) && osc_sym_range(src, nbytes, &off) != OSC_OK))
+    return;
   if (is_sym(dst) && is_sym(src)) {
     rc = oscd_copy(dst, src, nbytes, st);
     if (rc == OSC_OK && blocking) rc = oscd_stream_wait(st, osc_g.wait_spin_us);
@@ -567,6 +573,13 @@ static void strided(shmem_ctx_t ctx, void *dest, const void *src, size_t es,
   r = remote(rsym, lo, (size_t)(hi - lo), pe);
   if (r == NULL) return;
   if (is_sym(lp)) {
+    /* the local side's extent, as the remote one's above */
+    const ptrdiff_t llo = lst < 0 ? (ptrdiff_t)(n - 1) * lst * (ptrdiff_t)es : 0;
+    const ptrdiff_t lhi =
+        (lst > 0 ? (ptrdiff_t)(n - 1) * lst * (ptrdiff_t)es : 0) + (ptrdiff_t)es;
+    size_t loff;
+    if (osc_sym_range((char *)lp + llo, (size_t)(lhi - llo), &loff) != OSC_OK)
+      return;
     rc = is_put ? oscd_copy_strided(r, lp, es, (uint64_t)(rst * (ptrdiff_t)es),
                                     (uint64_t)(lst * (ptrdiff_t)es), n,
                                     st)

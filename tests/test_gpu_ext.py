@@ -410,3 +410,43 @@ def test_to_all_algorithm_variable(monkeypatch):
         assert osc.typed("shmem_float_sum_reduce")(h.world, d, s, n) == 0
         team = np.stack([h.get(d, n, np.float32, pe) for pe in range(P)])
         assert max_rel_err(team, src, "sum", "f32") <= 2.0 ** -23
+
+
+def test_rma_symmetric_local_side_is_range_checked(heap):
+    """A symmetric local buffer must hold the whole transfer inside its
+    PE's heap: a get into (or put from) the last bytes of PE 0's heap with
+    a byte count that is valid on the remote side is OSC_ERR_PTR, and the
+    bytes past the end (PE 1's slice) stay untouched."""
+    L = heap.L
+    a = heap.malloc(8192)
+    hb = L.osc_heap_bytes()
+    near_end = L.osc_heap_base() + hb - 100       # PE 0's copy, 100 B left
+    guard = np.arange(4096, dtype=np.uint8) ^ 0x5A
+    nxt = L.osc_heap_base() + L.osc_slice_bytes()  # PE 1's slice start
+    try:
+        heap.put(a, np.full(4096, 7, np.uint8), 3)
+        L.osc_put_host(C.c_void_p(nxt), guard.ctypes.data, 4096, 0)
+        cases = [
+            lambda: L.shmem_getmem(C.c_void_p(near_end), a, 4096, 3),
+            lambda: L.shmem_putmem(a, C.c_void_p(near_end), 4096, 3),
+            lambda: L.shmem_iget64(C.c_void_p(near_end), a, 1, 1, 512, 3),
+            lambda: L.shmem_iput64(a, C.c_void_p(near_end), 1, 1, 512, 3),
+        ]
+        for k, call in enumerate(cases):
+            L.osc_clear_error()
+            call()
+            assert L.osc_last_error_code() == -4, k
+            L.shmem_quiet()
+            got = np.zeros(4096, np.uint8)
+            L.osc_get_host(got.ctypes.data, C.c_void_p(nxt), 4096, 0)
+            assert np.array_equal(got, guard), k
+        # the remote side is untouched by the rejected puts
+        assert np.array_equal(heap.get(a, 4096, np.uint8, 3),
+                              np.full(4096, 7, np.uint8))
+        # the same calls within the heap still work
+        L.osc_clear_error()
+        L.shmem_getmem(C.c_void_p(near_end), a, 100, 3)
+        assert L.osc_last_error_code() == 0
+    finally:
+        L.osc_clear_error()
+        heap.free(a)

@@ -216,3 +216,18 @@ def test_request_reading_from_the_destination_arena_is_rejected(sw):
     assert st.rejected == 1 and out[5].status == -1
     assert st.completed == len(reqs) - 1
     assert all(r.status == 0 for k, r in enumerate(out) if k != 5)
+
+
+def test_wrapping_or_empty_destination_arena_is_an_argument_error(sw):
+    """dest_base + dest_bytes must not wrap in 64 bits, and the arena must
+    hold at least one page: a wrapped or empty arena is OSC_ERR_ARG before
+    any request is looked at, not a loop that never places anything."""
+    reqs = requests(sw, 11, 2, 1e4)
+    for base, nbytes in [(sw.arena, 2 ** 64 - 256),
+                         (2 ** 64 - 256, 512),
+                         (sw.arena, 1000)]:        # less than one page
+        cfg = sw.cfg(osc.SERVE_BATCHED)
+        cfg.dest_base, cfg.dest_bytes = base, nbytes
+        with pytest.raises(osc.OscError) as e:
+            osc.serve(reqs, cfg)
+        assert e.value.code == -1
