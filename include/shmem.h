@@ -99,13 +99,22 @@ void shmem_quiet(void);
 void shmem_fence(void);
 void shmem_info_get_version(int *major, int *minor);
 void shmem_info_get_name(char *name);
-/* proposed additions the reference also has (api.h:318-331): the spec
+/* proposed additions the reference also has (api.h:321-331), declared
+ * under PR463 as there (the library always exports them): the spec
  * version as major * 100 + minor; this library's own version 1.0.0, as a
  * number major * 10000 + minor * 100 + patch */
+#ifdef PR463
 void shmem_info_get_version_number(int *version);
 void shmem_info_get_vendor_version(int *major, int *minor, int *patch);
 void shmem_info_get_vendor_version_number(int *version);
+#endif
 void shmem_pcontrol(const int level, ...); /* profiling control: no effect */
+/* names deprecated since OpenSHMEM 1.2 (reference api.h:58, :203, :232):
+ * start_pes(n) is shmem_init (n is ignored, as the reference ignores it),
+ * _my_pe/_num_pes are shmem_my_pe/shmem_n_pes */
+void start_pes(int npes);
+int _my_pe(void);
+int _num_pes(void);
 
 /* ---- symmetric heap (reference src/api/shmalloc.c:47-195) --------------- */
 void *shmem_malloc(size_t size);
@@ -115,6 +124,11 @@ void *shmem_realloc(void *ptr, size_t size);
 void shmem_free(void *ptr);
 /* hints only describe expected use; the block is a plain shmem_malloc */
 void *shmem_malloc_with_hints(size_t size, long hints);
+/* names deprecated since OpenSHMEM 1.2 (reference api.h:1290-1318) */
+void *shmalloc(size_t size);
+void shfree(void *ptr);
+void *shrealloc(void *ptr, size_t size);
+void *shmemalign(size_t alignment, size_t size);
 /* the device address of dest's copy on PE pe (a global PE for shmem_ptr,
  * a team index for shmem_team_ptr), NULL if pe is not one */
 void *shmem_ptr(const void *dest, int pe);
@@ -743,8 +757,9 @@ void shmem_sync_all(void);
  * <stdint.h> and <stddef.h> type is a typedef of one of them (int64_t of
  * long, size_t of unsigned long, ...), so it selects the routine of that C
  * type, which has the same element size and arithmetic.  An element type
- * outside a routine's table does not compile.  There are no contexts in
- * this library, so there are no shmem_ctx_t forms. */
+ * outside a routine's table does not compile.  The RMA, atomic and
+ * signal forms also take a leading shmem_ctx_t, which selects the
+ * routine's shmem_ctx_ form (as the reference's generics do). */
 #if defined(__STDC_VERSION__) && __STDC_VERSION__ >= 201112L && !defined(__cplusplus)
 #define OSC_GEN_INTS_P(_pre, _op)                                              \
   char: _pre##char_##_op, signed char: _pre##schar_##_op,                      \

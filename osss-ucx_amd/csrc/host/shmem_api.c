@@ -129,6 +129,16 @@ void shmem_global_exit(int status) {
 
 void shmem_pcontrol(const int level, ...) { (void)level; }
 
+/* names deprecated since OpenSHMEM 1.2 (reference src/api/deprecated/
+ * init.c:21-28, ranks.c:18-32): the PE count argument of start_pes is
+ * ignored, the library starts as shmem_init does */
+void start_pes(int npes) {
+  (void)npes;
+  shmem_init();
+}
+int _my_pe(void) { return shmem_my_pe(); }
+int _num_pes(void) { return shmem_n_pes(); }
+
 int shmem_pe_accessible(int pe) {
   return osc_g.initialized && pe >= 0 && pe < osc_g.npes;
 }
@@ -282,6 +292,15 @@ void *shmem_team_ptr(shmem_team_t team, const void *dest, int pe) {
 void *shmem_malloc_with_hints(size_t size, long hints) {
   (void)hints;
   return shmem_malloc(size);
+}
+
+/* names deprecated since OpenSHMEM 1.2 (reference src/api/deprecated/
+ * shmalloc.c:25-48) */
+void *shmalloc(size_t size) { return shmem_malloc(size); }
+void shfree(void *ptr) { shmem_free(ptr); }
+void *shrealloc(void *ptr, size_t size) { return shmem_realloc(ptr, size); }
+void *shmemalign(size_t alignment, size_t size) {
+  return shmem_align(alignment, size);
 }
 
 int shmem_addr_accessible(const void *addr, int pe) {
