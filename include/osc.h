@@ -142,6 +142,8 @@ int osc_spmd_active(void); /* 1 inside osc_spmd_run */
  * address is outside the heap. */
 int osc_sym_offset(const void *sym, int *pe, size_t *offset);
 void *osc_pe_ptr(const void *sym, int pe); /* same object on PE pe */
+/* Blocking host <-> symmetric copies on PE pe, ordered after everything
+ * enqueued on the library stream before the call. */
 int osc_put_host(void *sym_dest, const void *host_src, size_t bytes, int pe);
 int osc_get_host(void *host_dest, const void *sym_src, size_t bytes, int pe);
 

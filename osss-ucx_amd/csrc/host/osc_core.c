@@ -231,6 +231,10 @@ int osc_put_host(void *sym_dest, const void *host_src, size_t bytes, int pe) {
   if (pe < 0 || pe >= osc_g.npes) return osc_set_error(OSC_ERR_ARG, "bad PE %d", pe);
   rc = osc_sym_range(sym_dest, bytes, &off);
   if (rc != OSC_OK) return rc;
+  /* the library stream is non-blocking: the copy below (null stream) must
+   * not overtake work enqueued on it before this call */
+  rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
+  if (rc != OSC_OK) return osc_hip_error(rc, "stream wait");
   rc = oscd_memcpy_h2d(osc_g.base + (size_t)pe * osc_g.slice + off, host_src, bytes);
   return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "hipMemcpy H2D");
 }
@@ -241,6 +245,8 @@ int osc_get_host(void *host_dest, const void *sym_src, size_t bytes, int pe) {
   if (pe < 0 || pe >= osc_g.npes) return osc_set_error(OSC_ERR_ARG, "bad PE %d", pe);
   rc = osc_sym_range(sym_src, bytes, &off);
   if (rc != OSC_OK) return rc;
+  rc = oscd_stream_wait(osc_g.stream, osc_g.wait_spin_us);
+  if (rc != OSC_OK) return osc_hip_error(rc, "stream wait");
   rc = oscd_memcpy_d2h(host_dest, osc_g.base + (size_t)pe * osc_g.slice + off, bytes);
   return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "hipMemcpy D2H");
 }

@@ -15,15 +15,17 @@ from gpu_util import bit_equal, inputs, max_rel_err, run_reduce
 
 pytestmark = pytest.mark.gpu
 
-P = 5
-_H = []
+# team sizes: 5 (no slice left over after the groups of 2, 4 or 8), 19
+# (leftover slices for the groups of 4 and 8, and an odd tree fold) and 64
+# (the reduce matrix's team: one slice left over after the groups of 2)
+TEAMS = [5, 19, 64]
 
 
-@pytest.fixture
-def heap():
-    if not _H or not _H[0].alive:
-        _H[:] = [osc.Heap(npes=P, heap_bytes=48 << 20)]
-    return _H[0]
+@pytest.fixture(scope="module", params=TEAMS)
+def heap(request):
+    h = osc.Heap(npes=request.param, heap_bytes=8 << 20)
+    yield h
+    h.close()
 
 
 # (dtype, op): 8-, 4- and 2-byte elements, a float special-value policy
@@ -43,7 +45,9 @@ def sizes_for(shape, vec):
 @pytest.mark.parametrize("dtype,op", CASES)
 def test_shapes_at_span_boundaries(heap, shape, dtype, op):
     vec = 16 // np.dtype(O.NP_DTYPE[dtype]).itemsize
-    rng = np.random.default_rng(zlib.crc32(repr((shape, dtype, op)).encode()))
+    P = heap.L.osc_npes()
+    rng = np.random.default_rng(zlib.crc32(repr((shape, dtype, op,
+                                                 P)).encode()))
     osc.check(heap.L.osc_set_reduce_shape(shape), "shape")
     try:
         for n in sizes_for(shape if shape >= 0 else 0, vec):
@@ -55,9 +59,10 @@ def test_shapes_at_span_boundaries(heap, shape, dtype, op):
                 got = run_reduce(heap, op, dtype, src, mode=mode)
                 if mode == osc.FP_REFERENCE:
                     want = O.rec_dbl(op, dtype, src)
-                    assert bit_equal(got, want, dtype), (shape, n)
+                    assert bit_equal(got, want, dtype), (P, shape, n)
                 else:
                     tol = 2.0 ** -23 if dtype == "f32" else 2.0 ** -52
-                    assert max_rel_err(got, src, op, dtype) <= tol, (shape, n)
+                    assert max_rel_err(got, src, op, dtype) <= tol, (P, shape,
+                                                                     n)
     finally:
         heap.L.osc_set_reduce_shape(-1)
