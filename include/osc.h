@@ -162,6 +162,18 @@ int osc_broadcast(void *team, void *dest, const void *src, size_t nbytes,
  * order (reference src/shcoll/src/collect.c:406-492 closed form). */
 int osc_collect(void *team, void *dest, const void *src, const size_t *nbytes,
                 void *stream);
+/* osc_collect without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device or pinned memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The per-member (offset, size) table goes into a static
+ * device array of the library (4096 members, 64 KiB per GPU, held until
+ * the process ends) through kernel arguments on `stream`; teams of more
+ * than 4096 members are rejected with OSC_ERR_ARG.  Every call on a GPU
+ * shares that array and rewrites the entries it uses, so calls that
+ * overlap in time on one GPU (other streams, unordered host threads)
+ * race; calls ordered on one stream do not.  Results equal osc_collect's. */
+int osc_collect_noalloc(void *team, void *dest, const void *src,
+                        const size_t *nbytes, void *stream);
 /* strided alltoall of nelems elements of elem_bytes per member pair;
  * dst/sst >= 1 in elements (reference src/shcoll/src/alltoalls.c:54-90). */
 int osc_alltoalls(void *team, void *dest, const void *src, ptrdiff_t dst,

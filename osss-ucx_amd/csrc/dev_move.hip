@@ -438,6 +438,81 @@ extern "C" int oscd_collect(const oscd_geom *g, uint64_t dest_off,
   return check_launch();
 }
 
+// ---- allocation-free collect ------------------------------------------------
+// The (offset, size) table of a collect in a file-scope device array
+// instead of memory the library allocates.  Each GPU holds its own copy of
+// the array (at its own address) until the process ends; oscd_scratch_init
+// records that address per device.  Host code must use the recorded
+// address, never the array's name: in host code the name is only a
+// host-side stand-in.
+namespace osc {
+
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) uint64_t scratch[2 * OSCD_STATIC_TAB_MEMBERS];
+static uint64_t *g_scratch[kMaxDevices];
+
+// The table arrives by value in the kernel arguments, kTabChunk words per
+// launch: no host staging buffer and no copy engine transfer.
+constexpr uint32_t kTabChunk = 240;
+struct TabChunk {
+  uint64_t v[kTabChunk];
+};
+
+__global__ __launch_bounds__(kMoveBlock) void k_tab_put(uint64_t *tab,
+                                                        const uint32_t first,
+                                                        const uint32_t count,
+                                                        const TabChunk c) {
+  if (threadIdx.x < count) tab[first + threadIdx.x] = c.v[threadIdx.x];
+}
+
+}  // namespace osc
+
+extern "C" int oscd_scratch_init(int dev) {
+  using namespace osc;
+  if (dev < 0 || dev >= kMaxDevices) return OSC_ERR_ARG;
+  if (g_scratch[dev] != nullptr) return OSC_OK;
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != dev) return OSC_ERR_STATE;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess ||
+      p == nullptr) {
+    (void)hipGetLastError();
+    return OSC_ERR_HIP;
+  }
+  g_scratch[dev] = (uint64_t *)p;
+  return OSC_OK;
+}
+
+extern "C" int oscd_collect_static(const oscd_geom *g, uint64_t dest_off,
+                                   uint64_t src_off, const uint64_t *nbytes,
+                                   uint64_t max_n, int unit_width, int dev,
+                                   void *stream) {
+  using namespace osc;
+  if (g == nullptr || nbytes == nullptr || g->npes <= 0) return OSC_ERR_ARG;
+  if (g->npes > OSCD_STATIC_TAB_MEMBERS) return OSC_ERR_ARG;
+  if (dev < 0 || dev >= kMaxDevices || g_scratch[dev] == nullptr)
+    return OSC_ERR_STATE;
+  if (max_n == 0) return OSC_OK;
+  uint64_t *tab = g_scratch[dev];
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t words = 2u * (uint32_t)g->npes;
+  uint64_t off = 0;
+  for (uint32_t first = 0; first < words; first += kTabChunk) {
+    TabChunk c;
+    const uint32_t cnt = words - first < kTabChunk ? words - first : kTabChunk;
+    for (uint32_t k = 0; k < cnt; k += 2) {  // first and kTabChunk are even
+      const uint64_t n = nbytes[(first + k) / 2];
+      c.v[k] = off;
+      c.v[k + 1] = n;
+      off += n;
+    }
+    hipLaunchKernelGGL(k_tab_put, dim3(1), dim3(kMoveBlock), 0, s, tab, first,
+                       cnt, c);
+    if (check_launch() != OSC_OK) return OSC_ERR_HIP;
+  }
+  return oscd_collect(g, dest_off, src_off, tab, max_n, unit_width, stream);
+}
+
 extern "C" int oscd_alltoalls(const oscd_geom *g, uint64_t dest_off,
                               uint64_t src_off, uint64_t es, uint64_t dst,
                               uint64_t sst, uint64_t nelems, void *stream) {

@@ -143,9 +143,25 @@ int osc_broadcast(void *team, void *dest, const void *src, size_t nbytes,
  * collect's kernel has read them; tab_mu serialises host threads. */
 static pthread_mutex_t tab_mu = PTHREAD_MUTEX_INITIALIZER;
 
+/* Widest unit (16, 8, 4, 2 or 1 bytes) dividing every offset and size of
+ * a collect, the arena base and the slice. */
+static int collect_unit(size_t doff, size_t soff, const size_t *nbytes,
+                        int npes) {
+  uint64_t align = (uint64_t)doff | (uint64_t)soff | (uint64_t)osc_g.slice |
+                   (uint64_t)(uintptr_t)osc_g.base;
+  size_t m = 0;
+  int i, W;
+  for (i = 0; i < npes; ++i) {
+    align |= (uint64_t)m | (uint64_t)nbytes[i];
+    m += nbytes[i];
+  }
+  for (W = 16; W > 1 && (align % (uint64_t)W) != 0; W >>= 1) {
+  }
+  return W;
+}
+
 static int collect_launch(const osc_team *t, size_t doff, size_t soff,
                           const size_t *nbytes, size_t maxn, void *s) {
-  uint64_t align;
   size_t m;
   oscd_geom g;
   int i, rc, W;
@@ -164,16 +180,12 @@ static int collect_launch(const osc_team *t, size_t doff, size_t soff,
     rc = oscd_event_sync(osc_g.tab_event);
     if (rc != OSC_OK) return osc_hip_error(rc, "collect table wait");
   }
-  align = (uint64_t)doff | (uint64_t)soff | (uint64_t)osc_g.slice |
-          (uint64_t)(uintptr_t)osc_g.base;
   for (i = 0, m = 0; i < t->npes; ++i) {
     osc_g.tab_host[2 * i] = m;
     osc_g.tab_host[2 * i + 1] = nbytes[i];
-    align |= (uint64_t)m | (uint64_t)nbytes[i];
     m += nbytes[i];
   }
-  for (W = 16; W > 1 && (align % (uint64_t)W) != 0; W >>= 1) {
-  }
+  W = collect_unit(doff, soff, nbytes, t->npes);
   rc = oscd_memcpy_h2d_async(osc_g.tab_dev, osc_g.tab_host,
                              (size_t)2 * t->npes * sizeof(uint64_t), s);
   if (rc != OSC_OK) return osc_hip_error(rc, "collect table copy");
@@ -186,6 +198,33 @@ static int collect_launch(const osc_team *t, size_t doff, size_t soff,
   return OSC_OK;
 }
 
+/* Argument checks shared by both collect entry points; *doff, *soff and
+ * *maxn describe the call (maxn == 0: nothing to move). */
+static int collect_check(const osc_team *t, void *dest, const void *src,
+                         const size_t *nbytes, size_t *doff, size_t *soff,
+                         size_t *maxn) {
+  size_t total = 0;
+  int i, rc;
+  *maxn = 0;
+  if (!osc_g.initialized) return osc_set_error(OSC_ERR_NOTINIT, "not initialised");
+  if (t == NULL) return osc_set_error(OSC_ERR_TEAM, "invalid team");
+  if (nbytes == NULL) return osc_set_error(OSC_ERR_ARG, "NULL size table");
+  for (i = 0; i < t->npes; ++i) {
+    if (nbytes[i] > ((size_t)-1) - total)
+      return osc_set_error(OSC_ERR_ARG, "size overflow");
+    total += nbytes[i];
+    if (nbytes[i] > *maxn) *maxn = nbytes[i];
+  }
+  rc = osc_sym_range(dest, total, doff);
+  if (rc != OSC_OK) return rc;
+  rc = osc_sym_range(src, *maxn, soff);
+  if (rc != OSC_OK) return rc;
+  if (total > 0 && *maxn > 0 && overlap(*doff, total, *soff, *maxn))
+    return osc_set_error(OSC_ERR_ARG, "dest and source overlap");
+  log_call("collect", t, -1, -1, total);
+  return OSC_OK;
+}
+
 /* Variable-size collect (collect.c:406-492 closed form): the per-member
  * byte counts are known to the one host caller, so the exclusive prefix
  * sum the reference computes with a scan over the PEs (util/scan.c:10-60)
@@ -195,31 +234,41 @@ static int collect_launch(const osc_team *t, size_t doff, size_t soff,
 int osc_collect(void *team, void *dest, const void *src, const size_t *nbytes,
                 void *stream) {
   const osc_team *t = osc_team_get(team);
-  size_t doff, soff, total = 0, maxn = 0;
+  size_t doff, soff, maxn;
   void *s = stream ? stream : osc_g.stream;
-  int i, rc;
-  if (!osc_g.initialized) return osc_set_error(OSC_ERR_NOTINIT, "not initialised");
-  if (t == NULL) return osc_set_error(OSC_ERR_TEAM, "invalid team");
-  if (nbytes == NULL) return osc_set_error(OSC_ERR_ARG, "NULL size table");
-  for (i = 0; i < t->npes; ++i) {
-    if (nbytes[i] > ((size_t)-1) - total)
-      return osc_set_error(OSC_ERR_ARG, "size overflow");
-    total += nbytes[i];
-    if (nbytes[i] > maxn) maxn = nbytes[i];
-  }
-  rc = osc_sym_range(dest, total, &doff);
-  if (rc != OSC_OK) return rc;
-  rc = osc_sym_range(src, maxn, &soff);
-  if (rc != OSC_OK) return rc;
-  if (total > 0 && maxn > 0 && overlap(doff, total, soff, maxn))
-    return osc_set_error(OSC_ERR_ARG, "dest and source overlap");
-  log_call("collect", t, -1, -1, total);
-  if (maxn == 0) return OSC_OK;
+  int rc = collect_check(t, dest, src, nbytes, &doff, &soff, &maxn);
+  if (rc != OSC_OK || maxn == 0) return rc;
   pthread_mutex_lock(&tab_mu);
   rc = collect_launch(t, doff, soff, nbytes, maxn, s);
   pthread_mutex_unlock(&tab_mu);
   return rc;
 }
+
+/* The same collect without any allocation: the table goes into the
+ * library's static device array (OSCD_STATIC_TAB_MEMBERS members) through
+ * kernel arguments on the call's stream, so the call neither allocates
+ * nor copies from host memory.  Every call on the device shares that
+ * array: calls that overlap in time on one GPU (different streams or host
+ * threads without ordering) race on it. */
+int osc_collect_noalloc(void *team, void *dest, const void *src,
+                        const size_t *nbytes, void *stream) {
+  const osc_team *t = osc_team_get(team);
+  size_t doff, soff, maxn;
+  oscd_geom g;
+  int rc = collect_check(t, dest, src, nbytes, &doff, &soff, &maxn);
+  if (rc != OSC_OK) return rc;
+  if (t->npes > OSCD_STATIC_TAB_MEMBERS)
+    return osc_set_error(OSC_ERR_ARG,
+                         "team of %d members: the allocation-free collect takes"
+                         " at most %d", t->npes, OSCD_STATIC_TAB_MEMBERS);
+  if (maxn == 0) return OSC_OK;
+  osc_team_geom(t, &g);
+  rc = oscd_collect_static(&g, doff, soff, (const uint64_t *)nbytes, maxn,
+                           collect_unit(doff, soff, nbytes, t->npes),
+                           osc_g.device, stream ? stream : osc_g.stream);
+  return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "collect launch");
+}
+
 
 /* Strided alltoall (alltoalls.c:54-90 closed form); dst/sst in elements
  * of elem_bytes, both >= 1 as the reference's checks require

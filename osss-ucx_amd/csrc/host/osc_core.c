@@ -102,6 +102,9 @@ int osc_init(int npes, size_t heap_bytes_per_pe, int device) {
                          device, ndev);
   rc = oscd_set_device(device);
   if (rc != OSC_OK) return osc_hip_error(rc, "hipSetDevice");
+  /* the allocation-free collect's table: this device's address of it */
+  rc = oscd_scratch_init(device);
+  if (rc != OSC_OK) return osc_hip_error(rc, "hipGetSymbolAddress(scratch)");
   osc_g.slice = slice_for(heap_bytes_per_pe);
   rc = oscd_malloc(&base, (size_t)npes * osc_g.slice);
   if (rc != OSC_OK)

@@ -93,6 +93,19 @@ int oscd_collect(const oscd_geom *g, uint64_t dest_off, uint64_t src_off,
                  const uint64_t *d_tab, uint64_t max_n, int unit_width,
                  void *stream);
 
+/* Allocation-free collect: the (offset, size) table lives in a static
+ * device array of OSCD_STATIC_TAB_MEMBERS entries per GPU, whose address
+ * oscd_scratch_init(dev) records (once per device, with dev current).
+ * nbytes: host array of npes per-member byte counts; the table is written
+ * by small kernels on `stream` whose arguments carry it, then the collect
+ * runs on the same stream.  Calls that overlap on one GPU race on the
+ * table. */
+#define OSCD_STATIC_TAB_MEMBERS 4096
+int oscd_scratch_init(int dev);
+int oscd_collect_static(const oscd_geom *g, uint64_t dest_off,
+                        uint64_t src_off, const uint64_t *nbytes,
+                        uint64_t max_n, int unit_width, int dev, void *stream);
+
 /* strided alltoall of nelems elements of es bytes per member pair:
  * dest_j[((i*nelems + t)*dst)*es ..+es) = src_i[((j*nelems + t)*sst)*es ..)
  * for t < nelems (dst, sst in elements, >= 1). */

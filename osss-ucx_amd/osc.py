@@ -163,6 +163,7 @@ def lib() -> C.CDLL:
         "osc_last_error_code": (i, []),
         "osc_clear_error": (None, []),
         "osc_collect": (i, [vp, vp, vp, C.POINTER(sz), vp]),
+        "osc_collect_noalloc": (i, [vp, vp, vp, C.POINTER(sz), vp]),
         "osc_alltoalls": (i, [vp, vp, vp, pd, pd, sz, sz, vp]),
         "shmem_collectmem": (i, [vp, vp, vp, sz]),
         "shmem_alltoallsmem": (i, [vp, vp, vp, pd, pd, sz]),
