@@ -73,16 +73,33 @@ typedef enum {
   OSC_T_COUNT
 } osc_dtype_t;
 
-/* Float SUM/PROD accuracy mode.
+/* Float mode: the reduction algorithm of float and complex types.
  *  ACCURATE : FP32 accumulates in FP64, FP64 in double-double; one final
  *             rounding.  Order fixed (team index 0..P-1), so results are
  *             reproducible run to run and call to call.
  *  REFERENCE: the type's own arithmetic in exactly the combine order of the
  *             recursive-doubling reduction (fold of non-power-of-two
  *             members, then the XOR butterfly): bit-identical to it.
- * MIN/MAX always reproduce the butterfly's per-PE selection semantics;
- * integer ops are exact in both modes. */
-typedef enum { OSC_FP_ACCURATE = 0, OSC_FP_REFERENCE = 1 } osc_fpmode_t;
+ * MIN/MAX reproduce the butterfly's per-PE selection semantics in both;
+ * integer ops are exact in every mode. */
+typedef enum {
+  OSC_FP_ACCURATE = 0,
+  OSC_FP_REFERENCE = 1,
+  /* the reference's other reduction algorithms, each in its own combine
+   * order and the type's own arithmetic (integer results are the same
+   * under every mode):
+   *  LINEAR: team-order fold op(acc, x_i), acc = x_0 (reduce.c:55-91);
+   *  BINOMIAL: binomial tree, a member folding its children's results in
+   *    ascending order as op(child, own) (reduce.c:102-171; the reference
+   *    folds them in arrival order, so this is its result when every child
+   *    has reported before its parent folds);
+   *  RABENSEIFNER: reduce-scatter + allgather (reduce.c:306-724, both
+   *    variants): REFERENCE's bits for + and *; float MIN/MAX elements with
+   *    a NaN or signed-zero extremum take their block owner's tree. */
+  OSC_FP_LINEAR = 2,
+  OSC_FP_BINOMIAL = 3,
+  OSC_FP_RABENSEIFNER = 4
+} osc_fpmode_t;
 
 /* Synthetic-input distributions for osc_fill().  Every value is a pure
  * function of (seed, global PE, element index) through the SplitMix64

@@ -8,7 +8,10 @@ and adds independent numpy formulations used to pin it:
 * reduce_closed_form: the closed form of recursive doubling
   (src/shcoll/src/reduce.c:182-293) -- member k's result is the balanced
   pairwise tree over the folded leaves k^0, k^1, ..., k^(p2s-1) -- written
-  without simulating any message, per member;
+  without simulating any message, per member; linear_closed_form,
+  binomial_closed_form and rabenseifner_closed_form likewise for the
+  reference's other reductions (a left fold; a truncated pairwise tree with
+  the right subtree first; each element's block owner's tree);
 * hash3 / gen: the SplitMix64-keyed input generator in numpy;
 * fcollect / alltoall / broadcast / collect closed forms as numpy
   concatenations, alltoalls as one fancy-index gather.
@@ -43,6 +46,11 @@ def lib() -> C.CDLL:
         pp = C.POINTER(C.c_void_p)
         L.orc_reduce_rec_dbl.argtypes = [C.c_int, C.c_int, C.c_int,
                                          C.c_size_t, pp, pp]
+        for f in ("orc_reduce_linear", "orc_reduce_binomial"):
+            getattr(L, f).argtypes = [C.c_int, C.c_int, C.c_int, C.c_size_t,
+                                      pp, pp]
+        L.orc_reduce_rabenseifner.argtypes = [C.c_int, C.c_int, C.c_int,
+                                              C.c_size_t, C.c_int, pp, pp]
         L.orc_reduce_accurate.argtypes = [C.c_int, C.c_int, C.c_int,
                                           C.c_size_t, pp, C.c_void_p,
                                           C.c_void_p]
@@ -95,6 +103,37 @@ def rec_dbl(op: str, dtype: str, src: np.ndarray) -> np.ndarray:
     if rc != 0:
         raise ValueError(f"oracle rejects {op} on {dtype}")
     return dest
+
+
+def _reduce_with(fn, op, dtype, src, *extra):
+    src = np.ascontiguousarray(src, dtype=NP_DTYPE[dtype])
+    P, n = src.shape
+    dest = np.zeros_like(src)
+    rc = fn(OPS[op], DTYPES[dtype], P, n, *extra,
+            _ptrs([src[i] for i in range(P)]),
+            _ptrs([dest[i] for i in range(P)]))
+    if rc != 0:
+        raise ValueError(f"oracle rejects {op} on {dtype}")
+    return dest
+
+
+def linear(op: str, dtype: str, src: np.ndarray) -> np.ndarray:
+    """Reference linear reduce (reduce.c:55-91) of src[P, n]."""
+    return _reduce_with(lib().orc_reduce_linear, op, dtype, src)
+
+
+def binomial(op: str, dtype: str, src: np.ndarray) -> np.ndarray:
+    """Reference binomial reduce (reduce.c:102-171), children taken in
+    ascending order (every child reported before its parent polls)."""
+    return _reduce_with(lib().orc_reduce_binomial, op, dtype, src)
+
+
+def rabenseifner(op: str, dtype: str, src: np.ndarray,
+                 ring: bool = False) -> np.ndarray:
+    """Reference Rabenseifner reduce (reduce.c:306-511), or its ring-
+    allgather variant rabenseifner2 (reduce.c:521-724) with ring=True."""
+    return _reduce_with(lib().orc_reduce_rabenseifner, op, dtype, src,
+                        int(ring))
 
 
 def accurate(op: str, dtype: str, src: np.ndarray):
@@ -295,10 +334,19 @@ def gen_int_range(seed, pes, n, lo, hi, dtype, idx=None) -> np.ndarray:
 
 
 def _bin(op, a, b):
-    """reduce.c:730-736 on numpy arrays of one dtype (elementwise)."""
+    """reduce.c:730-736 on numpy arrays of one dtype (elementwise).  A
+    complex product is C99's finite formula (ac - bd) + (ad + bc)i in real
+    arithmetic: numpy's own complex multiply loop does not round the same
+    way in every last bit."""
     if op == "sum":
         return a + b
     if op == "prod":
+        if np.iscomplexobj(a):
+            ar, ai, br, bi = a.real, a.imag, b.real, b.imag
+            out = np.empty(np.broadcast(a, b).shape, a.dtype)
+            out.real = ar * br - ai * bi
+            out.imag = ar * bi + ai * br
+            return out
         return a * b
     if op == "max":
         return np.where(a > b, a, b)
@@ -338,6 +386,76 @@ def reduce_closed_form(op: str, src: np.ndarray) -> np.ndarray:
             if m[k + 1] - m[k] == 2:
                 out[m[k] + 1] = level[0]
     return out
+
+
+def linear_closed_form(op: str, src: np.ndarray) -> np.ndarray:
+    """Linear reduce (reduce.c:55-91) as a left fold over the team,
+    acc = op(acc, x_i), the same value on every member."""
+    with np.errstate(all="ignore"):
+        acc = src[0].copy()
+        for i in range(1, src.shape[0]):
+            acc = _bin(op, acc, src[i])
+    return np.tile(acc, (src.shape[0], 1))
+
+
+def binomial_closed_form(op: str, src: np.ndarray) -> np.ndarray:
+    """Binomial reduce (reduce.c:102-171) with children taken in ascending
+    order, written as a tree: the node of the aligned block [lo, lo + s) is
+    op(node of its right half, node of its left half), a right half that
+    starts at or past P is absent; the root covers [0, 2^ceil(log2 P))."""
+    P = src.shape[0]
+
+    def node(lo, size):
+        if size == 1:
+            return src[lo]
+        half = size // 2
+        left = node(lo, half)
+        if lo + half >= P:
+            return left
+        return _bin(op, node(lo + half, half), left)
+
+    top = 1
+    while top < P:
+        top *= 2
+    with np.errstate(all="ignore"):
+        r = node(0, top)
+    return np.tile(r, (P, 1))
+
+
+def rabenseifner_closed_form(op: str, src: np.ndarray) -> np.ndarray:
+    """Rabenseifner reduce (reduce.c:306-511, :521-724): element e of n
+    belongs to block b, the largest b with floor(b*n/p2s) <= e, which the
+    reduce-scatter leaves on member k = reverse_bits(b); its value is
+    member k's recursive-doubling tree, whose folded leaves are
+    op(x[m+1], x[m]) below n//2 and op(x[m], x[m+1]) from n//2 on; every
+    member (and folded neighbour) receives it."""
+    P, n = src.shape
+    p2s, lg = 1, 0
+    while p2s * 2 <= P:
+        p2s, lg = p2s * 2, lg + 1
+    m = [-(-k * P // p2s) for k in range(p2s + 1)]
+    upper = np.arange(n) >= n // 2
+    with np.errstate(all="ignore"):
+        leaves = []
+        for k in range(p2s):
+            if m[k + 1] - m[k] == 2:
+                a, b = src[m[k]], src[m[k] + 1]
+                leaves.append(np.where(upper, _bin(op, a, b), _bin(op, b, a)))
+            else:
+                leaves.append(src[m[k]])
+        per_member = []
+        for k in range(p2s):
+            level = [leaves[k ^ j] for j in range(p2s)]
+            while len(level) > 1:
+                level = [_bin(op, level[2 * t], level[2 * t + 1])
+                         for t in range(len(level) // 2)]
+            per_member.append(level[0])
+    out = np.empty(n, src.dtype)
+    bounds = [(b * n) // p2s for b in range(p2s + 1)]
+    for b in range(p2s):
+        k = int(format(b, f"0{lg}b")[::-1], 2) if lg else 0
+        out[bounds[b]:bounds[b + 1]] = per_member[k][bounds[b]:bounds[b + 1]]
+    return np.tile(out, (P, 1))
 
 
 def fcollect_closed_form(src: np.ndarray) -> np.ndarray:

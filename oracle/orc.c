@@ -182,6 +182,211 @@ out:
   return rc;
 }
 
+/* ---- REDUCE_HELPER_LINEAR, reduce.c:55-91: the root (active-set index
+ *      0) copies its source to tmp, then for i = 1..P-1 gets PE i's source
+ *      into its dest and computes tmp = op(tmp, dest); its dest = tmp is
+ *      then broadcast (shcoll_broadcast8_linear) to every PE. ---- */
+int orc_reduce_linear(int op, int dtype, int P, size_t n,
+                      const void *const *src, void *const *dest) {
+  const local_fn f = pick_local(dtype);
+  const size_t nb = n * esize(dtype);
+  char *tmp;
+  int i, rc = 0;
+  if (f == NULL || P < 1) return -1;
+  tmp = (char *)malloc(nb ? nb : 1);
+  if (tmp == NULL) return -1;
+  memcpy(tmp, src[0], nb);
+  for (i = 1; i < P; ++i) {
+    memcpy(dest[0], src[i], nb);
+    rc |= f(op, tmp, tmp, dest[0], n);
+  }
+  memcpy(dest[0], tmp, nb);
+  for (i = 1; i < P; ++i) memcpy(dest[i], dest[0], nb);
+  free(tmp);
+  return rc;
+}
+
+/* ---- REDUCE_HELPER_BINOMIAL, reduce.c:102-171.  PE a copies its source
+ *      to dest, then receives from each child a | mask (mask below a's
+ *      lowest set bit, a | mask < P): tmp = dest; dest = the child's dest
+ *      (getmem); dest = op(dest, tmp).  The reference takes the children
+ *      in arrival order (the lowest bit of the arrived ones first); this
+ *      restatement is the schedule in which every child has finished and
+ *      reported before its parent's first poll, so the parent takes them in
+ *      ascending order.  PEs are run from the highest index down, which
+ *      finishes every child before its parent.  Root's dest is then
+ *      broadcast (shcoll_broadcast8_binomial_tree). ---- */
+int orc_reduce_binomial(int op, int dtype, int P, size_t n,
+                        const void *const *src, void *const *dest) {
+  const local_fn f = pick_local(dtype);
+  const size_t nb = n * esize(dtype);
+  char *tmp;
+  int me, rc = 0;
+  if (f == NULL || P < 1) return -1;
+  tmp = (char *)malloc(nb ? nb : 1);
+  if (tmp == NULL) return -1;
+  for (me = P - 1; me >= 0; --me) {
+    unsigned mask;
+    long to_receive = 0;
+    memcpy(dest[me], src[me], nb);
+    for (mask = 1; !(me & mask) && ((me | mask) < (unsigned)P); mask <<= 1)
+      to_receive |= (long)mask;
+    while (to_receive != 0) {
+      const long recv = to_receive & -to_receive; /* lowest pending child */
+      memcpy(tmp, dest[me], nb);
+      memcpy(dest[me], dest[me | (int)recv], nb);
+      rc |= f(op, dest[me], dest[me], tmp, n);
+      to_receive &= ~recv;
+    }
+  }
+  for (me = 1; me < P; ++me) memcpy(dest[me], dest[0], nb);
+  free(tmp);
+  return rc;
+}
+
+/* bit reversal of the lowest nbits bits (bithacks.c:11 reverse_bits) */
+static int rev_bits(int x, int nbits) {
+  int r = 0, i;
+  for (i = 0; i < nbits; ++i) r |= ((x >> i) & 1) << (nbits - 1 - i);
+  return r;
+}
+
+/* ---- REDUCE_HELPER_RABENSEIFNER, reduce.c:306-511 (ring = 0), and
+ *      REDUCE_HELPER_RABENSEIFNER2, reduce.c:521-724 (ring = 1), for all
+ *      PEs at once:
+ *  1. fold of non-power-of-two pairs (:362-404): the non-member u = m+1
+ *     gets the member's upper half of source into its dest, computes
+ *     op(x_m, x_u) there and puts it back to the member's dest; the
+ *     member gets u's lower half of source into its dest and computes
+ *     op(x_u, x_m); other members copy their source;
+ *  2. reduce-scatter by recursive halving (:409-446): in the round of
+ *     distance d a member keeps the lower half of its block range when
+ *     bit d of its index is clear, the upper half otherwise, reads the
+ *     partner's dest on that half and computes dest = op(dest, partner's);
+ *  3. allgather: recursive doubling over the block ranges starting at
+ *     block reverse_bits(k) (:452-480), or a ring in which member k sends
+ *     block reverse_bits(k - i) to member k+1 in step i (:652-676);
+ *  4. the member puts its whole dest to the folded neighbour (:482-494).
+ *  Block b covers elements [b*n/p2s, (b+1)*n/p2s). ---- */
+int orc_reduce_rabenseifner(int op, int dtype, int P, size_t n, int ring,
+                            const void *const *src, void *const *dest) {
+  const local_fn f = pick_local(dtype);
+  const size_t es = esize(dtype), nb = n * es;
+  int p2s, lg, me, d, i, rc = 0;
+  int *mp, *bb, *be;
+  char *tmp;
+  if (f == NULL || P < 1) return -1;
+  p2s = p2s_of(P);
+  for (lg = 0; (1 << lg) < p2s; ++lg) {
+  }
+  mp = (int *)malloc(sizeof(int) * (size_t)P);
+  bb = (int *)malloc(sizeof(int) * (size_t)P);
+  be = (int *)malloc(sizeof(int) * (size_t)P);
+  tmp = (char *)malloc((size_t)P * nb > 0 ? (size_t)P * nb : 1);
+  if (mp == NULL || bb == NULL || be == NULL || tmp == NULL) {
+    rc = -1;
+    goto out;
+  }
+  for (me = 0; me < P; ++me) mp[me] = member_p2s(me, P, p2s);
+  /* 1. fold */
+  for (me = 0; me < P; ++me) {
+    const size_t h = n / 2;
+    if (mp[me] < 0) {
+      const int m = me - 1;
+      memcpy((char *)dest[me] + h * es, (const char *)src[m] + h * es,
+             (n - h) * es);
+      rc |= f(op, (char *)dest[me] + h * es, (char *)dest[me] + h * es,
+              (const char *)src[me] + h * es, n - h);
+      memcpy((char *)dest[m] + h * es, (char *)dest[me] + h * es,
+             (n - h) * es);
+    } else if ((me + 1) * p2s / P == mp[me]) {
+      memcpy(dest[me], src[me + 1], h * es);
+      rc |= f(op, dest[me], dest[me], src[me], h);
+    } else {
+      memcpy(dest[me], src[me], nb);
+    }
+  }
+  /* 2. reduce-scatter; every read of a round precedes its combines */
+  for (me = 0; me < P; ++me) {
+    bb[me] = 0;
+    be[me] = p2s;
+  }
+  for (d = 1; d < p2s; d <<= 1) {
+    for (me = 0; me < P; ++me) {
+      int peer;
+      size_t o, o2;
+      if (mp[me] < 0) continue;
+      if ((mp[me] & d) == 0)
+        be[me] = (bb[me] + be[me]) / 2;
+      else
+        bb[me] = (bb[me] + be[me]) / 2;
+      peer = ((mp[me] ^ d) * P + p2s - 1) / p2s;
+      o = ((size_t)bb[me] * n) / (size_t)p2s;
+      o2 = ((size_t)be[me] * n) / (size_t)p2s;
+      memcpy(tmp + (size_t)me * nb, (const char *)dest[peer] + o * es,
+             (o2 - o) * es);
+    }
+    for (me = 0; me < P; ++me) {
+      size_t o, o2;
+      if (mp[me] < 0) continue;
+      o = ((size_t)bb[me] * n) / (size_t)p2s;
+      o2 = ((size_t)be[me] * n) / (size_t)p2s;
+      rc |= f(op, (char *)dest[me] + o * es, (char *)dest[me] + o * es,
+              tmp + (size_t)me * nb, o2 - o);
+    }
+  }
+  /* 3. allgather of the owned blocks */
+  if (!ring) {
+    for (me = 0; me < P; ++me) {
+      if (mp[me] < 0) continue;
+      bb[me] = rev_bits(mp[me], lg);
+      be[me] = bb[me] + 1;
+    }
+    for (d = p2s / 2; d > 0; d >>= 1) {
+      for (me = 0; me < P; ++me) { /* puts of this round (disjoint) */
+        int peer;
+        size_t o, o2;
+        if (mp[me] < 0) continue;
+        peer = ((mp[me] ^ d) * P + p2s - 1) / p2s;
+        o = ((size_t)bb[me] * n) / (size_t)p2s;
+        o2 = ((size_t)be[me] * n) / (size_t)p2s;
+        memcpy((char *)dest[peer] + o * es, (char *)dest[me] + o * es,
+               (o2 - o) * es);
+      }
+      for (me = 0; me < P; ++me) {
+        if (mp[me] < 0) continue;
+        if ((mp[me] & d) == 0)
+          be[me] += be[me] - bb[me];
+        else
+          bb[me] -= be[me] - bb[me];
+      }
+    }
+  } else {
+    for (i = 0; i < p2s; ++i)
+      for (me = 0; me < P; ++me) {
+        int peer, blk;
+        size_t o, o2;
+        if (mp[me] < 0) continue;
+        peer = (((mp[me] + 1) % p2s) * P + p2s - 1) / p2s;
+        blk = rev_bits((mp[me] - i + p2s) % p2s, lg);
+        o = ((size_t)blk * n) / (size_t)p2s;
+        o2 = ((size_t)(blk + 1) * n) / (size_t)p2s;
+        memcpy((char *)dest[peer] + o * es, (char *)dest[me] + o * es,
+               (o2 - o) * es);
+      }
+  }
+  /* 4. the folded neighbour gets its member's result */
+  for (me = 0; me < P; ++me)
+    if (mp[me] >= 0 && (me + 1) * p2s / P == mp[me])
+      memcpy(dest[me + 1], dest[me], nb);
+out:
+  free(mp);
+  free(bb);
+  free(be);
+  free(tmp);
+  return rc;
+}
+
 /* ---- high-precision references ------------------------------------------ */
 
 /* Shewchuk's exact expansion sum with the correctly rounded final step, as

@@ -5,6 +5,9 @@
  * for all PEs of a team inside one process (each PE = one host buffer),
  * step by step as the reference executes them:
  *   orc_reduce_rec_dbl   src/shcoll/src/reduce.c:182-293 (+ :35-43, :730-736)
+ *   orc_reduce_linear    src/shcoll/src/reduce.c:55-91
+ *   orc_reduce_binomial  src/shcoll/src/reduce.c:102-171
+ *   orc_reduce_rabenseifner  src/shcoll/src/reduce.c:306-511, :521-724
  *   orc_fcollect_bruck   src/shcoll/src/fcollect.c:370-403 + util/rotate.c:22-54
  *   orc_alltoall_shift   src/shcoll/src/alltoall.c:75-110, SHIFT_PEER :206
  *   orc_broadcast_linear src/shcoll/src/broadcast.c:58-71, :527-531
@@ -37,6 +40,16 @@ extern "C" {
 /* Reference reduce: src[i], dest[i] are PE i's buffers of n elements. */
 int orc_reduce_rec_dbl(int op, int dtype, int npes, size_t n,
                        const void *const *src, void *const *dest);
+/* the reference's other reductions (reduce.c:55-91, :102-171, :306-724);
+ * binomial in the schedule where children report before their parent
+ * polls; rabenseifner with the recursive-doubling (ring = 0) or the ring
+ * (ring = 1, rabenseifner2) allgather */
+int orc_reduce_linear(int op, int dtype, int npes, size_t n,
+                      const void *const *src, void *const *dest);
+int orc_reduce_binomial(int op, int dtype, int npes, size_t n,
+                        const void *const *src, void *const *dest);
+int orc_reduce_rabenseifner(int op, int dtype, int npes, size_t n, int ring,
+                            const void *const *src, void *const *dest);
 
 /* High-precision SUM/PROD of the npes inputs per element.  For complex
  * types the outputs hold 2n values (re, im interleaved). */

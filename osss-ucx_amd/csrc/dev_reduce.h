@@ -147,11 +147,24 @@ struct PolSeq {
   static constexpr int SHAPE = 0;  // default kernel shape
   static constexpr int WAVES81 = 0;
   static constexpr bool SPECIAL = SPEC;
+  static constexpr bool BINOM = false;
+  static constexpr bool RAB = false;
   static constexpr int OPC = OP;
   __device__ static Acc lift(T x) { return x; }
   __device__ static Acc seq(Acc a, T x) { return Bin<OP>::f(a, x); }
   __device__ static Acc comb(Acc a, Acc b) { return Bin<OP>::f(a, b); }
   __device__ static T out(Acc a) { return a; }
+};
+
+// Float MIN/MAX in the reference Rabenseifner reductions (reduce.c:
+// 306-511, 521-724): an element's result is its owner's recursive-doubling
+// tree (the owner of block b of the reduce-scatter is member
+// reverse_bits(b)), copied to every member; in the fold of a
+// non-power-of-two pair the upper half of the elements combine as
+// op(x[m], x[m+1]).  Only NaN and signed-zero elements can tell.
+template <class T, int OP>
+struct PolSeqRab : PolSeq<T, OP, true> {
+  static constexpr bool RAB = true;
 };
 
 // The type's own arithmetic in the reference's tree order.
@@ -164,11 +177,25 @@ struct PolTree {
   static constexpr int SHAPE = std::is_same<T, double>::value ? 3 : 2;
   static constexpr int WAVES81 = 0;
   static constexpr bool SPECIAL = false;
+  static constexpr bool BINOM = false;
+  static constexpr bool RAB = false;
   static constexpr int OPC = OP;
   __device__ static Acc lift(T x) { return x; }
   __device__ static Acc seq(Acc a, T x) { return Bin<OP>::f(a, x); }
   __device__ static Acc comb(Acc a, Acc b) { return Bin<OP>::f(a, b); }
   __device__ static T out(Acc a) { return a; }
+};
+
+// The type's own arithmetic in the reference binomial reduction's order
+// (reduce.c:102-171): member a folds each child a + 2^j (2^j below a's
+// lowest set bit, a + 2^j < P) as op(child's result, own partial), the
+// children in ascending order, which is the order of the reference when
+// every child has reported before its parent folds (the reference folds
+// children in arrival order).  That is a pairwise tree over all P members
+// in team order, node = op(right subtree, left subtree), truncated at P.
+template <class T, int OP>
+struct PolBinom : PolTree<T, OP> {
+  static constexpr bool BINOM = true;
 };
 
 // FP32 SUM/PROD accumulated in FP64, rounded once.
@@ -179,6 +206,8 @@ struct PolF32Acc {
   static constexpr int SHAPE = 1;  // default kernel shape
   static constexpr int WAVES81 = 0;
   static constexpr bool SPECIAL = false;
+  static constexpr bool BINOM = false;
+  static constexpr bool RAB = false;
   static constexpr int OPC = OP;
   __device__ static Acc lift(float x) { return (double)x; }
   __device__ static Acc seq(Acc a, float x) {
@@ -206,6 +235,8 @@ struct PolF64Acc {
   static constexpr int SHAPE = OP == OSC_OP_SUM ? 1 : 0;  // default shape
   static constexpr int WAVES81 = OP == OSC_OP_SUM ? 4 : 0;
   static constexpr bool SPECIAL = false;
+  static constexpr bool BINOM = false;
+  static constexpr bool RAB = false;
   static constexpr int OPC = OP;
   __device__ static Acc lift(double x) { return {x, 0.0}; }
   __device__ static Acc seq(Acc a, double x) {
@@ -233,6 +264,8 @@ struct PolC32Acc {
   static constexpr int SHAPE = 0;  // default kernel shape
   static constexpr int WAVES81 = 0;
   static constexpr bool SPECIAL = false;
+  static constexpr bool BINOM = false;
+  static constexpr bool RAB = false;
   static constexpr int OPC = OP;
   __device__ static Acc lift(cf32 x) { return {(double)x.re, (double)x.im}; }
   __device__ static Acc seq(Acc a, cf32 x) {
@@ -256,6 +289,8 @@ struct PolC64Acc {
   static constexpr int SHAPE = 0;  // default kernel shape
   static constexpr int WAVES81 = 0;
   static constexpr bool SPECIAL = false;
+  static constexpr bool BINOM = false;
+  static constexpr bool RAB = false;
   static constexpr int OPC = OP;
   __device__ static Acc lift(cf64 x) { return {{x.re, 0.0}, {x.im, 0.0}}; }
   __device__ static Acc seq(Acc a, cf64 x) {
@@ -342,6 +377,48 @@ __device__ __forceinline__ int member_of(int k, int P, int lg) {
   return (int)(((int64_t)k * P + ((1 << lg) - 1)) >> lg);
 }
 
+// The lowest lg bits of k in reverse order (the reference's reverse_bits,
+// src/shcoll/src/util/bithacks.c:11).
+__device__ __forceinline__ int rev_bits(int k, int lg) {
+  return lg == 0 ? 0 : (int)(__builtin_bitreverse32((uint32_t)k) >> (32 - lg));
+}
+
+// Rabenseifner's owner of element e of n: the reduce-scatter leaves block
+// b = [floor(b*n/p2s), floor((b+1)*n/p2s)) on member reverse_bits(b)
+// (reduce.c:411-446, collect phase :452-455), so e belongs to the largest
+// b with floor(b*n/p2s) <= e.
+__device__ __forceinline__ int rab_owner(uint64_t e, uint64_t n, int lg) {
+  const uint64_t p2s = 1ull << lg;
+  const uint64_t b = ((e + 1) * p2s + n - 1) / n - 1;
+  return rev_bits((int)b, lg);
+}
+
+// Member k's recursive-doubling result over snap[0..P) (leaves in the
+// XOR-permuted order of its butterfly, reduce.c:250-272).  A folded pair
+// is op(x[m+1], x[m]) (reduce.c:232-241), or op(x[m], x[m+1]) when swap
+// (the upper half of a Rabenseifner fold, reduce.c:370-382).
+template <class T, int OP>
+__device__ T member_tree(const T *snap, int k, int P, int lg, bool swap) {
+  const int p2s = 1 << lg;
+  T stk[16];
+  for (int j = 0; j < p2s; ++j) {
+    const int kk = k ^ j;
+    const int m = member_of(kk, P, lg);
+    const int m1 = member_of(kk + 1, P, lg);
+    T leaf = snap[m];
+    if (m1 - m == 2)
+      leaf = swap ? Bin<OP>::f(snap[m], snap[m + 1])
+                  : Bin<OP>::f(snap[m + 1], snap[m]);
+    int L = 0;
+    while ((j >> L) & 1) {
+      leaf = Bin<OP>::f(stk[L], leaf);
+      ++L;
+    }
+    stk[L] = leaf;
+  }
+  return stk[lg];
+}
+
 // ---- exact per-member MIN/MAX for special elements ----------------------
 // Cooperative over the workgroup: each special element is handled in turn.
 // Its P inputs are copied into LDS before any result is written (so
@@ -349,12 +426,15 @@ __device__ __forceinline__ int member_of(int k, int P, int lg) {
 // over the LDS copy and writes their results (and the folded neighbour's).
 // `mine` has bit k*VEC+v set for element v of the thread's k-th unit
 // (unit index ubase + k*256).
-template <class T, int OP, int VEC>
+// RAB: the Rabenseifner semantics instead, one owner's result to every
+// member.
+template <class T, int OP, int VEC, bool RAB>
 __device__ void special_block(const oscd_geom &g, uint64_t src_off,
                               uint64_t dest_off, uint64_t ubase,
-                              uint32_t mine, int lg, T *snap) {
+                              uint32_t mine, int lg, uint64_t n, T *snap) {
   __shared__ int s_owner;
   __shared__ unsigned long long s_e;
+  __shared__ T s_r;
   const int P = g.npes;
   const int p2s = 1 << lg;
   for (;;) {
@@ -374,26 +454,22 @@ __device__ void special_block(const oscd_geom &g, uint64_t src_off,
     for (int i = threadIdx.x; i < P; i += blockDim.x)
       snap[i] = *(const T *)(member_ptr(g, i, src_off) + e * sizeof(T));
     __syncthreads();
-    for (int k = threadIdx.x; k < p2s; k += blockDim.x) {
-      T stk[16];
-      for (int j = 0; j < p2s; ++j) {
-        const int kk = k ^ j;
-        const int m = member_of(kk, P, lg);
-        const int m1 = member_of(kk + 1, P, lg);
-        T leaf = (m1 - m == 2) ? Bin<OP>::f(snap[m + 1], snap[m]) : snap[m];
-        int L = 0;
-        while ((j >> L) & 1) {
-          leaf = Bin<OP>::f(stk[L], leaf);
-          ++L;
-        }
-        stk[L] = leaf;
+    if constexpr (RAB) {
+      if (threadIdx.x == 0)
+        s_r = member_tree<T, OP>(snap, rab_owner(e, n, lg), P, lg, e >= n / 2);
+      __syncthreads();
+      const T r = s_r;
+      for (int i = threadIdx.x; i < P; i += blockDim.x)
+        *(T *)(member_ptr(g, i, dest_off) + e * sizeof(T)) = r;
+    } else {
+      for (int k = threadIdx.x; k < p2s; k += blockDim.x) {
+        const T r = member_tree<T, OP>(snap, k, P, lg, false);
+        const int mk = member_of(k, P, lg);
+        const int mk1 = member_of(k + 1, P, lg);
+        *(T *)(member_ptr(g, mk, dest_off) + e * sizeof(T)) = r;
+        if (mk1 - mk == 2)
+          *(T *)(member_ptr(g, mk + 1, dest_off) + e * sizeof(T)) = r;
       }
-      const T r = stk[lg];
-      const int mk = member_of(k, P, lg);
-      const int mk1 = member_of(k + 1, P, lg);
-      *(T *)(member_ptr(g, mk, dest_off) + e * sizeof(T)) = r;
-      if (mk1 - mk == 2)
-        *(T *)(member_ptr(g, mk + 1, dest_off) + e * sizeof(T)) = r;
     }
     __syncthreads();
   }
@@ -546,7 +622,13 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
       auto eat = [&](const T (&b)[UPT][VEC], const int i) {
         Acc leaf[UPT][VEC];
         bool ready = true;
-        if (i == mk) {
+        if constexpr (Pol::BINOM) {
+          // binomial order: every member is a leaf, no fold
+#pragma unroll
+          for (int k = 0; k < UPT; ++k)
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) leaf[k][v] = Pol::lift(b[k][v]);
+        } else if (i == mk) {
           if (mn - mk == 2) {
 #pragma unroll
             for (int k = 0; k < UPT; ++k)
@@ -574,11 +656,14 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
               const bool occupied = ((kl >> L) & 1) != 0;
               const bool place = carry && !occupied;
               if (carry && occupied) {
+                // recursive doubling: op(left, right); binomial: op(right,
+                // left), the child's result first
 #pragma unroll
                 for (int k = 0; k < UPT; ++k)
 #pragma unroll
                   for (int v = 0; v < VEC; ++v)
-                    leaf[k][v] = Pol::comb(stk[L][k][v], leaf[k][v]);
+                    leaf[k][v] = Pol::BINOM ? Pol::comb(leaf[k][v], stk[L][k][v])
+                                            : Pol::comb(stk[L][k][v], leaf[k][v]);
               }
 #pragma unroll
               for (int k = 0; k < UPT; ++k)
@@ -589,8 +674,10 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
             }
           }
           ++kl;
-          mk = mn;
-          mn = member_of(kl + 1, P, lg);
+          if constexpr (!Pol::BINOM) {
+            mk = mn;
+            mn = member_of(kl + 1, P, lg);
+          }
         }
       };
       // groups of INFL slices with unconditional loads, then the leftover
@@ -619,14 +706,38 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
             load_unit<T, VEC>(sp + loff[k], buf[k], FAST || full[k], cnt[k]);
         eat(buf, i);
       }
+      if constexpr (Pol::BINOM) {
+        // P leaves leave one subtree per set bit of P (the highest is the
+        // left-most): merge them from the lowest up, each as op(what is
+        // to its right, it)
+        bool have = false;
 #pragma unroll
-      for (int L = 0; L < MAXLV; ++L)
-        if (L == lg) {
+        for (int L = 0; L < MAXLV; ++L)
+          if (L <= lg && ((P >> L) & 1)) {
+            if (have) {  // uniform: depends on P and L only
 #pragma unroll
-          for (int k = 0; k < UPT; ++k)
+              for (int k = 0; k < UPT; ++k)
 #pragma unroll
-            for (int v = 0; v < VEC; ++v) acc[k][v] = stk[L][k][v];
-        }
+                for (int v = 0; v < VEC; ++v)
+                  acc[k][v] = Pol::comb(acc[k][v], stk[L][k][v]);
+            } else {
+#pragma unroll
+              for (int k = 0; k < UPT; ++k)
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) acc[k][v] = stk[L][k][v];
+            }
+            have = true;
+          }
+      } else {
+#pragma unroll
+        for (int L = 0; L < MAXLV; ++L)
+          if (L == lg) {
+#pragma unroll
+            for (int k = 0; k < UPT; ++k)
+#pragma unroll
+              for (int v = 0; v < VEC; ++v) acc[k][v] = stk[L][k][v];
+          }
+      }
 #pragma unroll
       for (int k = 0; k < UPT; ++k)
 #pragma unroll
@@ -654,8 +765,8 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
   // wave of the workgroup had loaded all P slices.
   if constexpr (Pol::SPECIAL) {
     if (__syncthreads_or(spec != 0))
-      special_block<T, Pol::OPC, VEC>(g, src_off, dest_off, ubase, spec, lg,
-                                      lds);
+      special_block<T, Pol::OPC, VEC, Pol::RAB>(g, src_off, dest_off, ubase,
+                                                spec, lg, n, lds);
   }
 }
 
@@ -911,6 +1022,32 @@ __global__ __launch_bounds__(kBlock) void k_reduce_gather(
       constexpr int kLv = 11;
       const int p2s = 1 << lg;
       typename Pol::Acc stk[kLv];
+      if constexpr (Pol::BINOM) {
+        // the binomial order: all P members are leaves, a node is
+        // op(right subtree, left subtree), the subtrees left by a
+        // non-power-of-two P merged from the lowest up (as in the walk)
+        for (int j = 0; j < P; ++j) {
+          typename Pol::Acc leaf = Pol::lift(value(j));
+          bool carry = true;
+#pragma unroll
+          for (int L = 0; L < kLv; ++L) {
+            const bool occupied = ((j >> L) & 1) != 0;
+            if (carry && occupied) leaf = Pol::comb(leaf, stk[L]);
+            stk[L] = pick(carry && !occupied, leaf, stk[L]);
+            carry = carry && occupied;
+          }
+        }
+        bool have = false;
+#pragma unroll
+        for (int L = 0; L < kLv; ++L)
+          if ((P >> L) & 1) {
+            if (have)
+              acc = Pol::comb(acc, stk[L]);
+            else
+              acc = stk[L];
+            have = true;
+          }
+      } else {
       for (int j = 0; j < p2s; ++j) {
         const int m = member_of(j, P, lg);
         const int m1 = member_of(j + 1, P, lg);
@@ -929,6 +1066,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_gather(
 #pragma unroll
       for (int L = 0; L < kLv; ++L)
         if (L == lg) acc = stk[L];
+      }
     } else {
       acc = Pol::lift(value(0));
       nan = is_nan(value(0));
@@ -948,26 +1086,22 @@ __global__ __launch_bounds__(kBlock) void k_reduce_gather(
       // as the slice walk's special path does (the element is the same
       // for the whole wave, so this branch is uniform)
       const int p2s = 1 << lg;
-      for (int k = lane; k < p2s; k += 64) {
-        T stk[16];
-        for (int j = 0; j < p2s; ++j) {
-          const int kk = k ^ j;
-          const int m = member_of(kk, P, lg);
-          const int m1 = member_of(kk + 1, P, lg);
-          T leaf = (m1 - m == 2) ? Bin<Pol::OPC>::f(value(m + 1), value(m))
-                                 : value(m);
-          int L = 0;
-          while ((j >> L) & 1) {
-            leaf = Bin<Pol::OPC>::f(stk[L], leaf);
-            ++L;
-          }
-          stk[L] = leaf;
+      const T *snap = (const T *)x;  // float or double: the raw words
+      if constexpr (Pol::RAB) {
+        // Rabenseifner: the owner's tree, to every member
+        const T rk = member_tree<T, Pol::OPC>(snap, rab_owner(e, n, lg), P,
+                                              lg, e >= n / 2);
+        for (int j = lane; j < P; j += 64)
+          *(T *)(member_ptr(g, j, dest_off) + eb) = rk;
+      } else {
+        for (int k = lane; k < p2s; k += 64) {
+          const T rk = member_tree<T, Pol::OPC>(snap, k, P, lg, false);
+          const int mk = member_of(k, P, lg);
+          const int mk1 = member_of(k + 1, P, lg);
+          *(T *)(member_ptr(g, mk, dest_off) + eb) = rk;
+          if (mk1 - mk == 2)
+            *(T *)(member_ptr(g, mk + 1, dest_off) + eb) = rk;
         }
-        const T rk = stk[lg];
-        const int mk = member_of(k, P, lg);
-        const int mk1 = member_of(k + 1, P, lg);
-        *(T *)(member_ptr(g, mk, dest_off) + eb) = rk;
-        if (mk1 - mk == 2) *(T *)(member_ptr(g, mk + 1, dest_off) + eb) = rk;
       }
       }
     }
@@ -1234,21 +1368,49 @@ static int visit_int_t(int op, V &v) {
   return OSC_ERR_ARG;
 }
 
-// MIN/MAX: exact selection, per-member results on NaN / signed-zero ties.
-// SUM/PROD: reference-order tree (mode 1) or accurate accumulation.
+// SUM/PROD of a float or complex type by algorithm (float mode):
+//   accurate (hip): wider accumulation, rounded once;
+//   rec_dbl and rabenseifner: the recursive-doubling tree in the type's
+//     own arithmetic (Rabenseifner's reduce-scatter gives each element its
+//     owner's butterfly, whose + and * results equal every member's);
+//   linear: team-order fold op(acc, x_i) (reduce.c:55-91);
+//   binomial: the binomial tree (PolBinom).
+template <class T, template <int> class AccPol, int OP, class V>
+static int visit_arith(int mode, V &v) {
+  switch (mode) {
+    case OSC_FP_ACCURATE: return v.template run<T, AccPol<OP>>();
+    case OSC_FP_LINEAR: return v.template run<T, PolSeq<T, OP, false>>();
+    case OSC_FP_BINOMIAL: return v.template run<T, PolBinom<T, OP>>();
+    case OSC_FP_REFERENCE:
+    case OSC_FP_RABENSEIFNER: return v.template run<T, PolTree<T, OP>>();
+  }
+  return OSC_ERR_ARG;
+}
+
+// MIN/MAX of a float type: exact selection everywhere except elements with
+// a NaN or a signed-zero extremum, where the operand order of the ternary
+// decides: per-member recursive-doubling trees (accurate, rec_dbl), the
+// owner's tree (rabenseifner), the team-order fold (linear) or the
+// binomial tree, each as its reference algorithm leaves it.
+template <class T, int OP, class V>
+static int visit_sel(int mode, V &v) {
+  switch (mode) {
+    case OSC_FP_ACCURATE:
+    case OSC_FP_REFERENCE: return v.template run<T, PolSeq<T, OP, true>>();
+    case OSC_FP_LINEAR: return v.template run<T, PolSeq<T, OP, false>>();
+    case OSC_FP_BINOMIAL: return v.template run<T, PolBinom<T, OP>>();
+    case OSC_FP_RABENSEIFNER: return v.template run<T, PolSeqRab<T, OP>>();
+  }
+  return OSC_ERR_ARG;
+}
+
 template <class T, template <int> class AccPol, class V>
 static int visit_real_t(int op, int mode, V &v) {
   switch (op) {
-    case OSC_OP_MIN: return v.template run<T, PolSeq<T, OSC_OP_MIN, true>>();
-    case OSC_OP_MAX: return v.template run<T, PolSeq<T, OSC_OP_MAX, true>>();
-    case OSC_OP_SUM:
-      return mode == OSC_FP_REFERENCE
-                 ? v.template run<T, PolTree<T, OSC_OP_SUM>>()
-                 : v.template run<T, AccPol<OSC_OP_SUM>>();
-    case OSC_OP_PROD:
-      return mode == OSC_FP_REFERENCE
-                 ? v.template run<T, PolTree<T, OSC_OP_PROD>>()
-                 : v.template run<T, AccPol<OSC_OP_PROD>>();
+    case OSC_OP_MIN: return visit_sel<T, OSC_OP_MIN>(mode, v);
+    case OSC_OP_MAX: return visit_sel<T, OSC_OP_MAX>(mode, v);
+    case OSC_OP_SUM: return visit_arith<T, AccPol, OSC_OP_SUM>(mode, v);
+    case OSC_OP_PROD: return visit_arith<T, AccPol, OSC_OP_PROD>(mode, v);
   }
   return OSC_ERR_UNSUPPORTED;
 }
@@ -1256,14 +1418,8 @@ static int visit_real_t(int op, int mode, V &v) {
 template <class T, template <int> class AccPol, class V>
 static int visit_cplx_t(int op, int mode, V &v) {
   switch (op) {
-    case OSC_OP_SUM:
-      return mode == OSC_FP_REFERENCE
-                 ? v.template run<T, PolTree<T, OSC_OP_SUM>>()
-                 : v.template run<T, AccPol<OSC_OP_SUM>>();
-    case OSC_OP_PROD:
-      return mode == OSC_FP_REFERENCE
-                 ? v.template run<T, PolTree<T, OSC_OP_PROD>>()
-                 : v.template run<T, AccPol<OSC_OP_PROD>>();
+    case OSC_OP_SUM: return visit_arith<T, AccPol, OSC_OP_SUM>(mode, v);
+    case OSC_OP_PROD: return visit_arith<T, AccPol, OSC_OP_PROD>(mode, v);
   }
   return OSC_ERR_UNSUPPORTED;
 }

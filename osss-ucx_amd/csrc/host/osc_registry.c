@@ -11,13 +11,22 @@
  * here every (op, type) cell is resolved once, at init, into a table the
  * entry points index directly.
  *
- * Reduction algorithms on the GPU:
- *   "hip"     one-pass kernel; float SUM/PROD accumulate in FP64 (FP32) or
- *             double-double (FP64) and round once.  The default.
- *   "rec_dbl" the same kernel in the reference's recursive-doubling
- *             combine order and the type's own arithmetic: bit-identical
- *             to the reference default (src/shcoll/src/reduce.c:182-293).
- * Integer, bitwise and MIN/MAX results are the same under both.
+ * Reduction algorithms on the GPU (src/api/collectives/table.c:441-519
+ * registers the same five reference names for every reduction):
+ *   "hip"          one-pass kernel; float SUM/PROD accumulate in FP64
+ *                  (FP32) or double-double (FP64) and round once.  The
+ *                  default.
+ *   "rec_dbl"      the same kernel in the reference's recursive-doubling
+ *                  combine order and the type's own arithmetic:
+ *                  bit-identical to the reference default
+ *                  (src/shcoll/src/reduce.c:182-293).
+ *   "linear"       team-order fold (reduce.c:55-91).
+ *   "binomial"     binomial tree (reduce.c:102-171), children folded in
+ *                  ascending order.
+ *   "rabenseifner", "rabenseifner2"
+ *                  reduce-scatter + allgather (reduce.c:306-724).
+ * Integer and bitwise results are the same under all of them; float
+ * results follow each algorithm's combine order (osc.h, osc_fpmode_t).
  * Data movement (fcollect, alltoall, broadcast) is a permutation of bytes,
  * identical under every reference algorithm, so the reference's algorithm
  * names are accepted as aliases of the one GPU kernel.
@@ -116,6 +125,19 @@ int osc_registry_resolve(int kind, int op, const char *spec, int *fpmode) {
       if (fpmode) *fpmode = OSC_FP_REFERENCE;
       return OSC_OK;
     }
+    if (strcmp(algo, "linear") == 0) {
+      if (fpmode) *fpmode = OSC_FP_LINEAR;
+      return OSC_OK;
+    }
+    if (strcmp(algo, "binomial") == 0) {
+      if (fpmode) *fpmode = OSC_FP_BINOMIAL;
+      return OSC_OK;
+    }
+    if (strcmp(algo, "rabenseifner") == 0 ||
+        strcmp(algo, "rabenseifner2") == 0) {
+      if (fpmode) *fpmode = OSC_FP_RABENSEIFNER;
+      return OSC_OK;
+    }
     return OSC_ERR_ALGO;
   }
   if (strcmp(algo, "hip") == 0) return OSC_OK;
@@ -179,7 +201,7 @@ int osc_registry_init_from_env(void) {
 
 int osc_set_fp_mode(int mode) {
   int op, t;
-  if (mode != OSC_FP_ACCURATE && mode != OSC_FP_REFERENCE)
+  if (mode < OSC_FP_ACCURATE || mode > OSC_FP_RABENSEIFNER)
     return osc_set_error(OSC_ERR_ARG, "fp mode %d", mode);
   osc_g.fp_default = mode;
   for (op = 0; op < OSC_OP_COUNT; ++op)

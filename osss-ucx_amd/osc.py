@@ -32,6 +32,7 @@ NP_DTYPE = {"u8": np.uint8, "u16": np.uint16, "u32": np.uint32,
 DIST = {"bits": 0, "int_range": 1, "set_m1_1_2": 2, "normal": 3,
         "real_range": 4, "a2a_pattern": 5, "fcol_pattern": 6}
 FP_ACCURATE, FP_REFERENCE = 0, 1
+FP_LINEAR, FP_BINOMIAL, FP_RABENSEIFNER = 2, 3, 4
 
 # OpenSHMEM typename -> kernel dtype (include/shmem.h type tables, LP64)
 TYPENAME_DTYPE = {

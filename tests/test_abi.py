@@ -81,7 +81,11 @@ def test_parse_size_rejects(text):
     ("reduce", "hip:int", True, 0),
     ("reduce", "bogus", False, None),
     ("reduce", "rec_dbl:notatype", False, None),
-    ("reduce", "linear", False, None),
+    ("reduce", "linear", True, 2),
+    ("reduce", "binomial:double", True, 3),
+    ("reduce", "rabenseifner", True, 4),
+    ("reduce", "rabenseifner2:long", True, 4),
+    ("reduce", "rabenseifner3", False, None),
     ("fcollect", "bruck_inplace", True, None),
     ("fcollect", "hip", True, None),
     ("alltoall", "shift_exchange_barrier", True, None),
