@@ -134,6 +134,9 @@ int osc_get_fp_mode(void);
  * per thread x slices in flight (OSC_REDUCE_SHAPE sets it at init). */
 int osc_set_reduce_shape(int shape);
 int osc_get_reduce_shape(void);
+/* Parallel branches per wave in a batch's captured graph, 1..8 (default 4,
+ * OSC_BATCH_BRANCHES sets it at init); takes effect at the next capture. */
+int osc_set_batch_branches(int n);
 const char *osc_strerror(int code);
 const char *osc_last_error(void);
 int osc_last_error_code(void); /* code of the calling thread's last error */

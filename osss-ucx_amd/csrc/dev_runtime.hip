@@ -171,6 +171,11 @@ int oscd_event_record(void *e, void *s) {
   return OSC_OK;
 }
 
+int oscd_stream_wait_event(void *s, void *e) {
+  HIPCALL(hipStreamWaitEvent((hipStream_t)s, (hipEvent_t)e, 0));
+  return OSC_OK;
+}
+
 int oscd_event_sync(void *e) {
   HIPCALL(hipEventSynchronize((hipEvent_t)e));
   return OSC_OK;

@@ -9,7 +9,9 @@
  * movement: unit width) and each group becomes one persistent tile-loop
  * launch (osc_sched.c).  The whole launch sequence is captured once into a
  * hipGraph and replayed, so a batch of thousands of calls costs one graph
- * launch.
+ * launch; inside the graph the launches of a wave form parallel branches
+ * (osc_set_batch_branches, OSC_BATCH_BRANCHES), joined before the next
+ * wave reuses the destinations.
  *
  * The reference has no counterpart: its team collectives serialise on
  * WORLD barriers (src/shcoll/src/reduce.c:1090-1100), so concurrent calls
@@ -21,6 +23,8 @@
 
 #include "osc_sched.h"
 
+#define OSC_BATCH_MAX_BRANCHES 8
+
 struct osc_batch {
   int ncalls, nwaves;
   osc_call *calls;
@@ -31,7 +35,27 @@ struct osc_batch {
   void *graph;
   void *graph_stream;
   double alg_bytes;
+  /* parallel branches of the captured graph: streams, fork/join events */
+  int nbranch;
+  void *aux[OSC_BATCH_MAX_BRANCHES - 1];
+  void *ev_fork;
+  void *ev_join[OSC_BATCH_MAX_BRANCHES - 1];
 };
+
+/* streams and events for nbranch parallel branches (created once) */
+static int batch_branches(osc_batch *b, int nbranch) {
+  int i, rc = OSC_OK;
+  if (nbranch < 1) nbranch = 1;
+  if (nbranch > OSC_BATCH_MAX_BRANCHES) nbranch = OSC_BATCH_MAX_BRANCHES;
+  if (nbranch > 1 && b->ev_fork == NULL) rc = oscd_event_create(&b->ev_fork);
+  for (i = 0; i < nbranch - 1 && rc == OSC_OK; ++i) {
+    if (b->aux[i] == NULL) rc = oscd_stream_create(&b->aux[i]);
+    if (rc == OSC_OK && b->ev_join[i] == NULL)
+      rc = oscd_event_create(&b->ev_join[i]);
+  }
+  if (rc == OSC_OK) b->nbranch = nbranch;
+  return rc;
+}
 
 static uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
@@ -148,16 +172,21 @@ int osc_batch_launch(osc_batch *b, void *stream) {
   int rc;
   if (b == NULL) return osc_set_error(OSC_ERR_ARG, "null batch");
   if (stream == NULL) stream = osc_g.stream;
-  if (b->graph != NULL && b->graph_stream != stream) {
+  if (b->graph != NULL &&
+      (b->graph_stream != stream || b->nbranch != osc_g.batch_branches)) {
     oscd_graph_destroy(b->graph);
     b->graph = NULL;
   }
   if (b->graph == NULL) {
     void *exec = NULL;
     int rc2;
+    rc = batch_branches(b, osc_g.batch_branches);
+    if (rc != OSC_OK) return osc_hip_error(rc, "batch branch streams");
     rc = oscd_graph_begin(stream);
     if (rc != OSC_OK) return osc_hip_error(rc, "hipStreamBeginCapture");
-    rc = osc_plan_enqueue(&b->plan, b->calls, b->d_calls, b->d_tiles, stream);
+    rc = osc_plan_enqueue_branches(&b->plan, b->calls, b->d_calls, b->d_tiles,
+                                   stream, b->nbranch, b->aux, b->ev_fork,
+                                   b->ev_join);
     rc2 = oscd_graph_end(stream, &exec);
     if (rc != OSC_OK) {
       if (exec) oscd_graph_destroy(exec);
@@ -208,6 +237,14 @@ int osc_batch_destroy(osc_batch *b) {
   if (b == NULL) return OSC_OK;
   if (osc_g.initialized && osc_g.stream) oscd_stream_sync(osc_g.stream);
   if (b->graph) oscd_graph_destroy(b->graph);
+  {
+    int i;
+    for (i = 0; i < OSC_BATCH_MAX_BRANCHES - 1; ++i) {
+      if (b->aux[i]) oscd_stream_destroy(b->aux[i]);
+      if (b->ev_join[i]) oscd_event_destroy(b->ev_join[i]);
+    }
+    if (b->ev_fork) oscd_event_destroy(b->ev_fork);
+  }
   if (b->d_calls) oscd_free(b->d_calls);
   if (b->d_tiles) oscd_free(b->d_tiles);
   if (b->d_digest) oscd_free(b->d_digest);

@@ -84,6 +84,12 @@ int osc_init(int npes, size_t heap_bytes_per_pe, int device) {
   const char *spin = getenv("OSC_WAIT_SPIN_US");
   osc_g.log_calls = (lg != NULL && lg[0] != '\0' && lg[0] != '0');
   osc_g.reduce_shape = (shp != NULL && shp[0] != '\0') ? atoi(shp) : -1;
+  {
+    const char *br = getenv("OSC_BATCH_BRANCHES");
+    osc_g.batch_branches = (br != NULL && br[0] != '\0') ? atoi(br) : 4;
+    if (osc_g.batch_branches < 1) osc_g.batch_branches = 1;
+    if (osc_g.batch_branches > 8) osc_g.batch_branches = 8;
+  }
   osc_g.wait_spin_us =
       (spin != NULL && spin[0] != '\0') ? (unsigned)strtoul(spin, NULL, 10) : 500u;
   if (npes < 1 || npes > OSC_MAX_NPES)
@@ -178,6 +184,12 @@ int osc_set_reduce_shape(int shape) {
   return OSC_OK;
 }
 int osc_get_reduce_shape(void) { return osc_g.reduce_shape; }
+
+int osc_set_batch_branches(int n) {
+  if (n < 1 || n > 8) return osc_set_error(OSC_ERR_ARG, "batch branches %d (1..8)", n);
+  osc_g.batch_branches = n;
+  return OSC_OK;
+}
 int osc_npes(void) { return osc_g.npes; }
 size_t osc_slice_bytes(void) { return osc_g.slice; }
 size_t osc_heap_bytes(void) { return osc_g.heap_bytes; }

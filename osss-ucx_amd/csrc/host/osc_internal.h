@@ -49,6 +49,7 @@ typedef struct osc_state {
   unsigned char fpmode_to_all[OSC_OP_COUNT][OSC_T_COUNT]; /* active-set forms */
   int fp_default;
   int reduce_shape; /* -1 = per-policy default; OSC_REDUCE_SHAPE overrides */
+  int batch_branches; /* parallel branches per wave of a batch's graph */
   unsigned wait_spin_us; /* blocking calls: bounded spin on a completion
                             slot before hipStreamSynchronize (0: never) */
   void *aux_dev;    /* 256 device bytes for single-call descriptors */

@@ -304,6 +304,7 @@ int osc_plan_build_into(osc_plan *p, const osc_call *calls, int ncalls,
       L->kind = OSC_LAUNCH_DIRECT;
       L->call = ci;
       L->tile0 = k;
+      L->bytes = osc_call_alg_bytes(c);
       continue;
     }
     if (g->ntiles == 0) continue;
@@ -335,6 +336,7 @@ int osc_plan_build_into(osc_plan *p, const osc_call *calls, int ncalls,
     emit_tiles(c, g, (uint32_t)ci, out + k);
     k += g->ntiles;
     L->ntiles += g->ntiles;
+    L->bytes += osc_call_alg_bytes(c);
     if (lg_of(c->team_npes) > L->maxlg) L->maxlg = lg_of(c->team_npes);
     if (c->team_npes > L->maxpes) L->maxpes = c->team_npes;
     if ((uint32_t)g->grid > L->grid) L->grid = (uint32_t)g->grid;
@@ -355,27 +357,82 @@ int osc_plan_build(osc_plan *p, const osc_call *calls, int ncalls,
   return osc_plan_build_into(p, calls, ncalls, use_waves, NULL, 0);
 }
 
+static int enqueue_one(const osc_launch *L, const osc_call *calls,
+                       const oscd_bcall *d_calls, const oscd_tile *d_tiles,
+                       void *stream) {
+  const oscd_tile *tiles = d_tiles + L->tile0;
+  if (L->kind == OSC_LAUNCH_DIRECT)
+    return osc_call_launch_single(&calls[L->call], stream);
+  if (L->kind == OSC_CALL_REDUCE) {
+    const size_t lds = (L->dtype == OSC_T_F32 || L->dtype == OSC_T_F64) &&
+                               (L->op == OSC_OP_MIN || L->op == OSC_OP_MAX)
+                           ? (size_t)L->maxpes * osc_dtype_size(L->dtype)
+                           : 0;
+    return oscd_batch_reduce(L->dtype, L->op, L->mode, L->shape, L->maxlg,
+                             d_calls, tiles, (uint32_t)L->ntiles, L->grid,
+                             lds, osc_g.base, osc_g.slice, stream);
+  }
+  return oscd_batch_move(L->width, d_calls, tiles, (uint32_t)L->ntiles,
+                         L->grid, osc_g.base, osc_g.slice, stream);
+}
+
 int osc_plan_enqueue(const osc_plan *p, const osc_call *calls,
                      const oscd_bcall *d_calls, const oscd_tile *d_tiles,
                      void *stream) {
   int i, rc = OSC_OK;
-  for (i = 0; i < p->nlaunches && rc == OSC_OK; ++i) {
-    const osc_launch *L = &p->launches[i];
-    const oscd_tile *tiles = d_tiles + L->tile0;
-    if (L->kind == OSC_LAUNCH_DIRECT) {
-      rc = osc_call_launch_single(&calls[L->call], stream);
-    } else if (L->kind == OSC_CALL_REDUCE) {
-      const size_t lds = (L->dtype == OSC_T_F32 || L->dtype == OSC_T_F64) &&
-                                 (L->op == OSC_OP_MIN || L->op == OSC_OP_MAX)
-                             ? (size_t)L->maxpes * osc_dtype_size(L->dtype)
-                             : 0;
-      rc = oscd_batch_reduce(L->dtype, L->op, L->mode, L->shape, L->maxlg,
-                             d_calls, tiles, (uint32_t)L->ntiles, L->grid,
-                             lds, osc_g.base, osc_g.slice, stream);
-    } else {
-      rc = oscd_batch_move(L->width, d_calls, tiles, (uint32_t)L->ntiles,
-                           L->grid, osc_g.base, osc_g.slice, stream);
+  for (i = 0; i < p->nlaunches && rc == OSC_OK; ++i)
+    rc = enqueue_one(&p->launches[i], calls, d_calls, d_tiles, stream);
+  return rc;
+}
+
+#define OSC_MAX_BRANCHES 8
+
+int osc_plan_enqueue_branches(const osc_plan *p, const osc_call *calls,
+                              const oscd_bcall *d_calls,
+                              const oscd_tile *d_tiles, void *stream,
+                              int nbranch, void *const *aux, void *ev_fork,
+                              void *const *ev_join) {
+  int i0 = 0, rc = OSC_OK;
+  if (nbranch > OSC_MAX_BRANCHES) nbranch = OSC_MAX_BRANCHES;
+  if (nbranch <= 1) return osc_plan_enqueue(p, calls, d_calls, d_tiles, stream);
+  while (i0 < p->nlaunches && rc == OSC_OK) {
+    double load[OSC_MAX_BRANCHES] = {0};
+    int i1 = i0, i, b;
+    while (i1 < p->nlaunches && p->launches[i1].wave == p->launches[i0].wave)
+      ++i1;
+    if (i1 - i0 == 1) { /* nothing to overlap */
+      rc = enqueue_one(&p->launches[i0], calls, d_calls, d_tiles, stream);
+      i0 = i1;
+      continue;
     }
+    rc = oscd_event_record(ev_fork, stream);
+    for (b = 1; b < nbranch && rc == OSC_OK; ++b)
+      rc = oscd_stream_wait_event(aux[b - 1], ev_fork);
+    /* largest first onto the least loaded branch (launches of a wave are
+     * adjacent and few hundred at most: a selection pass per launch) */
+    {
+      char *done = (char *)calloc((size_t)(i1 - i0), 1);
+      if (done == NULL) return osc_set_error(OSC_ERR_NOMEM, "branches");
+      for (i = i0; i < i1 && rc == OSC_OK; ++i) {
+        int j, best = -1, bb = 0;
+        for (j = i0; j < i1; ++j)
+          if (!done[j - i0] &&
+              (best < 0 || p->launches[j].bytes > p->launches[best].bytes))
+            best = j;
+        for (b = 1; b < nbranch; ++b)
+          if (load[b] < load[bb]) bb = b;
+        done[best - i0] = 1;
+        load[bb] += p->launches[best].bytes;
+        rc = enqueue_one(&p->launches[best], calls, d_calls, d_tiles,
+                         bb == 0 ? stream : aux[bb - 1]);
+      }
+      free(done);
+    }
+    for (b = 1; b < nbranch && rc == OSC_OK; ++b) {
+      rc = oscd_event_record(ev_join[b - 1], aux[b - 1]);
+      if (rc == OSC_OK) rc = oscd_stream_wait_event(stream, ev_join[b - 1]);
+    }
+    i0 = i1;
   }
   return rc;
 }

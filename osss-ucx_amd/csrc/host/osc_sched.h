@@ -22,6 +22,7 @@ typedef struct {
   int maxlg, maxpes;
   uint32_t grid;
   uint64_t tile0, ntiles;
+  double bytes; /* algorithmic bytes of the launch's calls */
 } osc_launch;
 
 typedef struct {
@@ -58,6 +59,18 @@ int osc_plan_build_into(osc_plan *p, const osc_call *calls, int ncalls,
 int osc_plan_enqueue(const osc_plan *p, const osc_call *calls,
                      const oscd_bcall *d_calls, const oscd_tile *d_tiles,
                      void *stream);
+/* The same, with the launches of each wave spread over nbranch streams:
+ * `stream` and aux[0 .. nbranch-2], forked from `stream` at the start of
+ * the wave through ev_fork and joined back at its end through
+ * ev_join[0 .. nbranch-2] (launches of one wave are independent: disjoint
+ * destinations, read-only sources).  Branches are balanced by algorithmic
+ * bytes.  Works under stream capture, where it records a graph whose
+ * waves are sets of parallel branches. */
+int osc_plan_enqueue_branches(const osc_plan *p, const osc_call *calls,
+                              const oscd_bcall *d_calls,
+                              const oscd_tile *d_tiles, void *stream,
+                              int nbranch, void *const *aux, void *ev_fork,
+                              void *const *ev_join);
 /* One call through the per-call kernels (the "one by one" path). */
 int osc_call_launch_single(const osc_call *c, void *stream);
 

@@ -55,6 +55,7 @@ int oscd_event_create(void **e);
 int oscd_event_destroy(void *e);
 int oscd_event_record(void *e, void *s);
 int oscd_event_sync(void *e);
+int oscd_stream_wait_event(void *s, void *e); /* s waits for e's record */
 int oscd_event_query(void *e); /* 1 = done, 0 = pending, <0 error */
 int oscd_event_elapsed_ms(void *a, void *b, float *ms);
 int oscd_device_sync(void);

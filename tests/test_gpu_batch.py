@@ -229,3 +229,31 @@ def test_batch_rejects_wrapping_team_and_count(world):
     for run in (lambda: osc.Batch([c]), lambda: osc.request_alone(c, world.dest_off)):
         with pytest.raises(osc.OscError):
             run()
+
+
+@pytest.mark.parametrize("branches", [1, 2, 8])
+def test_parallel_branches_give_the_same_results(world, branches):
+    """The captured graph spreads each wave's launches over parallel
+    branches (4 by default); every branch count gives the digests of the
+    one-by-one path, on every replay."""
+    h = world.h
+    osc.check(h.L.osc_set_batch_branches(branches), "branches")
+    try:
+        rng = np.random.default_rng(500 + branches)
+        calls, nw = osc.plan(make_calls(rng, world, 400), world.dest_off,
+                             DEST_REGION)
+        for wv in range(nw):
+            wave = [c for c in calls if c.wave == wv]
+            b = osc.Batch(wave)
+            b.launch_serial()
+            d_serial = b.digests()
+            h.L.osc_device_copy(world.dest_base, world.src_base,
+                                DEST_REGION // 4, None)
+            for rep in range(2):
+                b.launch()
+                assert np.array_equal(b.digests(), d_serial), (wv, rep)
+            b.close()
+    finally:
+        h.L.osc_set_batch_branches(4)
+    assert h.L.osc_set_batch_branches(0) != 0
+    assert h.L.osc_set_batch_branches(9) != 0
