@@ -12,6 +12,7 @@
  */
 #include <pthread.h>
 #include <stdio.h>
+#include <stdlib.h>
 
 #include "osc_internal.h"
 
@@ -19,11 +20,78 @@ static int overlap(size_t a, size_t alen, size_t b, size_t blen) {
   return a < b + blen && b < a + alen;
 }
 
-static void log_call(const char *what, const osc_team *t, int op, int dtype,
-                     size_t n) {
-  if (osc_g.log_calls)
-    fprintf(stderr, "[osc] %s team(start=%d,stride=%d,npes=%d) op=%d type=%d n=%zu\n",
-            what, t->start, t->stride, t->npes, op, dtype, n);
+/* ---- per-call records (OSC_LOG=1) ------------------------------------------
+ * The reference logs each collective as a COLLECTIVES event with its
+ * arguments (src/shmemu/shmemu.h:135-151, logger.c:270).  Here a record
+ * also carries what the call cost: the launch is bracketed by two HIP
+ * events on the call's stream, the call waits for the second, and one
+ * JSON line goes to OSC_LOG_FILE (appended) or stderr:
+ *   {"event": "COLLECTIVES", "call": "reduce", "op": "sum", "type": "f32",
+ *    "algo": "hip", "team": [start, stride, npes], "count": n,
+ *    "bytes": algorithmic bytes, "us": ..., "GBps": ..., "pct_peak": ...}
+ * Algorithmic bytes: reduce 2*P*n*s, fcollect P*m + P*P*m, alltoall
+ * 2*P*P*b, broadcast m + P*m, collect sum(n_i) * (P + 1), alltoalls
+ * 2*P*P*n*s.  Recording serialises the calls that take it (one event pair,
+ * one lock): it is an observability mode. */
+#define OSC_PEAK_GBPS 8000.0
+static pthread_mutex_t rec_mu = PTHREAD_MUTEX_INITIALIZER;
+
+typedef struct {
+  int on;
+  void *s;
+} call_rec;
+
+static void rec_begin(call_rec *r, void *stream) {
+  r->on = 0;
+  if (!osc_g.log_calls) return;
+  pthread_mutex_lock(&rec_mu);
+  if ((osc_g.log_ev0 == NULL && oscd_event_create(&osc_g.log_ev0) != OSC_OK) ||
+      (osc_g.log_ev1 == NULL && oscd_event_create(&osc_g.log_ev1) != OSC_OK) ||
+      oscd_event_record(osc_g.log_ev0, stream) != OSC_OK) {
+    pthread_mutex_unlock(&rec_mu);
+    return;
+  }
+  r->on = 1;
+  r->s = stream;
+}
+
+static const char *const k_op_names[OSC_OP_COUNT] = {"and", "or",  "xor", "min",
+                                                     "max", "sum", "prod"};
+static const char *const k_type_names[OSC_T_COUNT] = {
+    "u8", "u16", "u32", "u64", "i8", "i16", "i32", "i64", "f32", "f64", "c32", "c64"};
+static const char *const k_algo_names[] = {"hip", "rec_dbl", "linear", "binomial",
+                                           "rabenseifner"};
+
+static void rec_end(call_rec *r, const char *what, const osc_team *t, int op,
+                    int dtype, int fpmode, uint64_t count, double bytes, int rc) {
+  float ms = 0.0f;
+  FILE *f;
+  if (!r->on) return;
+  if (rc == OSC_OK && oscd_event_record(osc_g.log_ev1, r->s) == OSC_OK &&
+      oscd_event_sync(osc_g.log_ev1) == OSC_OK &&
+      oscd_event_elapsed_ms(osc_g.log_ev0, osc_g.log_ev1, &ms) == OSC_OK) {
+    const double us = 1e3 * (double)ms;
+    const double gbps = us > 0.0 ? bytes / (us * 1e3) : 0.0;
+    const char *path = getenv("OSC_LOG_FILE");
+    f = (path != NULL && path[0] != '\0') ? fopen(path, "a") : stderr;
+    if (f != NULL) {
+      fprintf(f, "{\"event\": \"COLLECTIVES\", \"call\": \"%s\"", what);
+      if (op >= 0 && op < OSC_OP_COUNT)
+        fprintf(f, ", \"op\": \"%s\"", k_op_names[op]);
+      if (dtype >= 0 && dtype < OSC_T_COUNT)
+        fprintf(f, ", \"type\": \"%s\"", k_type_names[dtype]);
+      if (fpmode >= 0 && fpmode <= OSC_FP_RABENSEIFNER)
+        fprintf(f, ", \"algo\": \"%s\"", k_algo_names[fpmode]);
+      fprintf(f,
+              ", \"team\": [%d, %d, %d], \"count\": %llu, \"bytes\": %.0f, "
+              "\"us\": %.3f, \"GBps\": %.1f, \"pct_peak\": %.2f}\n",
+              t->start, t->stride, t->npes, (unsigned long long)count, bytes, us,
+              gbps, 100.0 * gbps / OSC_PEAK_GBPS);
+      if (f != stderr) fclose(f);
+    }
+  }
+  r->on = 0;
+  pthread_mutex_unlock(&rec_mu);
 }
 
 int osc_reduce(void *team, int op, int dtype, void *dest, const void *src,
@@ -56,12 +124,18 @@ int osc_reduce_mode(void *team, int op, int dtype, void *dest, const void *src,
   if (doff != soff && overlap(doff, nreduce * es, soff, nreduce * es))
     return osc_set_error(OSC_ERR_ARG, "dest and source partially overlap");
   if ((size_t)t->npes > (size_t)osc_g.npes) return osc_set_error(OSC_ERR_TEAM, "team too large");
-  log_call("reduce", t, op, dtype, nreduce);
   if (nreduce == 0) return OSC_OK;
   osc_team_geom(t, &g);
-  rc = oscd_reduce(&g, op, dtype, fpmode >= 0 ? fpmode : osc_g.fpmode[op][dtype],
-                   osc_g.reduce_shape,
-                   doff, soff, nreduce, stream ? stream : osc_g.stream);
+  if (fpmode < 0) fpmode = osc_g.fpmode[op][dtype];
+  {
+    void *s = stream ? stream : osc_g.stream;
+    call_rec r;
+    rec_begin(&r, s);
+    rc = oscd_reduce(&g, op, dtype, fpmode, osc_g.reduce_shape, doff, soff,
+                     nreduce, s);
+    rec_end(&r, "reduce", t, op, dtype, fpmode, nreduce,
+            2.0 * t->npes * (double)nreduce * (double)es, rc);
+  }
   return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "reduce launch");
 }
 
@@ -80,10 +154,16 @@ int osc_fcollect(void *team, void *dest, const void *src, size_t nbytes,
   if (rc != OSC_OK) return rc;
   if (nbytes > 0 && overlap(doff, nbytes * (size_t)t->npes, soff, nbytes))
     return osc_set_error(OSC_ERR_ARG, "dest and source overlap");
-  log_call("fcollect", t, -1, -1, nbytes);
   if (nbytes == 0) return OSC_OK;
   osc_team_geom(t, &g);
-  rc = oscd_fcollect(&g, doff, soff, nbytes, stream ? stream : osc_g.stream);
+  {
+    void *s = stream ? stream : osc_g.stream;
+    call_rec r;
+    rec_begin(&r, s);
+    rc = oscd_fcollect(&g, doff, soff, nbytes, s);
+    rec_end(&r, "fcollect", t, -1, -1, -1, nbytes,
+            (double)nbytes * t->npes * (1.0 + t->npes), rc);
+  }
   return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "fcollect launch");
 }
 
@@ -103,10 +183,16 @@ int osc_alltoall(void *team, void *dest, const void *src, size_t nbytes,
   if (rc != OSC_OK) return rc;
   if (tot > 0 && overlap(doff, tot, soff, tot))
     return osc_set_error(OSC_ERR_ARG, "dest and source overlap");
-  log_call("alltoall", t, -1, -1, nbytes);
   if (nbytes == 0) return OSC_OK;
   osc_team_geom(t, &g);
-  rc = oscd_alltoall(&g, doff, soff, nbytes, stream ? stream : osc_g.stream);
+  {
+    void *s = stream ? stream : osc_g.stream;
+    call_rec r;
+    rec_begin(&r, s);
+    rc = oscd_alltoall(&g, doff, soff, nbytes, s);
+    rec_end(&r, "alltoall", t, -1, -1, -1, nbytes,
+            2.0 * (double)nbytes * t->npes * t->npes, rc);
+  }
   return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "alltoall launch");
 }
 
@@ -125,11 +211,16 @@ int osc_broadcast_impl(void *team, void *dest, const void *src, size_t nbytes,
   if (rc != OSC_OK) return rc;
   if (doff != soff && overlap(doff, nbytes, soff, nbytes))
     return osc_set_error(OSC_ERR_ARG, "dest and source partially overlap");
-  log_call("broadcast", t, -1, -1, nbytes);
   if (nbytes == 0) return OSC_OK;
   osc_team_geom(t, &g);
-  rc = oscd_broadcast(&g, root, doff, soff, nbytes, skip_root,
-                      stream ? stream : osc_g.stream);
+  {
+    void *s = stream ? stream : osc_g.stream;
+    call_rec r;
+    rec_begin(&r, s);
+    rc = oscd_broadcast(&g, root, doff, soff, nbytes, skip_root, s);
+    rec_end(&r, "broadcast", t, -1, -1, -1, nbytes,
+            (double)nbytes * (1.0 + t->npes - (skip_root ? 1 : 0)), rc);
+  }
   return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "broadcast launch");
 }
 
@@ -221,8 +312,15 @@ static int collect_check(const osc_team *t, void *dest, const void *src,
   if (rc != OSC_OK) return rc;
   if (total > 0 && *maxn > 0 && overlap(*doff, total, *soff, *maxn))
     return osc_set_error(OSC_ERR_ARG, "dest and source overlap");
-  log_call("collect", t, -1, -1, total);
   return OSC_OK;
+}
+
+/* sum of a collect's per-member byte counts (checked not to wrap) */
+static double collect_total(const osc_team *t, const size_t *nbytes) {
+  double tot = 0.0;
+  int i;
+  for (i = 0; i < t->npes; ++i) tot += (double)nbytes[i];
+  return tot;
 }
 
 /* Variable-size collect (collect.c:406-492 closed form): the per-member
@@ -237,9 +335,13 @@ int osc_collect(void *team, void *dest, const void *src, const size_t *nbytes,
   size_t doff, soff, maxn;
   void *s = stream ? stream : osc_g.stream;
   int rc = collect_check(t, dest, src, nbytes, &doff, &soff, &maxn);
+  call_rec r;
   if (rc != OSC_OK || maxn == 0) return rc;
   pthread_mutex_lock(&tab_mu);
+  rec_begin(&r, s);
   rc = collect_launch(t, doff, soff, nbytes, maxn, s);
+  rec_end(&r, "collect", t, -1, -1, -1, (uint64_t)collect_total(t, nbytes),
+          collect_total(t, nbytes) * (1.0 + t->npes), rc);
   pthread_mutex_unlock(&tab_mu);
   return rc;
 }
@@ -263,9 +365,17 @@ int osc_collect_noalloc(void *team, void *dest, const void *src,
                          " at most %d", t->npes, OSCD_STATIC_TAB_MEMBERS);
   if (maxn == 0) return OSC_OK;
   osc_team_geom(t, &g);
-  rc = oscd_collect_static(&g, doff, soff, (const uint64_t *)nbytes, maxn,
-                           collect_unit(doff, soff, nbytes, t->npes),
-                           osc_g.device, stream ? stream : osc_g.stream);
+  {
+    void *s = stream ? stream : osc_g.stream;
+    call_rec r;
+    rec_begin(&r, s);
+    rc = oscd_collect_static(&g, doff, soff, (const uint64_t *)nbytes, maxn,
+                             collect_unit(doff, soff, nbytes, t->npes),
+                             osc_g.device, s);
+    rec_end(&r, "collect_noalloc", t, -1, -1, -1,
+            (uint64_t)collect_total(t, nbytes),
+            collect_total(t, nbytes) * (1.0 + t->npes), rc);
+  }
   return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "collect launch");
 }
 
@@ -300,10 +410,16 @@ int osc_alltoalls(void *team, void *dest, const void *src, ptrdiff_t dst,
   if (rc != OSC_OK) return rc;
   if (overlap(doff, need_d, soff, need_s))
     return osc_set_error(OSC_ERR_ARG, "dest and source overlap");
-  log_call("alltoalls", t, -1, -1, nelems);
   osc_team_geom(t, &g);
-  rc = oscd_alltoalls(&g, doff, soff, elem_bytes, (uint64_t)dst,
-                      (uint64_t)sst, nelems, stream ? stream : osc_g.stream);
+  {
+    void *s = stream ? stream : osc_g.stream;
+    call_rec r;
+    rec_begin(&r, s);
+    rc = oscd_alltoalls(&g, doff, soff, elem_bytes, (uint64_t)dst,
+                        (uint64_t)sst, nelems, s);
+    rec_end(&r, "alltoalls", t, -1, -1, -1, nelems,
+            2.0 * (double)nelems * (double)elem_bytes * t->npes * t->npes, rc);
+  }
   return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "alltoalls launch");
 }
 

@@ -164,6 +164,8 @@ int osc_finalize(void) {
   oscd_amo_fini();
   if (osc_g.aux_dev) oscd_free(osc_g.aux_dev);
   if (osc_g.tab_event) oscd_event_destroy(osc_g.tab_event);
+  if (osc_g.log_ev0) oscd_event_destroy(osc_g.log_ev0);
+  if (osc_g.log_ev1) oscd_event_destroy(osc_g.log_ev1);
   if (osc_g.tab_dev) oscd_free(osc_g.tab_dev);
   if (osc_g.tab_host) oscd_host_free(osc_g.tab_host);
   oscd_free(osc_g.base);

@@ -59,6 +59,8 @@ typedef struct osc_state {
   uint64_t *tab_dev;
   void *tab_event;
   int tab_used;
+  /* event pair of the per-call records (OSC_LOG) */
+  void *log_ev0, *log_ev1;
 } osc_state;
 
 extern osc_state osc_g;
