@@ -139,47 +139,84 @@ def measured_traffic(key):
         return None
 
 
-def cpu_baseline(seconds_budget=20.0):
-    """Oracle CPU restatement of the reference algorithm (rec_dbl, PEs as
-    threads, memcpy puts) on a bounded sample of the same matrix."""
+CPU_CELLS = [("xor", "u64"), ("sum", "i64"), ("sum", "f32"), ("sum", "f64"),
+             ("max", "f64")]
+
+
+def cpu_baseline():
+    """The reference's algorithms on the host cores, one forked process per
+    PE over one MAP_SHARED heap (the oracle's restatement: rec_dbl with its
+    pSync handshakes and the wrapper's two WORLD barriers, Bruck fcollect
+    with the byte-granular rotation, shift-exchange alltoall).  Runs in a
+    child process that never loads the HIP library, so no PE process holds
+    a device handle.  Main value: 5 of the 15 matrix cells at the full
+    64 PEs x 4,194,304 (one warm-up and one timed call each)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    threads = max(1, min(16, cores))
-    n = N // 16
-    total_b = 0.0
-    total_s = 0.0
-    t0 = time.time()
-    for op, dt, _, _, _ in CELLS:
-        sec = O.cpu_bench_reduce(op, dt, P, n, 3, threads)
-        total_b += 2.0 * P * n * ESIZE[dt]
+    total_b = total_s = 0.0
+    cells = {}
+    for op, dt in CPU_CELLS:
+        sec = O.cpu_bench_procs("reduce", P, N, 1, op, dt)
+        b = 2.0 * P * N * ESIZE[dt]
+        total_b += b
         total_s += sec
-        if time.time() - t0 > seconds_budget:
-            break
-    # the other workloads, each a few seconds at most: the small plain
-    # reduce's latency (one thread per PE, as one process per PE), alltoall
-    # at full size, and fcollect on a sample (its byte-granular rotation
-    # runs at well under 1 GB/s, so 256 PEs x 1 MiB would take minutes)
-    other = {
-        "int_sum_8pe_64K_call_us": round(1e6 * O.cpu_bench_reduce(
-            "sum", "i32", 8, 65536, 51, min(8, threads)), 1)}
-    sec = O.cpu_bench_alltoall(64, 65536, 5, threads)
-    other["alltoall_64pe_64KiB_GBps"] = round(2 * 64 * 64 * 65536 / sec / 1e9, 2)
-    sec = O.cpu_bench_fcollect(64, 65536, 3, threads)
-    other["fcollect_64pe_64KiB_GBps"] = round((64 + 64 * 64) * 65536 / sec / 1e9,
-                                              3)
+        cells[f"{op}_{dt}_GBps"] = round(b / sec / 1e9, 3)
+    other = {"cells": cells,
+             "int_sum_8pe_64K_call_us": round(1e6 * O.cpu_bench_procs(
+                 "reduce", 8, 65536, 51, "sum", "i32"), 1)}
+    sec = O.cpu_bench_procs("alltoall", 64, 65536, 5)
+    other["alltoall_64pe_64KiB_GBps"] = round(2 * 64 * 64 * 65536 / sec / 1e9,
+                                              2)
+    # fcollect at 256 PEs x 1 MiB needs 256 x (1 + 256) MiB = 64.25 GiB
+    need = 256 * 257 * (1 << 20)
+    avail = 0
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemAvailable:"):
+                    avail = int(line.split()[1]) * 1024
+    except OSError:
+        pass
+    if avail >= need * 1.25:
+        sec = O.cpu_bench_procs("fcollect", 256, 1 << 20, 1)
+        other["fcollect_256pe_1MiB_GBps"] = round(
+            (256 + 256 * 256) * (1 << 20) / sec / 1e9, 3)
+    else:
+        other["fcollect_256pe_1MiB"] = "skipped (RAM: %.1f GiB available, " \
+            "%.2f GiB needed)" % (avail / 2 ** 30, need / 2 ** 30)
+        sec = O.cpu_bench_procs("fcollect", 64, 65536, 3)
+        other["fcollect_64pe_64KiB_GBps"] = round(
+            (64 + 64 * 64) * 65536 / sec / 1e9, 3)
     return {"value": round(total_b / total_s / 1e9, 3), "unit": "GB/s",
-            "cores": threads, "kind": "port",
-            "sample": "rec_dbl restatement (oracle/orc.c), 64 PEs as %d "
-                      "threads, nreduce=%d (1/16 of the workload), op/type "
-                      "matrix cells, median of 3" % (threads, n),
+            "cores": cores, "kind": "port",
+            "sample": "rec_dbl restatement (oracle/orc.c), one forked process "
+                      "per PE over a MAP_SHARED heap on %d cores, 64 PEs x "
+                      "4194304 (full size), cells %s, one timed call each "
+                      "after a warm-up" % (cores, ", ".join(
+                          f"{o} {d}" for o, d in CPU_CELLS)),
             "other": other,
-            "other_sample": "medians: int sum 8 PEs x 65536 (51 calls), "
-                            "shift-exchange alltoall 64 PEs x 64 KiB (5), "
-                            "Bruck fcollect with rotation 64 PEs x 64 KiB (3)"}
+            "other_sample": "one process per PE: int sum 8 PEs x 65536 "
+                            "(median of 51 calls), shift-exchange alltoall "
+                            "64 PEs x 64 KiB (median of 5), Bruck fcollect "
+                            "with rotation " + (
+                                "256 PEs x 1 MiB (one call)"
+                                if "fcollect_256pe_1MiB_GBps" in other else
+                                "64 PEs x 64 KiB (median of 3; 256 PEs x "
+                                "1 MiB skipped for host RAM)")}
+
+
+def cpu_baseline_subprocess():
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__),
+                        "--cpu-baseline"], capture_output=True, text=True,
+                       timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError("CPU baseline failed: " + r.stderr[-2000:])
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def _median_event_ms(L, stream, fn, reps=10):
@@ -443,7 +480,12 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-movement", action="store_true")
+    ap.add_argument("--cpu-baseline", action="store_true",
+                    help="only the CPU baseline (run in a child process)")
     args = ap.parse_args()
+    if args.cpu_baseline:
+        print(json.dumps(cpu_baseline()))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the emulated PEs all live in one "
               "MI355X's HBM; --gpus %d is not supported" % args.gpus,
@@ -562,7 +604,7 @@ def main():
     out["batched"] = None if args.no_batched else bench_batched()
     out["served"] = None if args.no_served else bench_served()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline_subprocess()
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

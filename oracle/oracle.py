@@ -74,13 +74,9 @@ def lib() -> C.CDLL:
         L.orc_rotate.restype = None
         L.orc_hash3.restype = C.c_uint64
         L.orc_hash3.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
-        L.orc_cpu_bench_reduce.argtypes = [C.c_int, C.c_int, C.c_int,
-                                           C.c_size_t, C.c_int, C.c_int,
-                                           C.POINTER(C.c_double)]
-        L.orc_cpu_bench_fcollect.argtypes = [C.c_int, C.c_size_t, C.c_int,
-                                             C.c_int, C.POINTER(C.c_double)]
-        L.orc_cpu_bench_alltoall.argtypes = [C.c_int, C.c_size_t, C.c_int,
-                                             C.c_int, C.POINTER(C.c_double)]
+        L.orc_cpu_bench_procs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int,
+                                          C.c_size_t, C.c_int, pp, pp,
+                                          C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -249,38 +245,28 @@ def strided_copy(dst: np.ndarray, src: np.ndarray, es: int, tst: int,
     return d
 
 
-def cpu_bench_reduce(op, dtype, npes, n, iters, threads) -> float:
-    out = C.c_double()
-    rc = lib().orc_cpu_bench_reduce(OPS[op], DTYPES[dtype], npes, n, iters,
-                                    threads, C.byref(out))
+def cpu_bench_procs(kind: str, npes: int, n: int, iters: int, op="sum",
+                    dtype="u8", src=None, want_dest=False):
+    """One forked process per PE over one MAP_SHARED mapping (the reference's
+    process-per-PE model): kind "reduce" (rec_dbl, n elements of dtype),
+    "fcollect" (Bruck in place, n bytes per PE) or "alltoall" (shift
+    exchange, n-byte blocks).  Returns the median seconds per call (the
+    slowest PE), and every PE's destination when want_dest."""
+    k = {"reduce": 0, "fcollect": 1, "alltoall": 2}[kind]
+    es = np.dtype(NP_DTYPE[dtype]).itemsize if k == 0 else 1
+    dlen = n * es if k == 0 else n * npes
+    dest = np.zeros((npes, dlen), np.uint8) if want_dest else None
+    sp = None
+    if src is not None:
+        src = np.ascontiguousarray(src)
+        sp = _ptrs([src[i] for i in range(npes)])
+    dp = _ptrs([dest[i] for i in range(npes)]) if want_dest else None
+    sec = C.c_double()
+    rc = lib().orc_cpu_bench_procs(k, OPS[op], DTYPES[dtype], npes, n, iters,
+                                   sp, dp, C.byref(sec))
     if rc != 0:
-        raise RuntimeError("cpu bench failed")
-    return out.value
-
-
-def cpu_bench_fcollect(npes, nbytes, iters, threads) -> float:
-    """Median seconds of the Bruck in-place fcollect restatement."""
-    out = C.c_double()
-    rc = lib().orc_cpu_bench_fcollect(npes, C.c_size_t(nbytes), iters, threads,
-                                      C.byref(out))
-    if rc != 0:
-        raise RuntimeError("cpu bench failed")
-    return out.value
-
-
-def cpu_bench_alltoall(npes, nbytes, iters, threads) -> float:
-    """Median seconds of the shift-exchange alltoall restatement."""
-    out = C.c_double()
-    rc = lib().orc_cpu_bench_alltoall(npes, C.c_size_t(nbytes), iters, threads,
-                                      C.byref(out))
-    if rc != 0:
-        raise RuntimeError("cpu bench failed")
-    return out.value
-
-
-# ---- independent numpy formulations (pins) ----------------------------------
-
-_G = np.uint64(0x9E3779B97F4A7C15)
+        raise RuntimeError(f"process baseline failed ({rc})")
+    return (sec.value, dest) if want_dest else sec.value
 
 
 def _mix64(z: np.ndarray) -> np.ndarray:

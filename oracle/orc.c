@@ -13,11 +13,15 @@
 
 #include <complex.h>
 #include <math.h>
-#include <pthread.h>
 #include <sched.h>
+#include <signal.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/mman.h>
+#include <sys/wait.h>
 #include <time.h>
+#include <unistd.h>
 
 enum { AND, OR, XOR, MIN, MAX, SUM, PROD };
 enum { U8, U16, U32, U64, I8, I16, I32, I64, F32, F64, C32, C64, NTYPES };
@@ -733,7 +737,7 @@ uint64_t orc_hash3(uint64_t seed, uint64_t pe, uint64_t idx) {
   return mix64(mix64(mix64(seed) + G * (pe + 1)) + G * (idx + 1));
 }
 
-/* ---- CPU baseline: the reference algorithms with PE-parallel threads ---- */
+/* ---- timing helpers of the CPU baseline ---- */
 
 static double now_s(void) {
   struct timespec ts;
@@ -741,238 +745,311 @@ static double now_s(void) {
   return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
-typedef struct {
-  int kind; /* 0 reduce, 1 fcollect, 2 alltoall */
-  int op, dtype, P, p2s, T, iters;
-  size_t n, nb;
-  char **src, **dest, **tmp;
-  int *mp;
-  local_fn f;
-  pthread_barrier_t bar;
-  double *times;
-} bench_t;
-
-typedef struct {
-  bench_t *b;
-  int tid;
-} worker_t;
-
-static void reduce_phases(bench_t *b, int tid) {
-  const int P = b->P, p2s = b->p2s, T = b->T;
-  int me, mask;
-  /* per-call tmp allocation as the reference (:218-225) */
-  for (me = tid; me < P; me += T)
-    if (b->mp[me] >= 0) b->tmp[me] = (char *)malloc(b->nb);
-  for (me = tid; me < P; me += T) {
-    if (b->mp[me] < 0) continue;
-    if ((me + 1) * p2s / P == b->mp[me]) {
-      memcpy(b->dest[me], b->src[me + 1], b->nb);
-      b->f(b->op, b->tmp[me], b->dest[me], b->src[me], b->n);
-    } else {
-      memcpy(b->tmp[me], b->src[me], b->nb);
-    }
-  }
-  pthread_barrier_wait(&b->bar);
-  for (mask = 1; mask < p2s; mask <<= 1) {
-    for (me = tid; me < P; me += T) {
-      int pp, pa;
-      if (b->mp[me] < 0) continue;
-      pp = b->mp[me] ^ mask;
-      pa = (pp * P + p2s - 1) / p2s;
-      memcpy(b->dest[pa], b->tmp[me], b->nb); /* put my tmp to the peer */
-    }
-    pthread_barrier_wait(&b->bar);
-    for (me = tid; me < P; me += T)
-      if (b->mp[me] >= 0) b->f(b->op, b->tmp[me], b->tmp[me], b->dest[me], b->n);
-    pthread_barrier_wait(&b->bar);
-  }
-  for (me = tid; me < P; me += T)
-    if (b->mp[me] >= 0) memcpy(b->dest[me], b->tmp[me], b->nb);
-  pthread_barrier_wait(&b->bar);
-  for (me = tid; me < P; me += T)
-    if (b->mp[me] >= 0 && (me + 1) * p2s / P == b->mp[me])
-      memcpy(b->dest[me + 1], b->dest[me], b->nb);
-  for (me = tid; me < P; me += T) {
-    free(b->tmp[me]);
-    b->tmp[me] = NULL;
-  }
-}
-
-static void fcollect_phases(bench_t *b, int tid) {
-  const int P = b->P, T = b->T;
-  const size_t total = (size_t)P * b->nb;
-  size_t distance, sent = b->nb;
-  int me;
-  for (me = tid; me < P; me += T) memcpy(b->dest[me], b->src[me], b->nb);
-  pthread_barrier_wait(&b->bar);
-  for (distance = 1; distance < (size_t)P; distance <<= 1) {
-    const size_t to_send = (2 * sent <= total) ? sent : total - sent;
-    for (me = tid; me < P; me += T) {
-      const int peer = (int)(((size_t)me - distance + (size_t)P) % (size_t)P);
-      memcpy(b->dest[peer] + sent, b->dest[me], to_send);
-    }
-    sent += distance * b->nb;
-    pthread_barrier_wait(&b->bar);
-  }
-  for (me = tid; me < P; me += T)
-    orc_rotate_inplace(b->dest[me], total, (size_t)me * b->nb);
-}
-
-static void alltoall_phases(bench_t *b, int tid) {
-  const int P = b->P, T = b->T;
-  int me, i;
-  for (me = tid; me < P; me += T) {
-    memcpy(b->dest[me] + (size_t)me * b->nb, b->src[me] + (size_t)me * b->nb, b->nb);
-    for (i = 1; i < P; ++i) {
-      const int peer = (me + i) % P;
-      memcpy(b->dest[peer] + (size_t)me * b->nb, b->src[me] + (size_t)peer * b->nb,
-             b->nb);
-    }
-  }
-}
-
-static void *bench_worker(void *arg) {
-  worker_t *w = (worker_t *)arg;
-  bench_t *b = w->b;
-  int it;
-  for (it = 0; it < b->iters + 1; ++it) { /* first run warms up */
-    double t0;
-    pthread_barrier_wait(&b->bar);
-    t0 = now_s();
-    if (b->kind == 0) reduce_phases(b, w->tid);
-    else if (b->kind == 1) fcollect_phases(b, w->tid);
-    else alltoall_phases(b, w->tid);
-    pthread_barrier_wait(&b->bar);
-    if (w->tid == 0 && it > 0) b->times[it - 1] = now_s() - t0;
-  }
-  return NULL;
-}
-
 static int cmp_d(const void *a, const void *b) {
   const double x = *(const double *)a, y = *(const double *)b;
   return (x > y) - (x < y);
 }
 
-static void fill_src(bench_t *b, size_t bytes_per_pe) {
-  int me;
-  for (me = 0; me < b->P; ++me) {
-    size_t i;
-    if (b->dtype == F32 || b->dtype == F64) {
-      const size_t cnt = bytes_per_pe / esize(b->dtype);
-      for (i = 0; i < cnt; ++i) {
-        const uint64_t h = orc_hash3(0, (uint64_t)me, i);
-        const double u1 = (double)((h >> 11) + 1) * 0x1.0p-53;
-        const double u2 = (double)(mix64(h ^ 0xD1B54A32D192ED03ull) >> 11) * 0x1.0p-53;
-        const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
-        if (b->dtype == F32)
-          ((float *)b->src[me])[i] = (float)z;
-        else
-          ((double *)b->src[me])[i] = z;
-      }
-    } else {
-      uint64_t *p = (uint64_t *)b->src[me];
-      for (i = 0; i < bytes_per_pe / 8; ++i) p[i] = orc_hash3(0, (uint64_t)me, i);
-    }
+/* ---- CPU baseline with one process per PE ---------------------------------
+ * The reference runs one process per PE over UCX shared memory.  Here P
+ * forked processes share one MAP_SHARED mapping that holds every PE's
+ * source and destination slices, its pSync words and a barrier; a put or
+ * get is a memcpy into or out of the peer's slice (a lower bound on a UCX
+ * shared-memory copy), shmem_long_p a release store, and wait_until a
+ * spin with sched_yield (src/shmemc/ucx/waituntil.c:18-27).  PE i runs on
+ * CPU i mod C of the allowed set.  Children only touch this mapping and
+ * end with _exit.  Per iteration every PE times the call between the
+ * entry barrier and its return; the iteration's time is the maximum over
+ * PEs, and the result is the median over iterations. */
+enum { PS_WORDS = 64 };
+
+typedef struct {
+  long count, sense; /* sense-reversing barrier */
+  long fail;
+} mp_ctrl;
+
+typedef struct {
+  int kind, op, dtype, P, p2s, iters;
+  size_t n, nb, src_b, dest_b, slot;
+  local_fn f;
+  mp_ctrl *ctrl;
+  long *psync;   /* P x PS_WORDS */
+  double *times; /* (iters + 1) x P */
+  char *data;    /* P x slot: src then dest */
+} mp_t;
+
+static char *mp_src(mp_t *m, int pe) { return m->data + (size_t)pe * m->slot; }
+static char *mp_dest(mp_t *m, int pe) {
+  return m->data + (size_t)pe * m->slot + (m->src_b + 63) / 64 * 64;
+}
+static long *mp_ps(mp_t *m, int pe, int i) {
+  return m->psync + (size_t)pe * PS_WORDS + (size_t)i;
+}
+static void mp_put_flag(mp_t *m, int pe, int i, long v) {
+  __atomic_store_n(mp_ps(m, pe, i), v, __ATOMIC_RELEASE);
+}
+static void mp_wait_gt(mp_t *m, int me, int i, long v) {
+  while (__atomic_load_n(mp_ps(m, me, i), __ATOMIC_ACQUIRE) <= v) sched_yield();
+}
+static void mp_wait_ne(mp_t *m, int me, int i, long v) {
+  while (__atomic_load_n(mp_ps(m, me, i), __ATOMIC_ACQUIRE) == v) sched_yield();
+}
+
+static void mp_barrier(mp_t *m, long *local_sense) {
+  const long s = 1 - *local_sense;
+  *local_sense = s;
+  if (__atomic_add_fetch(&m->ctrl->count, 1, __ATOMIC_ACQ_REL) == m->P) {
+    __atomic_store_n(&m->ctrl->count, 0, __ATOMIC_RELAXED);
+    __atomic_store_n(&m->ctrl->sense, s, __ATOMIC_RELEASE);
+  } else {
+    while (__atomic_load_n(&m->ctrl->sense, __ATOMIC_ACQUIRE) != s) sched_yield();
   }
 }
 
-static int run_bench(bench_t *b, size_t src_bytes, size_t dest_bytes,
-                     double *sec_median) {
-  pthread_t *th;
-  worker_t *w;
-  int i, rc = 0;
-  b->src = (char **)calloc((size_t)b->P, sizeof(char *));
-  b->dest = (char **)calloc((size_t)b->P, sizeof(char *));
-  b->tmp = (char **)calloc((size_t)b->P, sizeof(char *));
-  b->times = (double *)calloc((size_t)b->iters, sizeof(double));
-  th = (pthread_t *)calloc((size_t)b->T, sizeof(pthread_t));
-  w = (worker_t *)calloc((size_t)b->T, sizeof(worker_t));
-  if (!b->src || !b->dest || !b->tmp || !b->times || !th || !w) {
-    rc = -1;
-    goto out;
+/* REDUCE_HELPER_REC_DBL (reduce.c:182-293) for PE me, with its handshakes */
+static int mp_rec_dbl(mp_t *m, int me) {
+  const int P = m->P, p2s = m->p2s;
+  const int me_p2s = member_p2s(me, P, p2s);
+  char *dest = mp_dest(m, me);
+  const char *src = mp_src(m, me);
+  char *tmp = NULL;
+  int rc = 0;
+  if (me_p2s != -1) {
+    tmp = (char *)malloc(m->nb ? m->nb : 1);
+    if (tmp == NULL) return -1;
   }
-  for (i = 0; i < b->P; ++i) {
-    b->src[i] = (char *)aligned_alloc(64, (src_bytes + 63) / 64 * 64);
-    b->dest[i] = (char *)aligned_alloc(64, (dest_bytes + 63) / 64 * 64);
-    if (!b->src[i] || !b->dest[i]) {
-      rc = -1;
-      goto out;
+  if (me_p2s == -1) {
+    mp_put_flag(m, me - 1, 0, 1);
+  } else if ((me + 1) * p2s / P == me_p2s) {
+    mp_wait_ne(m, me, 0, 0);
+    mp_put_flag(m, me, 0, 0);
+    memcpy(dest, mp_src(m, me + 1), m->nb);
+    rc |= m->f(m->op, tmp, dest, src, m->n);
+  } else {
+    memcpy(tmp, src, m->nb);
+  }
+  if (me_p2s != -1) {
+    int mask, i;
+    for (mask = 1, i = 1; mask < p2s; mask <<= 1, ++i) {
+      const int peer = ((me_p2s ^ mask) * P + p2s - 1) / p2s;
+      mp_put_flag(m, peer, i, 1);
+      mp_wait_gt(m, me, i, 0);
+      memcpy(mp_dest(m, peer), tmp, m->nb);
+      mp_put_flag(m, peer, i, 2);
+      mp_wait_gt(m, me, i, 1);
+      rc |= m->f(m->op, tmp, tmp, dest, m->n);
+      mp_put_flag(m, me, i, 0);
     }
-    memset(b->dest[i], 0, dest_bytes);
+    memcpy(dest, tmp, m->nb);
   }
-  fill_src(b, src_bytes);
-  pthread_barrier_init(&b->bar, NULL, (unsigned)b->T);
-  for (i = 0; i < b->T; ++i) {
-    w[i].b = b;
-    w[i].tid = i;
-    pthread_create(&th[i], NULL, bench_worker, &w[i]);
+  if (me_p2s == -1) {
+    mp_wait_ne(m, me, 0, 0);
+    mp_put_flag(m, me, 0, 0);
+  } else if ((me + 1) * p2s / P == me_p2s) {
+    memcpy(mp_dest(m, me + 1), dest, m->nb);
+    mp_put_flag(m, me + 1, 0, 1);
   }
-  for (i = 0; i < b->T; ++i) pthread_join(th[i], NULL);
-  pthread_barrier_destroy(&b->bar);
-  qsort(b->times, (size_t)b->iters, sizeof(double), cmp_d);
-  *sec_median = b->times[b->iters / 2];
-out:
-  if (b->src)
-    for (i = 0; i < b->P; ++i) {
-      free(b->src[i]);
-      free(b->dest[i]);
-    }
-  free(b->src);
-  free(b->dest);
-  free(b->tmp);
-  free(b->times);
-  free(th);
-  free(w);
-  free(b->mp);
+  free(tmp);
   return rc;
 }
 
-int orc_cpu_bench_reduce(int op, int dtype, int npes, size_t n, int iters,
-                         int nthreads, double *sec_median) {
-  bench_t b;
-  int me;
-  memset(&b, 0, sizeof(b));
-  if (npes < 1 || iters < 1 || nthreads < 1 || pick_local(dtype) == NULL) return -1;
-  b.kind = 0;
-  b.op = op;
-  b.dtype = dtype;
-  b.P = npes;
-  b.p2s = p2s_of(npes);
-  b.T = nthreads > npes ? npes : nthreads;
-  b.iters = iters;
-  b.n = n;
-  b.nb = n * esize(dtype);
-  b.f = pick_local(dtype);
-  b.mp = (int *)malloc(sizeof(int) * (size_t)npes);
-  if (b.mp == NULL) return -1;
-  for (me = 0; me < npes; ++me) b.mp[me] = member_p2s(me, npes, b.p2s);
-  return run_bench(&b, b.nb, b.nb, sec_median);
+/* fcollect_helper_bruck_inplace (fcollect.c:370-403) for PE me */
+static void mp_bruck(mp_t *m, int me) {
+  const int P = m->P;
+  const size_t total = (size_t)P * m->nb;
+  size_t distance, sent = m->nb;
+  int round;
+  char *dest = mp_dest(m, me);
+  memcpy(dest, mp_src(m, me), m->nb);
+  for (distance = 1, round = 0; distance < (size_t)P; distance <<= 1, ++round) {
+    const int peer = (int)(((size_t)me - distance + (size_t)P) % (size_t)P);
+    const size_t to_send = (2 * sent <= total) ? sent : total - sent;
+    memcpy(mp_dest(m, peer) + sent, dest, to_send);
+    mp_put_flag(m, peer, round, 1);
+    sent += distance * m->nb;
+    mp_wait_ne(m, me, round, 0);
+    mp_put_flag(m, me, round, 0);
+  }
+  orc_rotate_inplace(dest, total, (size_t)me * m->nb);
 }
 
-int orc_cpu_bench_fcollect(int npes, size_t nbytes, int iters, int nthreads,
-                           double *sec_median) {
-  bench_t b;
-  memset(&b, 0, sizeof(b));
-  if (npes < 1 || iters < 1 || nthreads < 1) return -1;
-  b.kind = 1;
-  b.P = npes;
-  b.T = nthreads > npes ? npes : nthreads;
-  b.iters = iters;
-  b.nb = nbytes;
-  b.dtype = U64;
-  return run_bench(&b, nbytes, nbytes * (size_t)npes, sec_median);
+/* alltoall_helper_shift_exchange_barrier (alltoall.c:75-110) for PE me,
+ * up to its final barrier */
+static void mp_shift(mp_t *m, int me) {
+  const int P = m->P;
+  int i;
+  memcpy(mp_dest(m, me) + (size_t)me * m->nb, mp_src(m, me) + (size_t)me * m->nb,
+         m->nb);
+  for (i = 1; i < P; ++i) {
+    const int peer = (me + i) % P;
+    memcpy(mp_dest(m, peer) + (size_t)me * m->nb,
+           mp_src(m, me) + (size_t)peer * m->nb, m->nb);
+  }
 }
 
-int orc_cpu_bench_alltoall(int npes, size_t nbytes, int iters, int nthreads,
-                           double *sec_median) {
-  bench_t b;
-  memset(&b, 0, sizeof(b));
-  if (npes < 1 || iters < 1 || nthreads < 1) return -1;
-  b.kind = 2;
-  b.P = npes;
-  b.T = nthreads > npes ? npes : nthreads;
-  b.iters = iters;
-  b.nb = nbytes;
-  b.dtype = U64;
-  return run_bench(&b, nbytes * (size_t)npes, nbytes * (size_t)npes, sec_median);
+/* PE i's seeded input (SplitMix64 bits, or N(0,1) by Box-Muller for
+ * floats), written by PE i itself */
+static void mp_fill(mp_t *m, int i) {
+  size_t k;
+  if (m->kind == 0 && (m->dtype == F32 || m->dtype == F64)) {
+    for (k = 0; k < m->n; ++k) {
+      const uint64_t h = orc_hash3(0, (uint64_t)i, k);
+      const double u1 = (double)((h >> 11) + 1) * 0x1.0p-53;
+      const double u2 = (double)(mix64(h ^ 0xD1B54A32D192ED03ull) >> 11) * 0x1.0p-53;
+      const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+      if (m->dtype == F32)
+        ((float *)mp_src(m, i))[k] = (float)z;
+      else
+        ((double *)mp_src(m, i))[k] = z;
+    }
+  } else {
+    for (k = 0; k + 8 <= m->src_b; k += 8) {
+      const uint64_t h = orc_hash3(0, (uint64_t)i, k / 8);
+      memcpy(mp_src(m, i) + k, &h, 8);
+    }
+  }
+  memset(mp_dest(m, i), 0, m->dest_b);
+}
+
+static void mp_child(mp_t *m, int me, int cpu, int fill) {
+  long sense = 0;
+  int it, rc = 0;
+  if (cpu >= 0) {
+    cpu_set_t cs;
+    CPU_ZERO(&cs);
+    CPU_SET(cpu, &cs);
+    (void)sched_setaffinity(0, sizeof(cs), &cs);
+  }
+  if (fill) mp_fill(m, me);
+  for (it = 0; it < m->iters + 1; ++it) {
+    double t0;
+    mp_barrier(m, &sense);
+    t0 = now_s();
+    if (m->kind == 0) {
+      /* the team wrapper's pWrk malloc and free: one WORLD barrier each
+       * (reduce.c:1090-1100, shmalloc.c:56,131) */
+      mp_barrier(m, &sense);
+      rc |= mp_rec_dbl(m, me);
+      mp_barrier(m, &sense);
+    } else if (m->kind == 1) {
+      mp_barrier(m, &sense); /* the wrapper's team sync (fcollect.c:574) */
+      mp_bruck(m, me);
+    } else {
+      mp_shift(m, me);
+      mp_barrier(m, &sense); /* quiet + the final barrier (alltoall.c:109) */
+    }
+    m->times[(size_t)it * (size_t)m->P + (size_t)me] = now_s() - t0;
+  }
+  if (rc) __atomic_store_n(&m->ctrl->fail, 1, __ATOMIC_RELAXED);
+  _exit(0);
+}
+
+/* kind 0: rec_dbl reduce of n elements of dtype with op; 1: Bruck fcollect
+ * of n bytes per PE; 2: shift-exchange alltoall of n-byte blocks.  src
+ * (optional): P rows of the per-PE input (else the seeded generator);
+ * dest_out (optional): receives every PE's destination after the last
+ * iteration.  Returns 0, -1 on bad arguments, -2 if the mapping or a fork
+ * failed. */
+int orc_cpu_bench_procs(int kind, int op, int dtype, int npes, size_t n,
+                        int iters, const void *const *src,
+                        void *const *dest_out, double *sec_median) {
+  mp_t m;
+  size_t ctrl_b, ps_b, tm_b, total;
+  char *map;
+  pid_t *pids;
+  int i, ncpu = 0, rc = 0, it;
+  int cpus[1024];
+  cpu_set_t cs;
+  memset(&m, 0, sizeof(m));
+  if (npes < 1 || iters < 1 || kind < 0 || kind > 2 || sec_median == NULL)
+    return -1;
+  m.kind = kind;
+  m.op = op;
+  m.dtype = kind == 0 ? dtype : U8;
+  m.P = npes;
+  m.p2s = p2s_of(npes);
+  m.iters = iters;
+  m.n = n;
+  if (kind == 0) {
+    m.f = pick_local(dtype);
+    if (m.f == NULL) return -1;
+    m.nb = n * esize(dtype);
+    m.src_b = m.dest_b = m.nb;
+  } else {
+    m.nb = n;
+    m.src_b = kind == 1 ? n : n * (size_t)npes;
+    m.dest_b = n * (size_t)npes;
+  }
+  if (kind != 0 && npes > PS_WORDS * 64) return -1;
+  m.slot = (m.src_b + 63) / 64 * 64 + (m.dest_b + 63) / 64 * 64;
+  ctrl_b = 4096;
+  ps_b = (size_t)npes * PS_WORDS * sizeof(long);
+  tm_b = (size_t)(iters + 1) * (size_t)npes * sizeof(double);
+  total = ctrl_b + (ps_b + 4095) / 4096 * 4096 + (tm_b + 4095) / 4096 * 4096 +
+          (size_t)npes * m.slot;
+  map = (char *)mmap(NULL, total, PROT_READ | PROT_WRITE,
+                     MAP_SHARED | MAP_ANONYMOUS, -1, 0);
+  if (map == MAP_FAILED) return -2;
+  m.ctrl = (mp_ctrl *)map;
+  m.psync = (long *)(map + ctrl_b);
+  m.times = (double *)(map + ctrl_b + (ps_b + 4095) / 4096 * 4096);
+  m.data = map + ctrl_b + (ps_b + 4095) / 4096 * 4096 + (tm_b + 4095) / 4096 * 4096;
+  if (src != NULL)
+    for (i = 0; i < npes; ++i) {
+      memcpy(mp_src(&m, i), src[i], m.src_b);
+      memset(mp_dest(&m, i), 0, m.dest_b);
+    }
+  if (sched_getaffinity(0, sizeof(cs), &cs) == 0)
+    for (i = 0; i < CPU_SETSIZE && ncpu < 1024; ++i)
+      if (CPU_ISSET(i, &cs)) cpus[ncpu++] = i;
+  pids = (pid_t *)calloc((size_t)npes, sizeof(pid_t));
+  if (pids == NULL) {
+    munmap(map, total);
+    return -2;
+  }
+  fflush(NULL);
+  for (i = 0; i < npes; ++i) {
+    pids[i] = fork();
+    if (pids[i] == 0)
+      mp_child(&m, i, ncpu > 0 ? cpus[i % ncpu] : -1, src == NULL);
+    if (pids[i] < 0) {
+      rc = -2;
+      break;
+    }
+  }
+  if (rc != 0) { /* cannot run: stop the PEs already started */
+    int k;
+    for (k = 0; k < i; ++k) kill(pids[k], SIGKILL);
+  }
+  for (i = 0; i < npes; ++i) {
+    int st;
+    if (pids[i] > 0) {
+      waitpid(pids[i], &st, 0);
+      if (!WIFEXITED(st) || WEXITSTATUS(st) != 0) rc = rc ? rc : -2;
+    }
+  }
+  if (rc == 0 && m.ctrl->fail) rc = -1;
+  if (rc == 0) {
+    double *per = (double *)malloc(sizeof(double) * (size_t)iters);
+    if (per == NULL) {
+      rc = -2;
+    } else {
+      for (it = 1; it <= iters; ++it) {
+        double mx = 0.0;
+        for (i = 0; i < npes; ++i) {
+          const double t = m.times[(size_t)it * (size_t)npes + (size_t)i];
+          if (t > mx) mx = t;
+        }
+        per[it - 1] = mx;
+      }
+      qsort(per, (size_t)iters, sizeof(double), cmp_d);
+      *sec_median = per[iters / 2];
+      free(per);
+    }
+    if (dest_out != NULL)
+      for (i = 0; i < npes; ++i) memcpy(dest_out[i], mp_dest(&m, i), m.dest_b);
+  }
+  free(pids);
+  munmap(map, total);
+  return rc;
 }

@@ -19,8 +19,8 @@
  * plus high-precision references for float SUM/PROD:
  *   orc_reduce_accurate  exact (Shewchuk expansion) sums, double-double
  *                        products; result as an unevaluated hi + lo pair
- * and a threaded CPU timing harness of the same reference algorithms
- * (orc_cpu_bench_*), used as the CPU baseline.
+ * and a CPU timing harness of the same reference algorithms with one forked
+ * process per PE (orc_cpu_bench_procs), used as the CPU baseline.
  *
  * Only the tests, the smoke check and the benchmark's CPU-baseline leg use
  * this library; the product (libosc) never links or calls it.
@@ -90,14 +90,16 @@ uint64_t orc_hash3(uint64_t seed, uint64_t pe, uint64_t idx);
 /* CPU baseline: run the reference algorithm with npes PEs spread over
  * nthreads threads (barrier between rounds), `iters` timed repetitions on
  * seeded inputs; *sec_median receives the median seconds per call. */
-int orc_cpu_bench_reduce(int op, int dtype, int npes, size_t n, int iters,
-                         int nthreads, double *sec_median);
-int orc_cpu_bench_fcollect(int npes, size_t nbytes, int iters, int nthreads,
-                           double *sec_median);
-int orc_cpu_bench_alltoall(int npes, size_t nbytes, int iters, int nthreads,
-                           double *sec_median);
 
 #ifdef __cplusplus
 }
 #endif
+/* CPU baseline with one forked process per PE over one MAP_SHARED
+ * mapping (kind 0: rec_dbl reduce of n elements; 1: Bruck fcollect of n
+ * bytes; 2: shift-exchange alltoall of n-byte blocks); median over iters of
+ * the slowest PE's time.  src / dest_out optional (P rows each). */
+int orc_cpu_bench_procs(int kind, int op, int dtype, int npes, size_t n,
+                        int iters, const void *const *src,
+                        void *const *dest_out, double *sec_median);
+
 #endif

@@ -116,6 +116,30 @@ def test_strided_copy_negative_and_zero_strides():
 def test_cpu_timing_harness_runs_every_collective():
     """The CPU baseline's timing entry points run and return a positive
     median time at tiny sizes."""
-    assert O.cpu_bench_reduce("sum", "i32", 3, 1000, 3, 2) > 0
-    assert O.cpu_bench_alltoall(5, 256, 3, 2) > 0
-    assert O.cpu_bench_fcollect(5, 256, 3, 2) > 0
+    assert O.cpu_bench_procs("reduce", 3, 1000, 3, "sum", "i32") > 0
+    assert O.cpu_bench_procs("alltoall", 5, 256, 3) > 0
+    assert O.cpu_bench_procs("fcollect", 5, 256, 3) > 0
+
+
+def test_process_per_pe_baseline_reproduces_the_oracle():
+    """The CPU baseline's process-per-PE run (fork, one MAP_SHARED heap,
+    pSync handshakes, memcpy puts) leaves the oracle's results in every
+    PE's destination: recursive doubling (power-of-two and folded teams),
+    Bruck fcollect with its rotation, shift-exchange alltoall."""
+    rng = np.random.default_rng(8)
+    for P in (1, 3, 6, 8):
+        x = rng.standard_normal((P, 777)).astype(np.float32)
+        _, d = O.cpu_bench_procs("reduce", P, 777, 2, "sum", "f32", src=x,
+                                 want_dest=True)
+        assert np.array_equal(d.view(np.uint32),
+                              O.rec_dbl("sum", "f32", x).view(np.uint32))
+        y = rng.integers(-2 ** 40, 2 ** 40, (P, 333))
+        _, d = O.cpu_bench_procs("reduce", P, 333, 1, "max", "i64", src=y,
+                                 want_dest=True)
+        assert np.array_equal(d.view(np.int64), O.rec_dbl("max", "i64", y))
+    b = rng.integers(0, 256, (7, 40), dtype=np.uint8)
+    _, d = O.cpu_bench_procs("fcollect", 7, 40, 2, src=b, want_dest=True)
+    assert np.array_equal(d, O.fcollect(b))
+    b = rng.integers(0, 256, (5, 5 * 24), dtype=np.uint8)
+    _, d = O.cpu_bench_procs("alltoall", 5, 24, 2, src=b, want_dest=True)
+    assert np.array_equal(d, O.alltoall(b))
