@@ -269,6 +269,11 @@ def cpu_bench_procs(kind: str, npes: int, n: int, iters: int, op="sum",
     return (sec.value, dest) if want_dest else sec.value
 
 
+# ---- independent numpy formulations (pins) ----------------------------------
+
+_G = np.uint64(0x9E3779B97F4A7C15)
+
+
 def _mix64(z: np.ndarray) -> np.ndarray:
     z = z.astype(np.uint64)
     with np.errstate(over="ignore"):
