@@ -438,6 +438,82 @@ extern "C" int oscd_collect(const oscd_geom *g, uint64_t dest_off,
   return check_launch();
 }
 
+// ---- replicate: member 0's bytes to every member -----------------------------
+// The second half of a large reduction (the walk stores member 0's result
+// only).  Members are taken in groups of G consecutive ones; virtual block
+// vb -> group vb % ngroups (fastest), chunk vb / ngroups of 32 KiB.  A
+// workgroup visit loads the chunk from member 0 once (cached: every group
+// reads it) and stores it to the G members of its group with non-temporal
+// stores.  Measured on MI355X (64 destinations of 16 and 32 MiB): G = 8
+// writes at 5.3-5.5 TB/s where one destination per visit gets 4.3-4.9
+// (loads or not) and a workgroup fanning out to all 64 (the slice walk's
+// own write phase) 5.1-5.3.  Member 0 is rewritten with its own bytes:
+// harmless (the values do not change) and it keeps the groups whole.
+namespace osc {
+
+template <int UPT, int G>
+__global__ __launch_bounds__(kMoveBlock) void k_replicate(
+    const oscd_geom g, const uint64_t off, const uint64_t nunits,
+    const uint64_t ngroups, const uint64_t total) {
+  const int P = g.npes;
+  constexpr uint64_t kChunk = (uint64_t)kMoveBlock * UPT;  // units
+  const char *src = member_ptr(g, 0, off);
+  for (uint64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const int d0 = (int)(vb % ngroups) * G;
+    const uint64_t u0 = (vb / ngroups) * kChunk + threadIdx.x;
+    typename Raw<16>::type v[UPT];
+#pragma unroll
+    for (int k = 0; k < UPT; ++k) {
+      const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
+      if (u < nunits) v[k] = ld<16>(src + u * 16);
+    }
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      if (d0 + q >= P) break;  // the last group may be short
+      char *dst = member_ptr(g, d0 + q, off);
+#pragma unroll
+      for (int k = 0; k < UPT; ++k) {
+        const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
+        if (u < nunits) st_nt<16>(dst + u * 16, v[k]);
+      }
+    }
+  }
+}
+
+template <int G>
+static int replicate_g(const oscd_geom *g, uint64_t off, uint64_t nbytes,
+                       hipStream_t s) {
+  constexpr int UPT = 8;  // 32 KiB per workgroup visit
+  const uint64_t nunits = nbytes / 16;
+  const uint64_t nch = (nunits + kMoveBlock * UPT - 1) / (kMoveBlock * UPT);
+  const uint64_t ngroups = ((uint64_t)g->npes + G - 1) / G;
+  const uint64_t total = nch * ngroups;
+  // about 4096 workgroups (64 per destination at 64 members), at least one
+  // pass over the groups
+  uint64_t grid = 4096;
+  if (grid > total) grid = total;
+  hipLaunchKernelGGL((k_replicate<UPT, G>), dim3((unsigned)grid),
+                     dim3(kMoveBlock), 0, s, *g, off, nunits, ngroups, total);
+  return check_launch();
+}
+
+}  // namespace osc
+
+extern "C" int oscd_replicate(const oscd_geom *g, uint64_t off, uint64_t nbytes,
+                              int group, void *stream) {
+  using namespace osc;
+  if (g == nullptr || g->npes <= 0 || nbytes % 16 != 0) return OSC_ERR_ARG;
+  if (nbytes == 0 || g->npes == 1) return OSC_OK;
+  hipStream_t s = (hipStream_t)stream;
+  switch (group) {
+    case 1: return replicate_g<1>(g, off, nbytes, s);
+    case 2: return replicate_g<2>(g, off, nbytes, s);
+    case 4: return replicate_g<4>(g, off, nbytes, s);
+    case 16: return replicate_g<16>(g, off, nbytes, s);
+    default: return replicate_g<8>(g, off, nbytes, s);
+  }
+}
+
 // ---- allocation-free collect ------------------------------------------------
 // The (offset, size) table of a collect in a file-scope device array
 // instead of memory the library allocates.  Each GPU holds its own copy of

@@ -497,7 +497,11 @@ __host__ __device__ constexpr int shape_infl(int s) {
 // plus a 32-bit lane offset (< 32 KiB), so the loads and stores use the
 // scalar-base addressing form.  FAST: every unit of the block is complete
 // and in range (all blocks but the last), so no per-unit guards.
-template <bool FAST, class T, class Pol, int VEC, int UPT, int INFL, int MAXLV>
+// ONE: store the result to member 0 only (the replicate kernel copies it to
+// every member afterwards); never with a SPECIAL policy, whose special
+// elements have per-member results.
+template <bool FAST, class T, class Pol, int VEC, int UPT, int INFL, int MAXLV,
+          bool ONE = false>
 __device__ __forceinline__ void reduce_block(const oscd_geom &g,
                                              const uint64_t dest_off,
                                              const uint64_t src_off,
@@ -746,7 +750,8 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
   }
 
   if (any) {
-    for (int j = 0; j < P; ++j) {
+    static_assert(!(ONE && Pol::SPECIAL), "per-member results");
+    for (int j = 0; j < (ONE ? 1 : P); ++j) {
       char *dp = d0 + j * step;
 #pragma unroll
       for (int k = 0; k < UPT; ++k)
@@ -797,7 +802,8 @@ constexpr int kernel_waves() {
   return (UPT == 8 && INFL == 1 && Pol::WAVES81 > 0) ? Pol::WAVES81 : 1;
 }
 
-template <class T, class Pol, int VEC, int UPT, int INFL, int MAXLV>
+template <class T, class Pol, int VEC, int UPT, int INFL, int MAXLV,
+          bool ONE = false>
 __global__ __launch_bounds__(kBlock) __attribute__((
     amdgpu_waves_per_eu(kernel_waves<Pol, UPT, INFL>()))) void k_reduce(const oscd_geom g,
                                                    const uint64_t dest_off,
@@ -817,12 +823,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((
   const uint64_t nspan = nfull / kSpan;
   const uint64_t b = blockIdx.x;
   if (b < nspan) {
-    reduce_block<true, T, Pol, VEC, UPT, INFL, MAXLV>(
+    reduce_block<true, T, Pol, VEC, UPT, INFL, MAXLV, ONE>(
         g, dest_off, src_off, n, lg, b * kSpan, (T *)k_reduce_lds);
   } else {
     const uint64_t sub = nspan * kSpan + (b - nspan) * kBlock;
     if (sub < nunits)
-      reduce_block<false, T, Pol, VEC, 1, 8, MAXLV>(
+      reduce_block<false, T, Pol, VEC, 1, 8, MAXLV, ONE>(
           g, dest_off, src_off, n, lg, sub, (T *)k_reduce_lds);
   }
 }
@@ -1169,7 +1175,7 @@ constexpr int vec_of() {
   return (16 / (int)sizeof(T)) > 8 ? 8 : (16 / (int)sizeof(T));
 }
 
-template <class T, class Pol, int VEC, int UPT, int INFL>
+template <class T, class Pol, int VEC, int UPT, int INFL, bool ONE = false>
 static void launch_shape(const oscd_geom *g, int lg, uint64_t doff,
                          uint64_t soff, uint64_t n, size_t lds,
                          hipStream_t s) {
@@ -1182,15 +1188,38 @@ static void launch_shape(const oscd_geom *g, int lg, uint64_t doff,
   const uint64_t grid = nspan + (units - nspan * span + kBlock - 1) / kBlock;
   if constexpr (Pol::TREE) {
     if (lg >= 8) {
-      hipLaunchKernelGGL((k_reduce<T, Pol, VEC, UPT, INFL, 13>),
+      hipLaunchKernelGGL((k_reduce<T, Pol, VEC, UPT, INFL, 13, ONE>),
                          dim3((unsigned)grid), dim3(kBlock), lds, s, *g, doff,
                          soff, n, lg);
       return;
     }
   }
-  hipLaunchKernelGGL((k_reduce<T, Pol, VEC, UPT, INFL, Pol::TREE ? 8 : 1>),
+  hipLaunchKernelGGL((k_reduce<T, Pol, VEC, UPT, INFL, Pol::TREE ? 8 : 1, ONE>),
                      dim3((unsigned)grid), dim3(kBlock), lds, s, *g, doff,
                      soff, n, lg);
+}
+
+// Large calls of 8- and 16-byte elements run in two kernels: the slice
+// walk stores the result to member 0 only, then oscd_replicate copies it to
+// every member.  Measured on MI355X at 64 members x 4M elements, both
+// variants alternating in one process: the walk's own write phase (every
+// workgroup fanning its 32 KiB span out to all 64 members) writes at
+// 5.1-5.3 TB/s, while the walk without the fan-out reads at 6.8 TB/s and
+// the replicate kernel writes at 5.3-5.5; u64/i64 cells gain 0.5-2%, f64
+// sum 5.35-5.43 -> 5.70 TB/s (+6%), f64 prod +2%, and the split cells stay
+// level over a run where the fused ones drift down by 1-3%.  f32 sum and
+// prod lose 1-1.7% (the fused f32 walk is the faster one: 5.86-5.95
+// TB/s), so 4-byte and narrower types keep one kernel.  The split costs one
+// extra read of n*s bytes (1/P of the traffic) and one launch.  Not for
+// per-member results (SPECIAL: float MIN/MAX).
+constexpr int kSplitMinPes = 8;
+constexpr uint64_t kSplitMinBytes = 1ull << 20;  // per member
+extern int g_split_group;  // 0: no split; else the replicate group size
+template <class T, class Pol>
+static bool split_pays(const oscd_geom *g, uint64_t n) {
+  return g_split_group > 0 && !Pol::SPECIAL && sizeof(T) >= 8 &&
+         g->npes >= kSplitMinPes && n * sizeof(T) >= kSplitMinBytes &&
+         (n * sizeof(T)) % 16 == 0;
 }
 
 // Default shape per policy (Pol::SHAPE), from the 64-PE x 4M sweeps on
@@ -1239,6 +1268,25 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
     shape = spans < kSmallSpans ? 3 : d;
   }
   shape = clamp_shape<Pol>(shape);
+  if constexpr (!Pol::SPECIAL) {
+    if (vec_ok && split_pays<T, Pol>(g, n)) {
+      if constexpr (Pol::TREE) {
+        if (shape == 3)
+          launch_shape<T, Pol, VB, shape_upt(3), shape_infl(3), true>(g, lg, doff, soff, n, 0, s);
+        else
+          launch_shape<T, Pol, VB, shape_upt(2), shape_infl(2), true>(g, lg, doff, soff, n, 0, s);
+      } else {
+        switch (shape) {
+          case 0: launch_shape<T, Pol, VB, shape_upt(0), shape_infl(0), true>(g, lg, doff, soff, n, 0, s); break;
+          case 1: launch_shape<T, Pol, VB, shape_upt(1), shape_infl(1), true>(g, lg, doff, soff, n, 0, s); break;
+          case 2: launch_shape<T, Pol, VB, shape_upt(2), shape_infl(2), true>(g, lg, doff, soff, n, 0, s); break;
+          default: launch_shape<T, Pol, VB, shape_upt(3), shape_infl(3), true>(g, lg, doff, soff, n, 0, s); break;
+        }
+      }
+      if (hipGetLastError() != hipSuccess) return OSC_ERR_HIP;
+      return oscd_replicate(g, doff, n * sizeof(T), g_split_group, s);
+    }
+  }
   if (!vec_ok) {
     launch_shape<T, Pol, 1, 1, 8>(g, lg, doff, soff, n, lds, s);
   } else if constexpr (Pol::TREE) {

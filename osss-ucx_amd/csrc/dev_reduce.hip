@@ -21,6 +21,15 @@ int visit_reduce(int dtype, int op, int mode, PerCallV *pc, BatchV *bv,
 
 }  // namespace osc
 
+namespace osc {
+int g_split_group = 8;
+}  // namespace osc
+
+extern "C" void oscd_set_reduce_split(int enabled, int group) {
+  osc::g_split_group =
+      !enabled ? 0 : (group == 1 || group == 2 || group == 4 || group == 16) ? group : 8;
+}
+
 static int log2_floor(int p) {
   int lg = 0;
   while ((2 << lg) <= p) ++lg;

@@ -85,6 +85,11 @@ int osc_init(int npes, size_t heap_bytes_per_pe, int device) {
   osc_g.log_calls = (lg != NULL && lg[0] != '\0' && lg[0] != '0');
   osc_g.reduce_shape = (shp != NULL && shp[0] != '\0') ? atoi(shp) : -1;
   {
+    const char *sp = getenv("OSC_REDUCE_SPLIT");
+    if (sp == NULL || sp[0] == '\0' || osc_set_reduce_split(atoi(sp)) != OSC_OK)
+      osc_set_reduce_split(8);
+  }
+  {
     const char *br = getenv("OSC_BATCH_BRANCHES");
     osc_g.batch_branches = (br != NULL && br[0] != '\0') ? atoi(br) : 4;
     if (osc_g.batch_branches < 1) osc_g.batch_branches = 1;
@@ -190,6 +195,14 @@ int osc_get_reduce_shape(void) { return osc_g.reduce_shape; }
 int osc_set_batch_branches(int n) {
   if (n < 1 || n > 8) return osc_set_error(OSC_ERR_ARG, "batch branches %d (1..8)", n);
   osc_g.batch_branches = n;
+  return OSC_OK;
+}
+int osc_set_reduce_split(int group) {
+  if (group != 0 && group != 1 && group != 2 && group != 4 && group != 8 &&
+      group != 16)
+    return osc_set_error(OSC_ERR_ARG, "replicate group %d (0, 1, 2, 4, 8, 16)",
+                         group);
+  oscd_set_reduce_split(group != 0, group);
   return OSC_OK;
 }
 int osc_npes(void) { return osc_g.npes; }

@@ -1,0 +1,134 @@
+"""Large reductions of 8- and 16-byte elements run in two kernels: the
+slice walk stores member 0's result only, then a replicate kernel copies it
+to every member, each loaded chunk to a group of consecutive members.
+Checked against the oracle just above and below the size where the split
+starts (1 MiB per member, at least 8 members, a multiple of 16 bytes), on
+power-of-two and odd teams, a strided team, in place, in both float modes,
+with the bytes after each destination untouched; 4-byte and narrower types
+(one kernel) at the same sizes; and bit-identical to the one-kernel path for
+every replicate group size, on teams that are not a multiple of the group."""
+import zlib
+
+import numpy as np
+import pytest
+
+import osc
+import oracle as O
+from gpu_util import TYPENAME, bit_equal, inputs, max_rel_err
+
+pytestmark = pytest.mark.gpu
+
+WORLD = 64
+_H = []
+
+
+@pytest.fixture
+def heap():
+    if not _H or not _H[0].alive:
+        _H[:] = [osc.Heap(npes=WORLD, heap_bytes=12 << 20)]
+    return _H[0]
+
+
+def reduce_checked(h, team, pes, op, dtype, src, mode, inplace=False):
+    P, n = src.shape
+    es = src.dtype.itemsize
+    tail = 4096
+    s = h.malloc(n * es + tail)
+    d = s if inplace else h.malloc(n * es + tail)
+    guard = np.frombuffer(np.random.default_rng(9).bytes(tail), np.uint8)
+    try:
+        for k, pe in enumerate(pes):
+            h.put(s, src[k], pe)
+            h.put(d + n * es, guard, pe)
+        osc.check(h.L.osc_set_fp_mode(mode), "mode")
+        rc = osc.typed(f"shmem_{TYPENAME[dtype]}_{op}_reduce")(team, d, s, n)
+        osc.check(rc, "reduce")
+        got = np.stack([h.get(d, n, src.dtype, pe) for pe in pes])
+        for pe in pes:
+            assert np.array_equal(h.get(d + n * es, tail, np.uint8, pe), guard)
+        return got
+    finally:
+        h.L.osc_set_fp_mode(0)
+        h.free(s)
+        if not inplace:
+            h.free(d)
+
+
+CASES = [  # (dtype, op, n): at, above and off the 1 MiB/member threshold
+    ("i64", "sum", 131072), ("i64", "sum", 131072 + 2 * 777),
+    ("u64", "xor", 131074), ("i64", "prod", 140000),
+    ("f64", "sum", 200002), ("f64", "prod", 150000),
+    ("f64", "sum", 131071),  # below 1 MiB per member: one kernel
+    ("c64", "prod", 70001), ("c64", "sum", 65535),  # 16-byte elements
+    ("c32", "sum", 140000), ("c32", "prod", 131073),
+    ("f32", "sum", 300004), ("f32", "prod", 262144),  # 4 bytes: one kernel
+    ("u8", "xor", (1 << 20) + 48), ("i32", "max", 262143 * 4),
+]
+
+
+@pytest.mark.parametrize("dtype,op,n", CASES)
+@pytest.mark.parametrize("team", [(0, 1, 8), (3, 1, 19), (0, 1, 64),
+                                  (1, 3, 13)])
+def test_split_reduce_matches_oracle(heap, dtype, op, n, team):
+    start, stride, P = team
+    pes = [start + k * stride for k in range(P)]
+    t = heap.world if P == WORLD else heap.split(start, stride, P)
+    rng = np.random.default_rng(zlib.crc32(repr((dtype, op, n, team)).encode()))
+    src = inputs(rng, op, dtype, P, n)
+    modes = [osc.FP_ACCURATE]
+    if dtype[0] in "fc":
+        modes.append(osc.FP_REFERENCE)
+    for mode in modes:
+        got = reduce_checked(heap, t, pes, op, dtype, src, mode,
+                             inplace=(mode == osc.FP_REFERENCE))
+        if dtype[0] in "iu" or mode == osc.FP_REFERENCE:
+            assert bit_equal(got, O.rec_dbl(op, dtype, src), dtype), mode
+        else:
+            tol = 2.0 ** -24 * 1.001 if dtype in ("f32", "c32") else \
+                2.0 ** -53 * 1.01
+            assert max_rel_err(got, src, op, dtype) <= tol
+            assert bit_equal(got, np.broadcast_to(got[:1], got.shape), dtype)
+
+
+def test_split_reduce_under_each_algorithm(heap):
+    """linear / binomial / rabenseifner orders survive the split."""
+    rng = np.random.default_rng(77)
+    for mode, ref in [(osc.FP_LINEAR, O.linear), (osc.FP_BINOMIAL, O.binomial),
+                      (osc.FP_RABENSEIFNER, O.rabenseifner)]:
+        for dtype in ("f64", "c32"):
+            src = inputs(rng, "sum", dtype, 19, 131072 + 6)
+            got = reduce_checked(heap, heap.split(2, 1, 19), range(2, 21),
+                                 "sum", dtype, src, mode)
+            assert bit_equal(got, ref("sum", dtype, src), dtype), (mode, dtype)
+
+
+@pytest.mark.parametrize("dtype,op,n", [("f64", "sum", 200002),
+                                        ("i64", "prod", 131072),
+                                        ("c64", "prod", 65536 + 2)])
+@pytest.mark.parametrize("team", [(0, 1, 8), (1, 2, 13), (0, 1, 64),
+                                  (5, 1, 37)])
+def test_split_matches_one_kernel_for_every_group(heap, dtype, op, n, team):
+    """Every replicate group size gives the one-kernel path's bits, on teams
+    of 8, 13, 37 and 64 members (partial last groups of 1..8 members)."""
+    start, stride, P = team
+    pes = [start + k * stride for k in range(P)]
+    t = heap.world if P == WORLD else heap.split(start, stride, P)
+    rng = np.random.default_rng(zlib.crc32(repr((dtype, op, n, team)).encode()))
+    src = inputs(rng, op, dtype, P, n)
+    try:
+        osc.check(heap.L.osc_set_reduce_split(0), "split off")
+        want = reduce_checked(heap, t, pes, op, dtype, src, osc.FP_ACCURATE)
+        for group in (1, 2, 4, 8, 16):
+            osc.check(heap.L.osc_set_reduce_split(group), "split group")
+            got = reduce_checked(heap, t, pes, op, dtype, src, osc.FP_ACCURATE)
+            assert bit_equal(got, want, dtype), group
+    finally:
+        heap.L.osc_set_reduce_split(8)
+    if dtype[0] == "i":
+        assert bit_equal(want, O.rec_dbl(op, dtype, src), dtype)
+
+
+def test_split_setting_rejects_other_groups(heap):
+    for bad in (-1, 3, 5, 32):
+        assert heap.L.osc_set_reduce_split(bad) == -1  # OSC_ERR_ARG
+    assert heap.L.osc_set_reduce_split(8) == 0
