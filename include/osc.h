@@ -137,12 +137,13 @@ int osc_get_reduce_shape(void);
 /* Parallel branches per wave in a batch's captured graph, 1..8 (default 4,
  * OSC_BATCH_BRANCHES sets it at init); takes effect at the next capture. */
 int osc_set_batch_branches(int n);
-/* Large reductions of 8- and 16-byte elements (1 MiB or more per member, 8+
- * members, non-per-member ops) run as a walk that stores member 0's result
- * and a replicate kernel that copies it to the other members, storing each
- * loaded 32 KiB chunk to `group` consecutive members: group 1, 2, 4, 8
- * (default) or 16; 0 = one kernel that stores to every member.  Results
- * are identical either way.  OSC_REDUCE_SPLIT sets it at init. */
+/* Large reductions (1 MiB or more per member, 8+ members, elements of 2, 4,
+ * 8 or 16 bytes, operators without per-member results) run as a walk that
+ * stores member 0's result and a replicate kernel that copies it to the
+ * other members, storing each loaded piece to `group` members: -1
+ * (default) picks 4 or 8 by size, 1, 2, 4, 8 or 16 fix it; 0 = one kernel
+ * that stores to every member.  Results are identical either way.
+ * OSC_REDUCE_SPLIT sets it at init. */
 int osc_set_reduce_split(int group);
 const char *osc_strerror(int code);
 const char *osc_last_error(void);

@@ -440,42 +440,63 @@ extern "C" int oscd_collect(const oscd_geom *g, uint64_t dest_off,
 
 // ---- replicate: member 0's bytes to every member -----------------------------
 // The second half of a large reduction (the walk stores member 0's result
-// only).  Members are taken in groups of G consecutive ones; virtual block
-// vb -> group vb % ngroups (fastest), chunk vb / ngroups of 32 KiB.  A
-// workgroup visit loads the chunk from member 0 once (cached: every group
-// reads it) and stores it to the G members of its group with non-temporal
-// stores.  Measured on MI355X (64 destinations of 16 and 32 MiB): G = 8
-// writes at 5.3-5.5 TB/s where one destination per visit gets 4.3-4.9
-// (loads or not) and a workgroup fanning out to all 64 (the slice walk's
-// own write phase) 5.1-5.3.  Member 0 is rewritten with its own bytes:
-// harmless (the values do not change) and it keeps the groups whole.
+// only): members 1..P-1 receive member 0's [off, off + nbytes).
+//
+// The kernel is write-bound, and HBM write bandwidth depends on the order in
+// which the chip's workgroups issue their stores.  Measured on MI355X with
+// 2 GiB of non-temporal 16-byte stores: each workgroup writing its own
+// contiguous 32 KiB runs at 5.2-5.4 TB/s, while S = 32 consecutive
+// workgroups that interleave 1 KiB pieces (workgroup x writes pieces x,
+// x+S, x+2S, ... of a 32*S KiB range) run at 6.3-6.5 TB/s; destinations
+// that change every 32 KiB (the older replicate order) stay at 5.5-5.8
+// whatever the piece size.  Reads gain nothing from the interleave (7.0 TB/s
+// either way).
+//
+// So here the S workgroups of an interleave group split one super-chunk of
+// S*32 KiB of every destination into 1 KiB pieces (one wave's 64 lanes x 16
+// bytes; wave w's k-th piece is k*4 + w), load each piece from member 0
+// once (cached: the next member group of the same super-chunk re-reads it
+// soon after) and store it to a group of G destinations.  Virtual block vb
+// -> x = vb % S, then super-chunk and member group with the member group
+// fastest (offset-major).  Measured (64 members x 32 MiB / 16 MiB, 128 x
+// 16 MiB, 16 x 32 MiB, 8 x 64 MiB): 320 / 157 / 300 / 89 / 94 us against
+// 393 / 200 / 402 / 109 / 112 us for the older order.
 namespace osc {
 
-template <int UPT, int G>
+constexpr int kRepS = 16;                     // workgroups per interleave group
+constexpr uint64_t kRepPiece = 64;            // units (1 KiB) per piece
+constexpr uint64_t kRepWgUnits = kMoveBlock * 8;  // 2048 units per visit
+constexpr uint64_t kRepMaxGrid = 1u << 20;    // a multiple of kRepS
+
+template <int G>
 __global__ __launch_bounds__(kMoveBlock) void k_replicate(
     const oscd_geom g, const uint64_t off, const uint64_t nunits,
     const uint64_t ngroups, const uint64_t total) {
   const int P = g.npes;
-  constexpr uint64_t kChunk = (uint64_t)kMoveBlock * UPT;  // units
+  const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
   const char *src = member_ptr(g, 0, off);
+  // gridDim.x is a multiple of kRepS, so every visit of a workgroup keeps
+  // its place x in the interleave group
   for (uint64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
-    const int d0 = (int)(vb % ngroups) * G;
-    const uint64_t u0 = (vb / ngroups) * kChunk + threadIdx.x;
-    typename Raw<16>::type v[UPT];
+    const uint64_t x = vb % kRepS, r = vb / kRepS;
+    const uint64_t gi = r % ngroups, sc = r / ngroups;
+    const uint64_t scu = sc * (uint64_t)kRepS * kRepWgUnits;
+    uint64_t u[8];
+    typename Raw<16>::type v[8];
 #pragma unroll
-    for (int k = 0; k < UPT; ++k) {
-      const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
-      if (u < nunits) v[k] = ld<16>(src + u * 16);
+    for (int k = 0; k < 8; ++k) {
+      const uint64_t q = (uint64_t)(k * 4) + w;  // piece of this workgroup
+      u[k] = scu + (q * kRepS + x) * kRepPiece + l;
+      if (u[k] < nunits) v[k] = ld<16>(src + u[k] * 16);
     }
+    const int d0 = 1 + (int)gi * G;  // member 0 already holds the result
 #pragma unroll
     for (int q = 0; q < G; ++q) {
       if (d0 + q >= P) break;  // the last group may be short
       char *dst = member_ptr(g, d0 + q, off);
 #pragma unroll
-      for (int k = 0; k < UPT; ++k) {
-        const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
-        if (u < nunits) st_nt<16>(dst + u * 16, v[k]);
-      }
+      for (int k = 0; k < 8; ++k)
+        if (u[k] < nunits) st_nt<16>(dst + u[k] * 16, v[k]);
     }
   }
 }
@@ -483,18 +504,22 @@ __global__ __launch_bounds__(kMoveBlock) void k_replicate(
 template <int G>
 static int replicate_g(const oscd_geom *g, uint64_t off, uint64_t nbytes,
                        hipStream_t s) {
-  constexpr int UPT = 8;  // 32 KiB per workgroup visit
   const uint64_t nunits = nbytes / 16;
-  const uint64_t nch = (nunits + kMoveBlock * UPT - 1) / (kMoveBlock * UPT);
-  const uint64_t ngroups = ((uint64_t)g->npes + G - 1) / G;
-  const uint64_t total = nch * ngroups;
-  // about 4096 workgroups (64 per destination at 64 members), at least one
-  // pass over the groups
-  uint64_t grid = 4096;
-  if (grid > total) grid = total;
-  hipLaunchKernelGGL((k_replicate<UPT, G>), dim3((unsigned)grid),
-                     dim3(kMoveBlock), 0, s, *g, off, nunits, ngroups, total);
+  const uint64_t scu = (uint64_t)kRepS * kRepWgUnits;
+  const uint64_t nsc = (nunits + scu - 1) / scu;
+  const uint64_t ngroups = ((uint64_t)g->npes - 1 + G - 1) / G;
+  const uint64_t total = nsc * ngroups * kRepS;
+  const uint64_t grid = total < kRepMaxGrid ? total : kRepMaxGrid;
+  hipLaunchKernelGGL((k_replicate<G>), dim3((unsigned)grid), dim3(kMoveBlock),
+                     0, s, *g, off, nunits, ngroups, total);
   return check_launch();
+}
+
+// Members per workgroup when the caller leaves it to the library: 4 for
+// 8..24 MiB per member, else 8 (the measured optimum at 16 MiB vs at 4, 32
+// and 64 MiB: 157 vs 168-173 us, 320 vs 349-392 us).
+static int replicate_auto_group(uint64_t nbytes) {
+  return (nbytes >= (8ull << 20) && nbytes < (24ull << 20)) ? 4 : 8;
 }
 
 }  // namespace osc
@@ -505,6 +530,7 @@ extern "C" int oscd_replicate(const oscd_geom *g, uint64_t off, uint64_t nbytes,
   if (g == nullptr || g->npes <= 0 || nbytes % 16 != 0) return OSC_ERR_ARG;
   if (nbytes == 0 || g->npes == 1) return OSC_OK;
   hipStream_t s = (hipStream_t)stream;
+  if (group < 0) group = replicate_auto_group(nbytes);
   switch (group) {
     case 1: return replicate_g<1>(g, off, nbytes, s);
     case 2: return replicate_g<2>(g, off, nbytes, s);

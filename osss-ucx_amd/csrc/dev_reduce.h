@@ -1199,27 +1199,28 @@ static void launch_shape(const oscd_geom *g, int lg, uint64_t doff,
                      soff, n, lg);
 }
 
-// Large calls of 8- and 16-byte elements run in two kernels: the slice
-// walk stores the result to member 0 only, then oscd_replicate copies it to
-// every member.  Measured on MI355X at 64 members x 4M elements, both
-// variants alternating in one process: the walk's own write phase (every
-// workgroup fanning its 32 KiB span out to all 64 members) writes at
-// 5.1-5.3 TB/s, while the walk without the fan-out reads at 6.8 TB/s and
-// the replicate kernel writes at 5.3-5.5; u64/i64 cells gain 0.5-2%, f64
-// sum 5.35-5.43 -> 5.70 TB/s (+6%), f64 prod +2%, and the split cells stay
-// level over a run where the fused ones drift down by 1-3%.  f32 sum and
-// prod lose 1-1.7% (the fused f32 walk is the faster one: 5.86-5.95
-// TB/s), so 4-byte and narrower types keep one kernel.  The split costs one
-// extra read of n*s bytes (1/P of the traffic) and one launch.  Not for
-// per-member results (SPECIAL: float MIN/MAX).
+// Large calls run in two kernels: the slice walk stores the result to
+// member 0 only, then oscd_replicate copies it to the other members.  The
+// walk then only reads (plus 1/P of writes) and the replicate only writes,
+// and HBM serves each phase faster than it serves the two mixed: measured
+// on MI355X at 64 members x 4M elements, the walk to member 0 reads at
+// 6.8-7.2 TB/s for every element type (u64 xor 313 us, f64 sum 329 us at
+// 8x2, f32 sum 159 us), while a workgroup that fans its span out to all 64
+// members writes at 5.1-5.3 TB/s and a fused u64 cell takes 735 us; the
+// replicate writes in an interleaved order that reaches 6.3-6.8 TB/s.
+// Every type whose 16-byte unit holds whole elements (2, 4, 8 and 16
+// bytes) takes it, from 1 MiB per member and 8 members.  The split costs
+// one extra launch and (P-1)/G cached re-reads of member 0's result.  Not
+// for per-member results (SPECIAL: float MIN/MAX).
 constexpr int kSplitMinPes = 8;
 constexpr uint64_t kSplitMinBytes = 1ull << 20;  // per member
-extern int g_split_group;  // 0: no split; else the replicate group size
+// 0: no split; -1: replicate group picked by size; else the group size
+extern int g_split_group;
 template <class T, class Pol>
 static bool split_pays(const oscd_geom *g, uint64_t n) {
-  return g_split_group > 0 && !Pol::SPECIAL && sizeof(T) >= 8 &&
-         g->npes >= kSplitMinPes && n * sizeof(T) >= kSplitMinBytes &&
-         (n * sizeof(T)) % 16 == 0;
+  return g_split_group != 0 && !Pol::SPECIAL &&
+         vec_of<T>() * sizeof(T) == 16 && g->npes >= kSplitMinPes &&
+         n * sizeof(T) >= kSplitMinBytes && (n * sizeof(T)) % 16 == 0;
 }
 
 // Default shape per policy (Pol::SHAPE), from the 64-PE x 4M sweeps on

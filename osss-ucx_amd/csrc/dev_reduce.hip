@@ -22,12 +22,15 @@ int visit_reduce(int dtype, int op, int mode, PerCallV *pc, BatchV *bv,
 }  // namespace osc
 
 namespace osc {
-int g_split_group = 8;
+int g_split_group = -1;
 }  // namespace osc
 
 extern "C" void oscd_set_reduce_split(int enabled, int group) {
   osc::g_split_group =
-      !enabled ? 0 : (group == 1 || group == 2 || group == 4 || group == 16) ? group : 8;
+      !enabled ? 0
+      : (group == 1 || group == 2 || group == 4 || group == 8 || group == 16)
+          ? group
+          : -1;
 }
 
 static int log2_floor(int p) {

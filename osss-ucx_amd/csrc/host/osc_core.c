@@ -87,7 +87,7 @@ int osc_init(int npes, size_t heap_bytes_per_pe, int device) {
   {
     const char *sp = getenv("OSC_REDUCE_SPLIT");
     if (sp == NULL || sp[0] == '\0' || osc_set_reduce_split(atoi(sp)) != OSC_OK)
-      osc_set_reduce_split(8);
+      osc_set_reduce_split(-1);
   }
   {
     const char *br = getenv("OSC_BATCH_BRANCHES");
@@ -198,10 +198,10 @@ int osc_set_batch_branches(int n) {
   return OSC_OK;
 }
 int osc_set_reduce_split(int group) {
-  if (group != 0 && group != 1 && group != 2 && group != 4 && group != 8 &&
-      group != 16)
-    return osc_set_error(OSC_ERR_ARG, "replicate group %d (0, 1, 2, 4, 8, 16)",
-                         group);
+  if (group != -1 && group != 0 && group != 1 && group != 2 && group != 4 &&
+      group != 8 && group != 16)
+    return osc_set_error(OSC_ERR_ARG,
+                         "replicate group %d (-1, 0, 1, 2, 4, 8, 16)", group);
   oscd_set_reduce_split(group != 0, group);
   return OSC_OK;
 }

@@ -73,13 +73,14 @@ int oscd_reduce(const oscd_geom *g, int op, int dtype, int fpmode, int shape,
                 uint64_t dest_off, uint64_t src_off, uint64_t n, void *stream);
 
 /* replicate: member j's [off, off + nbytes) = member 0's, for every j
- * (nbytes a multiple of 16), each loaded chunk stored to `group` (1, 2, 4,
- * 8 or 16) consecutive members; the second kernel of a large reduction */
+ * (nbytes a multiple of 16), each loaded piece stored to `group` (1, 2, 4,
+ * 8 or 16; -1 picks by size) consecutive members; the second kernel of a
+ * large reduction */
 int oscd_replicate(const oscd_geom *g, uint64_t off, uint64_t nbytes,
                    int group, void *stream);
 /* large-reduction split: 0 = one kernel (every workgroup stores to all
  * members), else the walk to member 0 + oscd_replicate with that group
- * size (default 8) */
+ * size (-1, the default: picked by size) */
 void oscd_set_reduce_split(int enabled, int group);
 
 /* fcollect: dest_j[i*nbytes ..) = src_i[0 .. nbytes) for all members i, j */

@@ -1,12 +1,15 @@
-"""Large reductions of 8- and 16-byte elements run in two kernels: the
-slice walk stores member 0's result only, then a replicate kernel copies it
-to every member, each loaded chunk to a group of consecutive members.
-Checked against the oracle just above and below the size where the split
-starts (1 MiB per member, at least 8 members, a multiple of 16 bytes), on
-power-of-two and odd teams, a strided team, in place, in both float modes,
-with the bytes after each destination untouched; 4-byte and narrower types
-(one kernel) at the same sizes; and bit-identical to the one-kernel path for
-every replicate group size, on teams that are not a multiple of the group."""
+"""Large reductions of 2-, 4-, 8- and 16-byte elements run in two kernels:
+the slice walk stores member 0's result only, then a replicate kernel copies
+it to the other members, each loaded piece to a group of consecutive
+members, in 1 KiB pieces interleaved over 16 workgroups.  Checked against
+the oracle just above and below the size where the split starts (1 MiB per
+member, at least 8 members, a multiple of 16 bytes), on power-of-two and
+odd teams, a strided team, in place, in both float modes, with the bytes
+after each destination untouched; 1-byte types and float min/max (one
+kernel) at the same sizes; sizes that leave the last interleave group of
+pieces partly empty; and bit-identical to the one-kernel path for every
+replicate group size and the size-picked group, on teams that are not a
+multiple of the group."""
 import zlib
 
 import numpy as np
@@ -25,7 +28,7 @@ _H = []
 @pytest.fixture
 def heap():
     if not _H or not _H[0].alive:
-        _H[:] = [osc.Heap(npes=WORLD, heap_bytes=12 << 20)]
+        _H[:] = [osc.Heap(npes=WORLD, heap_bytes=24 << 20)]
     return _H[0]
 
 
@@ -61,8 +64,15 @@ CASES = [  # (dtype, op, n): at, above and off the 1 MiB/member threshold
     ("f64", "sum", 131071),  # below 1 MiB per member: one kernel
     ("c64", "prod", 70001), ("c64", "sum", 65535),  # 16-byte elements
     ("c32", "sum", 140000), ("c32", "prod", 131073),
-    ("f32", "sum", 300004), ("f32", "prod", 262144),  # 4 bytes: one kernel
-    ("u8", "xor", (1 << 20) + 48), ("i32", "max", 262143 * 4),
+    ("f32", "sum", 300004), ("f32", "prod", 262144),
+    ("i32", "max", 262143 * 4), ("i16", "sum", (1 << 19) + 8 * 3),
+    ("u16", "and", 1 << 19),
+    # one kernel: 1-byte elements, per-member float min/max
+    ("u8", "xor", (1 << 20) + 48), ("f32", "max", 262148),
+    ("f64", "min", 131072 + 10),
+    # 8 MiB per member (the size-picked group of 4) and a size that leaves
+    # the last interleave group of 1 KiB pieces partly empty
+    ("f64", "sum", 1 << 20), ("i64", "xor", (1 << 20) + 131072 + 16 * 5),
 ]
 
 
@@ -81,7 +91,8 @@ def test_split_reduce_matches_oracle(heap, dtype, op, n, team):
     for mode in modes:
         got = reduce_checked(heap, t, pes, op, dtype, src, mode,
                              inplace=(mode == osc.FP_REFERENCE))
-        if dtype[0] in "iu" or mode == osc.FP_REFERENCE:
+        if dtype[0] in "iu" or mode == osc.FP_REFERENCE or \
+                op in ("min", "max"):
             assert bit_equal(got, O.rec_dbl(op, dtype, src), dtype), mode
         else:
             tol = 2.0 ** -24 * 1.001 if dtype in ("f32", "c32") else \
@@ -104,7 +115,8 @@ def test_split_reduce_under_each_algorithm(heap):
 
 @pytest.mark.parametrize("dtype,op,n", [("f64", "sum", 200002),
                                         ("i64", "prod", 131072),
-                                        ("c64", "prod", 65536 + 2)])
+                                        ("c64", "prod", 65536 + 2),
+                                        ("f32", "sum", 262144 + 4 * 33)])
 @pytest.mark.parametrize("team", [(0, 1, 8), (1, 2, 13), (0, 1, 64),
                                   (5, 1, 37)])
 def test_split_matches_one_kernel_for_every_group(heap, dtype, op, n, team):
@@ -118,17 +130,18 @@ def test_split_matches_one_kernel_for_every_group(heap, dtype, op, n, team):
     try:
         osc.check(heap.L.osc_set_reduce_split(0), "split off")
         want = reduce_checked(heap, t, pes, op, dtype, src, osc.FP_ACCURATE)
-        for group in (1, 2, 4, 8, 16):
+        for group in (1, 2, 4, 8, 16, -1):
             osc.check(heap.L.osc_set_reduce_split(group), "split group")
             got = reduce_checked(heap, t, pes, op, dtype, src, osc.FP_ACCURATE)
             assert bit_equal(got, want, dtype), group
     finally:
-        heap.L.osc_set_reduce_split(8)
+        heap.L.osc_set_reduce_split(-1)
     if dtype[0] == "i":
         assert bit_equal(want, O.rec_dbl(op, dtype, src), dtype)
 
 
 def test_split_setting_rejects_other_groups(heap):
-    for bad in (-1, 3, 5, 32):
+    for bad in (-2, 3, 5, 32):
         assert heap.L.osc_set_reduce_split(bad) == -1  # OSC_ERR_ARG
     assert heap.L.osc_set_reduce_split(8) == 0
+    assert heap.L.osc_set_reduce_split(-1) == 0
