@@ -33,6 +33,35 @@ namespace osc {
 
 constexpr int kMoveBlock = 256;
 
+// Interleaved write order (k_replicate): HBM write bandwidth on
+// MI355X depends on the order in which the workgroups issue their stores.
+// 2 GiB of non-temporal 16-byte stores, each workgroup writing its own
+// contiguous 32 KiB, run at 5.2-5.4 TB/s; S consecutive workgroups that
+// interleave 1 KiB pieces (workgroup x of the group writes pieces x, x+S,
+// x+2S, ... of an S*32 KiB super-chunk) run at 6.3-6.5 TB/s, and
+// destinations that change every 16-32 KiB stay at 5.5-5.8 TB/s whatever
+// the piece size.  Reads gain nothing from the interleave.  A piece is one
+// wave's 64 lanes x 16 bytes; wave w's k-th piece is k*4 + w.  S is a
+// multiple of the 8 XCDs, so workgroup x of every interleave group runs on
+// the same XCD.  (The same order applied to fcollect at 256 members x 1 MiB,
+// 4, 8 or 16 destinations per workgroup, was measured against k_gather at
+// 16 placements of the arena: within +-3% either way, so fcollect keeps
+// k_gather.  Both move by up to 30% with the arena's physical placement:
+// 9.8-13.0 ms for the same layout and kernel.)
+constexpr int kRepS = 16;                         // workgroups per group
+constexpr uint64_t kRepPiece = 64;                // units (1 KiB) per piece
+constexpr uint64_t kRepWgUnits = kMoveBlock * 8;  // 2048 units per visit
+constexpr uint64_t kRepScUnits = kRepS * kRepWgUnits;  // per super-chunk
+constexpr uint64_t kRepMaxGrid = 1u << 20;        // a multiple of kRepS
+
+// unit of the region that lane l of wave w stores as its k-th unit, for
+// workgroup x of the interleave group of super-chunk sc
+__device__ __forceinline__ uint64_t il_unit(uint64_t sc, uint64_t x, int k,
+                                            uint32_t w, uint32_t l) {
+  const uint64_t q = (uint64_t)(k * 4) + w;
+  return sc * kRepScUnits + (q * kRepS + x) * kRepPiece + l;
+}
+
 // Gather, destination-major: member j's [doff + i*doff_step, +nbytes) =
 // source member i's [soff, +nbytes), for every source i (fcollect,
 // src_member < 0) or for i = src_member only (broadcast, doff_step = 0).
@@ -440,33 +469,17 @@ extern "C" int oscd_collect(const oscd_geom *g, uint64_t dest_off,
 
 // ---- replicate: member 0's bytes to every member -----------------------------
 // The second half of a large reduction (the walk stores member 0's result
-// only): members 1..P-1 receive member 0's [off, off + nbytes).
-//
-// The kernel is write-bound, and HBM write bandwidth depends on the order in
-// which the chip's workgroups issue their stores.  Measured on MI355X with
-// 2 GiB of non-temporal 16-byte stores: each workgroup writing its own
-// contiguous 32 KiB runs at 5.2-5.4 TB/s, while S = 32 consecutive
-// workgroups that interleave 1 KiB pieces (workgroup x writes pieces x,
-// x+S, x+2S, ... of a 32*S KiB range) run at 6.3-6.5 TB/s; destinations
-// that change every 32 KiB (the older replicate order) stay at 5.5-5.8
-// whatever the piece size.  Reads gain nothing from the interleave (7.0 TB/s
-// either way).
-//
-// So here the S workgroups of an interleave group split one super-chunk of
-// S*32 KiB of every destination into 1 KiB pieces (one wave's 64 lanes x 16
-// bytes; wave w's k-th piece is k*4 + w), load each piece from member 0
-// once (cached: the next member group of the same super-chunk re-reads it
-// soon after) and store it to a group of G destinations.  Virtual block vb
-// -> x = vb % S, then super-chunk and member group with the member group
-// fastest (offset-major).  Measured (64 members x 32 MiB / 16 MiB, 128 x
-// 16 MiB, 16 x 32 MiB, 8 x 64 MiB): 320 / 157 / 300 / 89 / 94 us against
-// 393 / 200 / 402 / 109 / 112 us for the older order.
+// only): members 1..P-1 receive member 0's [off, off + nbytes), in the
+// interleaved write order described at the top of this file.  The S
+// workgroups of an interleave group load each 1 KiB piece of member 0's
+// super-chunk once (cached: the next member group re-reads it soon after)
+// and store it to a group of G destinations; virtual block vb -> x = vb %
+// S, then super-chunk and member group, member group fastest.  Measured on
+// MI355X (64 members x 32 MiB / 16 MiB, 128 x 16 MiB, 16 x 32 MiB, 8 x 64
+// MiB): 320 / 157 / 300 / 89 / 94 us against 393 / 200 / 402 / 109 / 112
+// us for the older order (one 32 KiB chunk per visit, member group
+// fastest, 5.3-5.5 TB/s).
 namespace osc {
-
-constexpr int kRepS = 16;                     // workgroups per interleave group
-constexpr uint64_t kRepPiece = 64;            // units (1 KiB) per piece
-constexpr uint64_t kRepWgUnits = kMoveBlock * 8;  // 2048 units per visit
-constexpr uint64_t kRepMaxGrid = 1u << 20;    // a multiple of kRepS
 
 template <int G>
 __global__ __launch_bounds__(kMoveBlock) void k_replicate(
@@ -480,13 +493,11 @@ __global__ __launch_bounds__(kMoveBlock) void k_replicate(
   for (uint64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
     const uint64_t x = vb % kRepS, r = vb / kRepS;
     const uint64_t gi = r % ngroups, sc = r / ngroups;
-    const uint64_t scu = sc * (uint64_t)kRepS * kRepWgUnits;
     uint64_t u[8];
     typename Raw<16>::type v[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const uint64_t q = (uint64_t)(k * 4) + w;  // piece of this workgroup
-      u[k] = scu + (q * kRepS + x) * kRepPiece + l;
+      u[k] = il_unit(sc, x, k, w, l);
       if (u[k] < nunits) v[k] = ld<16>(src + u[k] * 16);
     }
     const int d0 = 1 + (int)gi * G;  // member 0 already holds the result
@@ -505,8 +516,7 @@ template <int G>
 static int replicate_g(const oscd_geom *g, uint64_t off, uint64_t nbytes,
                        hipStream_t s) {
   const uint64_t nunits = nbytes / 16;
-  const uint64_t scu = (uint64_t)kRepS * kRepWgUnits;
-  const uint64_t nsc = (nunits + scu - 1) / scu;
+  const uint64_t nsc = (nunits + kRepScUnits - 1) / kRepScUnits;
   const uint64_t ngroups = ((uint64_t)g->npes - 1 + G - 1) / G;
   const uint64_t total = nsc * ngroups * kRepS;
   const uint64_t grid = total < kRepMaxGrid ? total : kRepMaxGrid;
@@ -515,11 +525,15 @@ static int replicate_g(const oscd_geom *g, uint64_t off, uint64_t nbytes,
   return check_launch();
 }
 
-// Members per workgroup when the caller leaves it to the library: 4 for
-// 8..24 MiB per member, else 8 (the measured optimum at 16 MiB vs at 4, 32
-// and 64 MiB: 157 vs 168-173 us, 320 vs 349-392 us).
+// Members per workgroup when the caller leaves it to the library: 4 from
+// 8 MiB per member, else 8.  Measured on MI355X with the members 232 MiB
+// apart, at 16 placements of the arena: 64 x 32 MiB 321-331 us with 4
+// against 345-355 with 8 (old order 389-401), 64 x 16 MiB 172-177 against
+// 175-186; at 4 MiB per member 8 is ahead (47 vs 50-52 us).  The best group
+// also moves with the distance between members (at 40 MiB apart, 8 won at
+// 32 MiB: 320 vs 390-404 us).
 static int replicate_auto_group(uint64_t nbytes) {
-  return (nbytes >= (8ull << 20) && nbytes < (24ull << 20)) ? 4 : 8;
+  return nbytes >= (8ull << 20) ? 4 : 8;
 }
 
 }  // namespace osc
