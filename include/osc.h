@@ -138,13 +138,17 @@ int osc_get_reduce_shape(void);
  * OSC_BATCH_BRANCHES sets it at init); takes effect at the next capture. */
 int osc_set_batch_branches(int n);
 /* Large reductions (1 MiB or more per member, 8+ members, elements of 2, 4,
- * 8 or 16 bytes, operators without per-member results) run as a walk that
- * stores member 0's result and a replicate kernel that copies it to the
- * other members, storing each loaded piece to `group` members: -1
- * (default) picks 4 or 8 by size, 1, 2, 4, 8 or 16 fix it; 0 = one kernel
- * that stores to every member.  Results are identical either way.
- * OSC_REDUCE_SPLIT sets it at init. */
+ * 8 or 16 bytes) run as a walk that stores member 0's result and a
+ * replicate kernel that copies it to the other members, storing each loaded
+ * piece to `group` members: -1 (default) takes 8 below 8 MiB per member and
+ * from 8 MiB the faster of 4 and 8, timed on the first call of each team
+ * size and power-of-two size class (that call waits for its stream); 1, 2,
+ * 4, 8 or 16 fix it; 0 = one kernel that stores to every member.  Results
+ * are identical either way.  OSC_REDUCE_SPLIT sets it at init. */
 int osc_set_reduce_split(int group);
+/* The group the split uses for npes members and nbytes per member (0: not
+ * timed yet). */
+int osc_reduce_split_group(int npes, size_t nbytes);
 const char *osc_strerror(int code);
 const char *osc_last_error(void);
 int osc_last_error_code(void); /* code of the calling thread's last error */

@@ -484,7 +484,8 @@ namespace osc {
 template <int G>
 __global__ __launch_bounds__(kMoveBlock) void k_replicate(
     const oscd_geom g, const uint64_t off, const uint64_t nunits,
-    const uint64_t ngroups, const uint64_t total) {
+    const uint64_t ngroups, const uint64_t total,
+    const uint8_t *__restrict__ skip) {
   const int P = g.npes;
   const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
   const char *src = member_ptr(g, 0, off);
@@ -500,6 +501,17 @@ __global__ __launch_bounds__(kMoveBlock) void k_replicate(
       u[k] = il_unit(sc, x, k, w, l);
       if (u[k] < nunits) v[k] = ld<16>(src + u[k] * 16);
     }
+    if (skip != nullptr) {
+      // pieces of 256 units that the walk already stored to every member;
+      // a wave's 64 units lie in one such piece, so the flag is read once
+      // per wave, after the data loads are in flight
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint64_t pc = (uint64_t)__builtin_amdgcn_readfirstlane(
+            (uint32_t)(il_unit(sc, x, k, w, 0) / 256));
+        if (u[k] < nunits && skip[pc] != 0) u[k] = nunits;
+      }
+    }
     const int d0 = 1 + (int)gi * G;  // member 0 already holds the result
 #pragma unroll
     for (int q = 0; q < G; ++q) {
@@ -514,24 +526,21 @@ __global__ __launch_bounds__(kMoveBlock) void k_replicate(
 
 template <int G>
 static int replicate_g(const oscd_geom *g, uint64_t off, uint64_t nbytes,
-                       hipStream_t s) {
+                       const uint8_t *skip, hipStream_t s) {
   const uint64_t nunits = nbytes / 16;
   const uint64_t nsc = (nunits + kRepScUnits - 1) / kRepScUnits;
   const uint64_t ngroups = ((uint64_t)g->npes - 1 + G - 1) / G;
   const uint64_t total = nsc * ngroups * kRepS;
   const uint64_t grid = total < kRepMaxGrid ? total : kRepMaxGrid;
   hipLaunchKernelGGL((k_replicate<G>), dim3((unsigned)grid), dim3(kMoveBlock),
-                     0, s, *g, off, nunits, ngroups, total);
+                     0, s, *g, off, nunits, ngroups, total, skip);
   return check_launch();
 }
 
-// Members per workgroup when the caller leaves it to the library: 4 from
-// 8 MiB per member, else 8.  Measured on MI355X with the members 232 MiB
-// apart, at 16 placements of the arena: 64 x 32 MiB 321-331 us with 4
-// against 345-355 with 8 (old order 389-401), 64 x 16 MiB 172-177 against
-// 175-186; at 4 MiB per member 8 is ahead (47 vs 50-52 us).  The best group
-// also moves with the distance between members (at 40 MiB apart, 8 won at
-// 32 MiB: 320 vs 390-404 us).
+// Members per workgroup for group -1: 8 below 8 MiB per member (at 4 MiB
+// 8 is ahead: 47 vs 50-52 us), else 4.  From 8 MiB the reduce path picks
+// between 4 and 8 by measurement instead (replicate_tuned): the faster one
+// differs between boxes.
 static int replicate_auto_group(uint64_t nbytes) {
   return nbytes >= (8ull << 20) ? 4 : 8;
 }
@@ -539,18 +548,19 @@ static int replicate_auto_group(uint64_t nbytes) {
 }  // namespace osc
 
 extern "C" int oscd_replicate(const oscd_geom *g, uint64_t off, uint64_t nbytes,
-                              int group, void *stream) {
+                              int group, const void *skip, void *stream) {
   using namespace osc;
   if (g == nullptr || g->npes <= 0 || nbytes % 16 != 0) return OSC_ERR_ARG;
   if (nbytes == 0 || g->npes == 1) return OSC_OK;
   hipStream_t s = (hipStream_t)stream;
+  const uint8_t *sk = (const uint8_t *)skip;
   if (group < 0) group = replicate_auto_group(nbytes);
   switch (group) {
-    case 1: return replicate_g<1>(g, off, nbytes, s);
-    case 2: return replicate_g<2>(g, off, nbytes, s);
-    case 4: return replicate_g<4>(g, off, nbytes, s);
-    case 16: return replicate_g<16>(g, off, nbytes, s);
-    default: return replicate_g<8>(g, off, nbytes, s);
+    case 1: return replicate_g<1>(g, off, nbytes, sk, s);
+    case 2: return replicate_g<2>(g, off, nbytes, sk, s);
+    case 4: return replicate_g<4>(g, off, nbytes, sk, s);
+    case 16: return replicate_g<16>(g, off, nbytes, sk, s);
+    default: return replicate_g<8>(g, off, nbytes, sk, s);
   }
 }
 

@@ -498,15 +498,19 @@ __host__ __device__ constexpr int shape_infl(int s) {
 // scalar-base addressing form.  FAST: every unit of the block is complete
 // and in range (all blocks but the last), so no per-unit guards.
 // ONE: store the result to member 0 only (the replicate kernel copies it to
-// every member afterwards); never with a SPECIAL policy, whose special
-// elements have per-member results.
+// every member afterwards).  With a SPECIAL policy (float MIN/MAX, whose
+// special elements have per-member results) a block whose span holds a
+// special element stores every member itself, as without ONE, and sets
+// the flag of each of its 256-unit pieces in `flags` (one byte per piece of
+// the call, written for every piece: 1 = skip it, 0 = replicate it).
 template <bool FAST, class T, class Pol, int VEC, int UPT, int INFL, int MAXLV,
           bool ONE = false>
 __device__ __forceinline__ void reduce_block(const oscd_geom &g,
                                              const uint64_t dest_off,
                                              const uint64_t src_off,
                                              const uint64_t n, const int lg,
-                                             const uint64_t ub, T *lds) {
+                                             const uint64_t ub, T *lds,
+                                             uint8_t *flags = nullptr) {
   typedef typename Pol::Acc Acc;
   constexpr uint32_t UB = VEC * (uint32_t)sizeof(T);
   const int P = g.npes;
@@ -749,9 +753,8 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
     }
   }
 
-  if (any) {
-    static_assert(!(ONE && Pol::SPECIAL), "per-member results");
-    for (int j = 0; j < (ONE ? 1 : P); ++j) {
+  auto store_members = [&](const int j0, const int j1) {
+    for (int j = j0; j < j1; ++j) {
       char *dp = d0 + j * step;
 #pragma unroll
       for (int k = 0; k < UPT; ++k)
@@ -761,7 +764,8 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
                                                 ((1u << VEC) - 1u)
                                           : 0u);
     }
-  }
+  };
+  if (any) store_members(0, ONE ? 1 : P);
 
   // Special elements last: the stores above skip them, so their inputs
   // are still intact (also when dest == source), and the block-wide vote
@@ -769,7 +773,13 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
   // write phase, where it held each wave's stores back until the slowest
   // wave of the workgroup had loaded all P slices.
   if constexpr (Pol::SPECIAL) {
-    if (__syncthreads_or(spec != 0))
+    const bool sp = __syncthreads_or(spec != 0);
+    if constexpr (ONE) {
+      // this span's pieces of 256 units (ub is a multiple of 256)
+      if (threadIdx.x < (uint32_t)UPT) flags[ub / kBlock + threadIdx.x] = sp;
+      if (sp && any) store_members(1, P);
+    }
+    if (sp)
       special_block<T, Pol::OPC, VEC, Pol::RAB>(g, src_off, dest_off, ubase,
                                                 spec, lg, n, lds);
   }
@@ -809,7 +819,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((
                                                    const uint64_t dest_off,
                                                    const uint64_t src_off,
                                                    const uint64_t n,
-                                                   const int lg) {
+                                                   const int lg,
+                                                   uint8_t *const flags) {
   static_assert(!Pol::SPECIAL || UPT * VEC <= 32,
                 "special-element mask is 32 bits");
   extern __shared__ unsigned char k_reduce_lds[];
@@ -824,12 +835,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((
   const uint64_t b = blockIdx.x;
   if (b < nspan) {
     reduce_block<true, T, Pol, VEC, UPT, INFL, MAXLV, ONE>(
-        g, dest_off, src_off, n, lg, b * kSpan, (T *)k_reduce_lds);
+        g, dest_off, src_off, n, lg, b * kSpan, (T *)k_reduce_lds, flags);
   } else {
     const uint64_t sub = nspan * kSpan + (b - nspan) * kBlock;
     if (sub < nunits)
       reduce_block<false, T, Pol, VEC, 1, 8, MAXLV, ONE>(
-          g, dest_off, src_off, n, lg, sub, (T *)k_reduce_lds);
+          g, dest_off, src_off, n, lg, sub, (T *)k_reduce_lds, flags);
   }
 }
 
@@ -1178,7 +1189,7 @@ constexpr int vec_of() {
 template <class T, class Pol, int VEC, int UPT, int INFL, bool ONE = false>
 static void launch_shape(const oscd_geom *g, int lg, uint64_t doff,
                          uint64_t soff, uint64_t n, size_t lds,
-                         hipStream_t s) {
+                         hipStream_t s, uint8_t *flags = nullptr) {
   const uint64_t units = (n + VEC - 1) / VEC;
   const uint64_t span = (uint64_t)kBlock * UPT;
   // every full span gets its own workgroup, and every 256 units of the
@@ -1190,13 +1201,13 @@ static void launch_shape(const oscd_geom *g, int lg, uint64_t doff,
     if (lg >= 8) {
       hipLaunchKernelGGL((k_reduce<T, Pol, VEC, UPT, INFL, 13, ONE>),
                          dim3((unsigned)grid), dim3(kBlock), lds, s, *g, doff,
-                         soff, n, lg);
+                         soff, n, lg, flags);
       return;
     }
   }
   hipLaunchKernelGGL((k_reduce<T, Pol, VEC, UPT, INFL, Pol::TREE ? 8 : 1, ONE>),
                      dim3((unsigned)grid), dim3(kBlock), lds, s, *g, doff,
-                     soff, n, lg);
+                     soff, n, lg, flags);
 }
 
 // Large calls run in two kernels: the slice walk stores the result to
@@ -1210,18 +1221,31 @@ static void launch_shape(const oscd_geom *g, int lg, uint64_t doff,
 // replicate writes in an interleaved order that reaches 6.3-6.8 TB/s.
 // Every type whose 16-byte unit holds whole elements (2, 4, 8 and 16
 // bytes) takes it, from 1 MiB per member and 8 members.  The split costs
-// one extra launch and (P-1)/G cached re-reads of member 0's result.  Not
-// for per-member results (SPECIAL: float MIN/MAX).
+// one extra launch and (P-1)/G cached re-reads of member 0's result.
+// Float MIN/MAX (SPECIAL) split too: the walk flags every 256-unit piece of
+// a span that holds a special element (whose members get per-member
+// results), stores those spans to every member itself, and the replicate
+// skips the flagged pieces.  The flags live in a per-stream device buffer
+// (span_flags); without one (a stream under graph capture) the call runs
+// as one kernel.
 constexpr int kSplitMinPes = 8;
 constexpr uint64_t kSplitMinBytes = 1ull << 20;  // per member
 // 0: no split; -1: replicate group picked by size; else the group size
 extern int g_split_group;
 template <class T, class Pol>
 static bool split_pays(const oscd_geom *g, uint64_t n) {
-  return g_split_group != 0 && !Pol::SPECIAL &&
-         vec_of<T>() * sizeof(T) == 16 && g->npes >= kSplitMinPes &&
-         n * sizeof(T) >= kSplitMinBytes && (n * sizeof(T)) % 16 == 0;
+  return g_split_group != 0 && vec_of<T>() * sizeof(T) == 16 &&
+         g->npes >= kSplitMinPes && n * sizeof(T) >= kSplitMinBytes &&
+         (n * sizeof(T)) % 16 == 0;
 }
+// One flag byte per 256-unit piece for a split float MIN/MAX call on stream
+// s, or nullptr when the stream has none to give (under graph capture, more
+// pieces than the heap slice holds, or no memory).
+uint8_t *span_flags(hipStream_t s, uint64_t npieces);
+// The replicate of a split call, with the group picked by measurement on
+// the first call of its (team size, size class) when g_split_group is -1.
+int replicate_tuned(const oscd_geom *g, uint64_t off, uint64_t nbytes,
+                    const uint8_t *skip, hipStream_t s);
 
 // Default shape per policy (Pol::SHAPE), from the 64-PE x 4M sweeps on
 // MI355X: 8x2 for the order-free kernels, 8x1 for FP32 accumulated in FP64
@@ -1269,23 +1293,26 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
     shape = spans < kSmallSpans ? 3 : d;
   }
   shape = clamp_shape<Pol>(shape);
-  if constexpr (!Pol::SPECIAL) {
-    if (vec_ok && split_pays<T, Pol>(g, n)) {
+  if (vec_ok && split_pays<T, Pol>(g, n)) {
+    uint8_t *fl = nullptr;
+    if constexpr (Pol::SPECIAL)
+      fl = span_flags(s, (n * sizeof(T) / 16 + kBlock - 1) / kBlock);
+    if (!Pol::SPECIAL || fl != nullptr) {
       if constexpr (Pol::TREE) {
         if (shape == 3)
-          launch_shape<T, Pol, VB, shape_upt(3), shape_infl(3), true>(g, lg, doff, soff, n, 0, s);
+          launch_shape<T, Pol, VB, shape_upt(3), shape_infl(3), true>(g, lg, doff, soff, n, lds, s, fl);
         else
-          launch_shape<T, Pol, VB, shape_upt(2), shape_infl(2), true>(g, lg, doff, soff, n, 0, s);
+          launch_shape<T, Pol, VB, shape_upt(2), shape_infl(2), true>(g, lg, doff, soff, n, lds, s, fl);
       } else {
         switch (shape) {
-          case 0: launch_shape<T, Pol, VB, shape_upt(0), shape_infl(0), true>(g, lg, doff, soff, n, 0, s); break;
-          case 1: launch_shape<T, Pol, VB, shape_upt(1), shape_infl(1), true>(g, lg, doff, soff, n, 0, s); break;
-          case 2: launch_shape<T, Pol, VB, shape_upt(2), shape_infl(2), true>(g, lg, doff, soff, n, 0, s); break;
-          default: launch_shape<T, Pol, VB, shape_upt(3), shape_infl(3), true>(g, lg, doff, soff, n, 0, s); break;
+          case 0: launch_shape<T, Pol, VB, shape_upt(0), shape_infl(0), true>(g, lg, doff, soff, n, lds, s, fl); break;
+          case 1: launch_shape<T, Pol, VB, shape_upt(1), shape_infl(1), true>(g, lg, doff, soff, n, lds, s, fl); break;
+          case 2: launch_shape<T, Pol, VB, shape_upt(2), shape_infl(2), true>(g, lg, doff, soff, n, lds, s, fl); break;
+          default: launch_shape<T, Pol, VB, shape_upt(3), shape_infl(3), true>(g, lg, doff, soff, n, lds, s, fl); break;
         }
       }
       if (hipGetLastError() != hipSuccess) return OSC_ERR_HIP;
-      return oscd_replicate(g, doff, n * sizeof(T), g_split_group, s);
+      return replicate_tuned(g, doff, n * sizeof(T), fl, s);
     }
   }
   if (!vec_ok) {

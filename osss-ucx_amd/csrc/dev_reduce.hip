@@ -21,9 +21,142 @@ int visit_reduce(int dtype, int op, int mode, PerCallV *pc, BatchV *bv,
 
 }  // namespace osc
 
+#include <mutex>
+#include <unordered_map>
+
 namespace osc {
 int g_split_group = -1;
+
+// Span flags of split float MIN/MAX reductions: one device buffer per
+// stream, since calls on different streams may run at the same time while
+// calls on one stream run in order (the walk writes every flag of its call,
+// the replicate that follows it on the stream reads them).  Allocated on a
+// stream's first such call; never under graph capture (the call then runs
+// as one kernel).
+namespace {
+std::mutex g_flag_mu;
+std::unordered_map<hipStream_t, uint8_t *> g_flag_bufs;
+uint64_t g_flag_cap = 0;                 // bytes per buffer
+constexpr size_t kMaxFlagStreams = 256;  // beyond: one-kernel calls
+}  // namespace
+
+uint8_t *span_flags(hipStream_t s, uint64_t npieces) {
+  std::lock_guard<std::mutex> lk(g_flag_mu);
+  if (npieces == 0 || npieces > g_flag_cap) return nullptr;
+  auto it = g_flag_bufs.find(s);
+  if (it != g_flag_bufs.end()) return it->second;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess ||
+      st != hipStreamCaptureStatusNone)
+    return nullptr;
+  if (g_flag_bufs.size() >= kMaxFlagStreams) return nullptr;
+  void *p = nullptr;
+  if (hipMalloc(&p, g_flag_cap) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  g_flag_bufs.emplace(s, (uint8_t *)p);
+  return (uint8_t *)p;
+}
+// Replicate group by measurement.  The fastest number of members per
+// workgroup of the interleaved replicate differs between MI355X boxes (64
+// members x 32 MiB: 4 took 322-331 us where 8 took 345-355 on one box, and
+// 377-388 us on another where 8 took 329 us); within one box it held across
+// 16 placements of the arena.  So from 8 MiB per member, the first call of
+// each (team size, power-of-two size class) times groups 4 and 8 twice each
+// on its own destination -- a replicate is idempotent, the result is the
+// same after every run -- and later calls take the faster.  The first call
+// waits for its stream meanwhile.  Calls under graph capture, and calls
+// while the table is full, take group 4.
+namespace {
+constexpr uint64_t kTuneMinBytes = 8ull << 20;
+constexpr int kTuneSlots = 64;
+struct TuneSlot {
+  int npes;
+  int cls;
+  int group;
+};
+std::mutex g_tune_mu;
+TuneSlot g_tune[kTuneSlots];
+int g_tune_n = 0;
+hipEvent_t g_tune_ev[2] = {nullptr, nullptr};
+
+int size_class(uint64_t nbytes) { return 63 - __builtin_clzll(nbytes); }
+}  // namespace
+
+int replicate_tuned(const oscd_geom *g, uint64_t off, uint64_t nbytes,
+                    const uint8_t *skip, hipStream_t s) {
+  if (g_split_group > 0 || nbytes < kTuneMinBytes)
+    return oscd_replicate(g, off, nbytes, g_split_group, skip, s);
+  const int cls = size_class(nbytes);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (int i = 0; i < g_tune_n; ++i)
+    if (g_tune[i].npes == g->npes && g_tune[i].cls == cls)
+      return oscd_replicate(g, off, nbytes, g_tune[i].group, skip, s);
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (g_tune_n >= kTuneSlots || hipStreamIsCapturing(s, &st) != hipSuccess ||
+      st != hipStreamCaptureStatusNone)
+    return oscd_replicate(g, off, nbytes, 4, skip, s);
+  for (int e = 0; e < 2; ++e)
+    if (g_tune_ev[e] == nullptr && hipEventCreate(&g_tune_ev[e]) != hipSuccess)
+      return oscd_replicate(g, off, nbytes, 4, skip, s);
+  const int cand[2] = {4, 8};
+  float best[2] = {1e30f, 1e30f};
+  for (int rep = 0; rep < 2; ++rep)
+    for (int c = 0; c < 2; ++c) {
+      if (hipEventRecord(g_tune_ev[0], s) != hipSuccess) return OSC_ERR_HIP;
+      const int rc = oscd_replicate(g, off, nbytes, cand[c], skip, s);
+      if (rc != OSC_OK) return rc;
+      float ms = 0.0f;
+      if (hipEventRecord(g_tune_ev[1], s) != hipSuccess ||
+          hipEventSynchronize(g_tune_ev[1]) != hipSuccess ||
+          hipEventElapsedTime(&ms, g_tune_ev[0], g_tune_ev[1]) != hipSuccess)
+        return OSC_ERR_HIP;
+      if (ms < best[c]) best[c] = ms;
+    }
+  g_tune[g_tune_n++] = {g->npes, cls, best[1] < best[0] ? 8 : 4};
+  return OSC_OK;
+}
+
+int replicate_choice(int npes, uint64_t nbytes) {
+  if (g_split_group > 0) return g_split_group;
+  if (nbytes < kTuneMinBytes) return 8;
+  const int cls = size_class(nbytes);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (int i = 0; i < g_tune_n; ++i)
+    if (g_tune[i].npes == npes && g_tune[i].cls == cls) return g_tune[i].group;
+  return 0;
+}
+
+void replicate_tune_reset() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_n = 0;
+  for (int e = 0; e < 2; ++e)
+    if (g_tune_ev[e] != nullptr) {
+      (void)hipEventDestroy(g_tune_ev[e]);
+      g_tune_ev[e] = nullptr;
+    }
+}
 }  // namespace osc
+
+extern "C" int oscd_replicate_group(int npes, uint64_t nbytes) {
+  return osc::replicate_choice(npes, nbytes);
+}
+
+extern "C" void oscd_set_span_flag_capacity(uint64_t bytes_per_member) {
+  std::lock_guard<std::mutex> lk(osc::g_flag_mu);
+  osc::g_flag_cap = bytes_per_member / 4096 + 1;
+}
+
+extern "C" void oscd_release_span_flags(void) {
+  {
+    std::lock_guard<std::mutex> lk(osc::g_flag_mu);
+    for (auto &kv : osc::g_flag_bufs) (void)hipFree(kv.second);
+    osc::g_flag_bufs.clear();
+    osc::g_flag_cap = 0;
+  }
+  osc::replicate_tune_reset();
+}
 
 extern "C" void oscd_set_reduce_split(int enabled, int group) {
   osc::g_split_group =

@@ -117,6 +117,7 @@ int osc_init(int npes, size_t heap_bytes_per_pe, int device) {
   rc = oscd_scratch_init(device);
   if (rc != OSC_OK) return osc_hip_error(rc, "hipGetSymbolAddress(scratch)");
   osc_g.slice = slice_for(heap_bytes_per_pe);
+  oscd_set_span_flag_capacity(osc_g.slice);
   rc = oscd_malloc(&base, (size_t)npes * osc_g.slice);
   if (rc != OSC_OK)
     return osc_set_error(rc, "symmetric arena of %d x %zu bytes: %s", npes,
@@ -164,6 +165,7 @@ int osc_finalize(void) {
   if (!osc_g.initialized) return OSC_OK;
   osc_ctx_release_all();
   oscd_stream_sync(osc_g.stream);
+  oscd_release_span_flags();
   oscd_stream_destroy(osc_g.stream);
   oscd_wait_fini();
   oscd_amo_fini();
@@ -204,6 +206,9 @@ int osc_set_reduce_split(int group) {
                          "replicate group %d (-1, 0, 1, 2, 4, 8, 16)", group);
   oscd_set_reduce_split(group != 0, group);
   return OSC_OK;
+}
+int osc_reduce_split_group(int npes, size_t nbytes) {
+  return oscd_replicate_group(npes, (uint64_t)nbytes);
 }
 int osc_npes(void) { return osc_g.npes; }
 size_t osc_slice_bytes(void) { return osc_g.slice; }

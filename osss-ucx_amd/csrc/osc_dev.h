@@ -75,9 +75,19 @@ int oscd_reduce(const oscd_geom *g, int op, int dtype, int fpmode, int shape,
 /* replicate: member j's [off, off + nbytes) = member 0's, for every j
  * (nbytes a multiple of 16), each loaded piece stored to `group` (1, 2, 4,
  * 8 or 16; -1 picks by size) consecutive members; the second kernel of a
- * large reduction */
+ * large reduction.  skip (device bytes, one per 4096-byte piece, or NULL):
+ * pieces with a nonzero byte are left alone. */
 int oscd_replicate(const oscd_geom *g, uint64_t off, uint64_t nbytes,
-                   int group, void *stream);
+                   int group, const void *skip, void *stream);
+/* replicate group the reduce path uses for a team of npes members and
+ * nbytes per member: the fixed group, 8 below 8 MiB, the measured one, or
+ * 0 while no call of that size class has measured it yet */
+int oscd_replicate_group(int npes, uint64_t nbytes);
+/* span flags of split float MIN/MAX reductions: per-stream device buffers
+ * of one byte per 4096 bytes of the largest per-member call
+ * (bytes_per_member, the heap slice); release frees them all */
+void oscd_set_span_flag_capacity(uint64_t bytes_per_member);
+void oscd_release_span_flags(void);
 /* large-reduction split: 0 = one kernel (every workgroup stores to all
  * members), else the walk to member 0 + oscd_replicate with that group
  * size (-1, the default: picked by size) */

@@ -101,6 +101,7 @@ def lib() -> C.CDLL:
         "osc_get_reduce_shape": (i, []),
         "osc_set_batch_branches": (i, [i]),
         "osc_set_reduce_split": (i, [i]),
+        "osc_reduce_split_group": (i, [i, sz]),
         "osc_get_fp_mode": (i, []),
         "osc_strerror": (C.c_char_p, [i]),
         "osc_last_error": (C.c_char_p, []),

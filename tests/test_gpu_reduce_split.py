@@ -5,11 +5,12 @@ members, in 1 KiB pieces interleaved over 16 workgroups.  Checked against
 the oracle just above and below the size where the split starts (1 MiB per
 member, at least 8 members, a multiple of 16 bytes), on power-of-two and
 odd teams, a strided team, in place, in both float modes, with the bytes
-after each destination untouched; 1-byte types and float min/max (one
-kernel) at the same sizes; sizes that leave the last interleave group of
-pieces partly empty; and bit-identical to the one-kernel path for every
-replicate group size and the size-picked group, on teams that are not a
-multiple of the group."""
+after each destination untouched; 1-byte types (one kernel) at the same
+sizes; sizes that leave the last interleave group of pieces partly empty;
+bit-identical to the one-kernel path for every replicate group size and the
+size-picked group, on teams that are not a multiple of the group; and float
+min/max with NaN and signed-zero extremes planted in some spans (those
+spans keep per-member results, the others are replicated)."""
 import zlib
 
 import numpy as np
@@ -67,9 +68,10 @@ CASES = [  # (dtype, op, n): at, above and off the 1 MiB/member threshold
     ("f32", "sum", 300004), ("f32", "prod", 262144),
     ("i32", "max", 262143 * 4), ("i16", "sum", (1 << 19) + 8 * 3),
     ("u16", "and", 1 << 19),
-    # one kernel: 1-byte elements, per-member float min/max
-    ("u8", "xor", (1 << 20) + 48), ("f32", "max", 262148),
-    ("f64", "min", 131072 + 10),
+    # float min/max: split with per-span flags
+    ("f32", "max", 262148), ("f64", "min", 131072 + 10),
+    # one kernel: 1-byte elements
+    ("u8", "xor", (1 << 20) + 48),
     # 8 MiB per member (the size-picked group of 4) and a size that leaves
     # the last interleave group of 1 KiB pieces partly empty
     ("f64", "sum", 1 << 20), ("i64", "xor", (1 << 20) + 131072 + 16 * 5),
@@ -145,3 +147,50 @@ def test_split_setting_rejects_other_groups(heap):
         assert heap.L.osc_set_reduce_split(bad) == -1  # OSC_ERR_ARG
     assert heap.L.osc_set_reduce_split(8) == 0
     assert heap.L.osc_set_reduce_split(-1) == 0
+
+
+def planted(rng, dtype, P, n):
+    """Normal inputs with special elements planted in a few spans: a NaN in
+    one member, a +0/-0 extremum, +-inf, and a NaN in the ragged tail."""
+    npt = O.NP_DTYPE[dtype]
+    src = rng.standard_normal((P, n)).astype(npt)
+    src[3 % P, 5] = np.nan
+    zeros = np.zeros(P, npt)
+    zeros[1::2] = -0.0  # +0 on even members, -0 on odd ones
+    src[:, n // 3] = zeros
+    src[P // 2, n // 2 + 1] = np.inf
+    src[0, n // 2 + 2] = -np.inf
+    src[P - 1, n - 2] = np.nan
+    return src
+
+
+@pytest.mark.parametrize("dtype,op", [("f32", "max"), ("f64", "min"),
+                                      ("f64", "max"), ("f32", "min")])
+@pytest.mark.parametrize("team", [(0, 1, 8), (1, 3, 13), (0, 1, 64),
+                                  (5, 1, 37)])
+def test_split_minmax_special_spans(heap, dtype, op, team):
+    """Split float MIN/MAX: spans holding a special element get every
+    member's own result (the walk stores them, the replicate skips them);
+    bit-exact per member against the oracle's recursive doubling and
+    Rabenseifner orders, in place, and equal to the one-kernel path."""
+    start, stride, P = team
+    pes = [start + k * stride for k in range(P)]
+    t = heap.world if P == WORLD else heap.split(start, stride, P)
+    es = 4 if dtype == "f32" else 8
+    n = (1 << 20) // es + 4 * 1000 + 3
+    rng = np.random.default_rng(zlib.crc32(repr((dtype, op, team)).encode()))
+    src = planted(rng, dtype, P, n)
+    want = O.rec_dbl(op, dtype, src)
+    for mode, ref, inplace in [(osc.FP_ACCURATE, want, False),
+                               (osc.FP_REFERENCE, want, True),
+                               (osc.FP_RABENSEIFNER,
+                                O.rabenseifner(op, dtype, src), False)]:
+        got = reduce_checked(heap, t, pes, op, dtype, src, mode,
+                             inplace=inplace)
+        assert bit_equal(got, ref, dtype), mode
+    try:
+        osc.check(heap.L.osc_set_reduce_split(0), "split off")
+        one = reduce_checked(heap, t, pes, op, dtype, src, osc.FP_ACCURATE)
+    finally:
+        heap.L.osc_set_reduce_split(-1)
+    assert bit_equal(one, want, dtype)
