@@ -31,6 +31,13 @@ import ctypes as C  # noqa: E402
 import osc  # noqa: E402
 
 PEAK_HBM_GBPS = 8000.0  # MI355X HBM3E, AMD published
+_T0 = time.perf_counter()
+
+
+def log(msg):
+    """Progress line on stderr (stdout carries only the JSON line)."""
+    print("[bench %7.1fs] %s" % (time.perf_counter() - _T0, msg),
+          file=sys.stderr, flush=True)
 P, N = 64, 4194304
 CELLS = [  # (op, dtype, dist, a, b)
     ("and", "u64", "bits", 0, 0), ("or", "u64", "bits", 0, 0),
@@ -103,40 +110,83 @@ def accuracy(L, h, src, dest, windows=4, width=256, seed=9):
     return res
 
 
-def kernel_key(op, dtype, mode):
-    """The kernel instance the library launches for one cell at the default
-    shape (8 units x 2 slices in flight; FP32 accumulated in FP64 and the
-    FP64 double-double sum: 8 x 1; reference-order tree: 2 x 4 for float,
-    1 x 8 for double), in the short form the profiles/ summaries use."""
-    o = osc.OPS[op]
-    if dtype in ("u64", "i64"):
-        t = "long" if (dtype == "i64" and op in ("min", "max")) else "u64"
-        return "k_reduce<%s, PolSeq<%s, %d, false>, 2, 8, 2, 1>" % (t, t, o)
-    t = "float" if dtype == "f32" else "double"
-    vec = 4 if dtype == "f32" else 2
-    if op in ("min", "max"):
-        return "k_reduce<%s, PolSeq<%s, %d, true>, %d, 8, 2, 1>" % (t, t, o,
-                                                                     vec)
-    if mode == 0:
-        if dtype == "f32":
-            return "k_reduce<float, PolF32Acc<%d>, 4, 8, 1, 1>" % o
-        return "k_reduce<double, PolF64Acc<%d>, 2, 8, %d, 1>" % (
-            o, 1 if op == "sum" else 2)
-    if dtype == "f32":
-        return "k_reduce<float, PolTree<float, %d>, 4, 2, 4, 8>" % o
-    return "k_reduce<double, PolTree<double, %d>, 2, 1, 8, 8>" % o
+def last_kernels(L):
+    """The kernels the last reduce call launched, as rocprof names them."""
+    buf = C.create_string_buffer(4096)
+    L.osc_last_reduce_kernels(buf, len(buf))
+    return [k for k in buf.value.decode().split(";") if k]
 
 
-def measured_traffic(key):
-    """HBM bytes per launch of `key` from the committed PMC summary
-    (FETCH_SIZE + WRITE_SIZE, calibrated on the plain copy kernel)."""
-    path = os.path.join(ROOT, "profiles", "reduce_matrix_64pe_4M_traffic.json")
-    try:
-        with open(path) as f:
-            k = json.load(f)["kernels"].get(key)
-        return None if k is None else k["traffic_bytes"]
-    except (OSError, ValueError, KeyError):
-        return None
+HELPERS = ("k_wait_ticket", "__amd_rocclr", "k_fill", "k_sink_clear",
+           "k_xor_read", "k_fill64")
+
+
+def _pmc_pass(counter, cell, mode):
+    """One rocprofv3 counter pass over `--traffic-probe`: per-dispatch
+    values of `counter` in order, as (kernel name, value) pairs."""
+    import csv
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        raise RuntimeError("rocprofv3 not found")
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d,
+               "-o", "pmc", "--", sys.executable, os.path.abspath(__file__),
+               "--traffic-probe", str(cell), "--fp-mode", mode]
+        env = dict(os.environ, TMPDIR="/tmp")
+        pr = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE,
+                              stderr=subprocess.PIPE, text=True,
+                              start_new_session=True)
+        try:
+            out, err = pr.communicate(timeout=120)
+        except subprocess.TimeoutExpired:
+            os.killpg(pr.pid, signal.SIGKILL)
+            pr.communicate()
+            raise RuntimeError("rocprofv3 %s pass timed out" % counter)
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"),
+                          recursive=True)
+        if not files:
+            raise RuntimeError("rocprofv3 %s pass wrote no counter file "
+                               "(rc %d): %s" % (counter, pr.returncode,
+                                                err[-500:]))
+        rows = []
+        for fn in files:
+            with open(fn) as f:
+                for r in csv.DictReader(f):
+                    if r.get("Counter_Name") == counter:
+                        rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"],
+                                     float(r["Counter_Value"])))
+        rows.sort()
+        return [(k, v) for _, k, v in rows]
+
+
+def measured_traffic(cell, mode):
+    """HBM bytes per call of CELLS[cell], measured now: two rocprofv3
+    passes (FETCH_SIZE, then WRITE_SIZE) over a child process that runs a
+    warm-up call, three 1 GiB copies and three calls of the cell.  Each
+    counter is scaled by what it reads for the copies (1 GiB read, 1 GiB
+    written); the cell's dispatches after the last copy are summed and
+    divided by three.  Returns (bytes, detail)."""
+    det = {}
+    total = 0.0
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        rows = _pmc_pass(counter, cell, mode)
+        copies = [i for i, (k, _) in enumerate(rows) if "k_copy<" in k]
+        if len(copies) < 3:
+            raise RuntimeError("%s: calibration copies not found" % counter)
+        scale = float(1 << 30) / (sum(rows[i][1] for i in copies[-3:]) / 3.0)
+        cell_rows = [(k, v) for k, v in rows[copies[-1] + 1:]
+                     if not any(h in k for h in HELPERS)]
+        per_call = sum(v for _, v in cell_rows) * scale / 3.0
+        det[counter] = {"scale_to_bytes": round(scale, 4),
+                        "bytes_per_call": round(per_call),
+                        "dispatches": len(cell_rows)}
+        total += per_call
+    return total, det
 
 
 CPU_CELLS = [("xor", "u64"), ("sum", "i64"), ("sum", "f32"), ("sum", "f64"),
@@ -264,10 +314,29 @@ def bench_movement():
         ms = _median_event_ms(L, st, lambda: L.osc_fcollect(
             h.world, d, s, n * 4, st), reps=5)
         b = P * n * 4.0 + P * P * n * 4.0
+        # write-only ceiling over the same footprint: every member's
+        # destination region (P x 256 MiB) filled, timed as one group
+        sl = L.osc_slice_bytes()
+        t = osc.Timer(st)
+
+        def fill_all():
+            for j in range(P):
+                osc.check(L.osc_device_fill64(d + j * sl, P * n * 4, j, st),
+                          "fill")
+        fill_all()
+        h.sync()
+        t.start()
+        fill_all()
+        t.stop()
+        wr = P * P * n * 4.0 / (t.ms() / 1e3) / 1e9
+        t.close()
         out["fcollect_256pe_1MiB"] = {"ms": round(ms, 3),
                                       "GBps": round(b / ms / 1e6, 1),
                                       "pct_peak": round(b / ms / 1e6 /
-                                                        PEAK_HBM_GBPS * 100, 1)}
+                                                        PEAK_HBM_GBPS * 100, 1),
+                                      "write_ceiling_64GiB_GBps": round(wr, 1),
+                                      "frac_of_write_ceiling": round(
+                                          b / ms / 1e6 / wr, 4)}
     P, n = 8, 65536
     with osc.Heap(npes=P, heap_bytes=4 << 20) as h:
         s, d = h.malloc(n * 4), h.malloc(n * 4)
@@ -395,10 +464,21 @@ def bench_served(seed=5, nreq=2000, rate=20000.0, verify=96):
     arena = int((budget_total / npes - inputs) // (2 << 20)) * (2 << 20) \
         - (64 << 20)
     scratch = 128 << 20
-    world = osc_calls.CallWorld(npes=npes, max_reduce=4194304,
-                                move_bytes=128 << 20,
-                                dest_bytes=arena + scratch,
-                                keys=["f32", "f64", "i64"], seed=seed)
+    # the arena is most of the GPU's memory: memory a process that just
+    # ended still holds can take a moment to come back, so a failed
+    # allocation is retried a few times
+    for attempt in range(5):
+        try:
+            world = osc_calls.CallWorld(npes=npes, max_reduce=4194304,
+                                        move_bytes=128 << 20,
+                                        dest_bytes=arena + scratch,
+                                        keys=["f32", "f64", "i64"], seed=seed)
+            break
+        except osc.OscError as e:
+            if e.code != -5 or attempt == 4:  # OSC_ERR_NOMEM
+                raise
+            log("served: arena allocation failed, retrying in 2 s")
+            time.sleep(2.0)
     try:
         rng = np.random.default_rng(seed)
         teams = osc_calls.served_teams(rng, world, 16, 8, 128)
@@ -469,6 +549,82 @@ def bench_served(seed=5, nreq=2000, rate=20000.0, verify=96):
         world.close()
 
 
+def matrix_heap(L, only=None):
+    """The workload's heap: 64 PEs, one source array per (type,
+    distribution) of the matrix, filled on the device, one destination of
+    4M 8-byte elements and a 16-byte sink.  only: fill just that cell's
+    source (the layout stays the same)."""
+    dists = sorted({(dt, dist, a, b) for _, dt, dist, a, b in CELLS})
+    per_pe = sum(N * ESIZE[d[0]] for d in dists) + N * 8 + (8 << 20)
+    h = osc.Heap(npes=P, heap_bytes=per_pe)
+    src = {}
+    for k, (dt, dist, a, b) in enumerate(dists):
+        src[(dt, dist, a, b)] = h.malloc(N * ESIZE[dt])
+        if only is None or only == (dt, dist, a, b):
+            h.fill(None, src[(dt, dist, a, b)], N, dt, dist, a, b, seed=k + 1)
+    dest = h.malloc(N * 8)
+    sink = h.malloc(16)
+    h.sync()
+    return h, src, dest, sink
+
+
+def traffic_probe(L, cell, mode):
+    """Child process of measured_traffic (run under rocprofv3 --pmc): one
+    warm-up call of CELLS[cell], three 1 GiB copies (the counters'
+    calibration), three calls of the cell."""
+    op, dt, dist, a, b = CELLS[cell]
+    h, src, dest, _ = matrix_heap(L, only=(dt, dist, a, b))
+    osc.check(L.osc_set_fp_mode(mode), "fp mode")
+    st = L.osc_default_stream()
+
+    def call():
+        osc.check(L.osc_reduce(h.world, osc.OPS[op], osc.DTYPES[dt], dest,
+                               src[(dt, dist, a, b)], N, st), "osc_reduce")
+    call()
+    h.sync()
+    base = L.osc_heap_base()
+    half = (P // 2) * L.osc_slice_bytes()
+    for _ in range(3):
+        osc.check(L.osc_device_copy(base + half, base, 1 << 30, st), "copy")
+    h.sync()
+    for _ in range(3):
+        call()
+    h.sync()
+    h.close()
+
+
+def ceilings(L, h, sink, stream, reps=5):
+    """Plain-copy, read-only and write-only rates on the workload's arena
+    (after the timed region: they overwrite slices 32..63), 2 GiB each way
+    like one 8-byte cell, and the bound they set for a kernel that reads
+    and writes equal bytes in two phases (read phase, then write phase)."""
+    base = L.osc_heap_base()
+    half = (P // 2) * L.osc_slice_bytes()
+    nb = 1 << 31
+    t = osc.Timer(stream)
+
+    def rate(fn, nbytes):
+        fn()
+        h.sync()
+        t.start()
+        for _ in range(reps):
+            fn()
+        t.stop()
+        return reps * nbytes / (t.ms() / 1e3) / 1e9
+    # 1 GiB read + 1 GiB written, as in the earlier measurements
+    copy = rate(lambda: L.osc_device_copy(base + half, base, nb // 2, stream),
+                nb)
+    copy2 = rate(lambda: L.osc_device_copy(base + half, base, nb, stream),
+                 2 * nb)
+    rd = rate(lambda: L.osc_device_xor_read(base, nb, sink, stream), nb)
+    wr = rate(lambda: L.osc_device_fill64(base + half, nb, 0x0123456789abcdef,
+                                          stream), nb)
+    t.close()
+    return {"copy_1GiB_GBps": round(copy, 1), "copy_2GiB_GBps": round(copy2, 1),
+            "read_2GiB_GBps": round(rd, 1), "write_2GiB_GBps": round(wr, 1),
+            "read_then_write_GBps": round(2.0 / (1.0 / rd + 1.0 / wr), 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -482,6 +638,10 @@ def main():
     ap.add_argument("--no-movement", action="store_true")
     ap.add_argument("--cpu-baseline", action="store_true",
                     help="only the CPU baseline (run in a child process)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter passes")
+    ap.add_argument("--traffic-probe", type=int, default=None,
+                    help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_baseline:
         print(json.dumps(cpu_baseline()))
@@ -494,15 +654,11 @@ def main():
 
     L = osc.lib()
     mode = 0 if args.fp_mode == "accurate" else 1
-    dists = sorted({(dt, dist, a, b) for _, dt, dist, a, b in CELLS})
-    per_pe = sum(N * ESIZE[d[0]] for d in dists) + N * 8 + (8 << 20)
-    h = osc.Heap(npes=P, heap_bytes=per_pe)
-    src = {}
-    for k, (dt, dist, a, b) in enumerate(dists):
-        src[(dt, dist, a, b)] = h.malloc(N * ESIZE[dt])
-        h.fill(None, src[(dt, dist, a, b)], N, dt, dist, a, b, seed=k + 1)
-    dest = h.malloc(N * 8)
-    h.sync()
+    if args.traffic_probe is not None:
+        traffic_probe(L, args.traffic_probe, mode)
+        return
+    log("matrix: heap and inputs")
+    h, src, dest, sink = matrix_heap(L)
     osc.check(L.osc_set_fp_mode(mode), "fp mode")
     stream = L.osc_default_stream()
     evs = []
@@ -521,9 +677,17 @@ def main():
         if record:
             L.osc_event_record(evs[-1], stream)
 
+    # the kernels each cell launches (the first call of a size class also
+    # times the replicate groups)
+    cell_kernels = []
+    for i, (op, dt, dist, a, b) in enumerate(CELLS):
+        osc.check(L.osc_reduce(h.world, osc.OPS[op], osc.DTYPES[dt], dest,
+                               src[(dt, dist, a, b)], N, stream), "osc_reduce")
+        cell_kernels.append(last_kernels(L))
     for _ in range(args.warmup):
         step(False)
     h.sync()
+    log("matrix: %d timed steps" % args.steps)
     cell_ms = [0.0] * len(CELLS)
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -543,28 +707,26 @@ def main():
         avg = cell_ms[i] / args.steps
         b = 2.0 * P * N * ESIZE[dt]
         cells.append({"op": op, "type": dt, "ms": round(avg, 4),
-                      "GBps": round(b / (avg / 1e3) / 1e9, 1)})
+                      "GBps": round(b / (avg / 1e3) / 1e9, 1),
+                      "kernels": cell_kernels[i]})
     dom = max(range(len(CELLS)), key=lambda i: cell_ms[i])
     dom_avg = cell_ms[dom] / args.steps / 1e3
     dom_b = 2.0 * P * N * ESIZE[CELLS[dom][1]]
     achieved = dom_b / dom_avg / 1e9
 
+    log("matrix: accuracy")
     acc = accuracy(L, h, src, dest)
     osc.check(L.osc_set_fp_mode(mode), "fp mode")
 
-    # plain-copy ceiling on the same arena after the timed region (it
-    # overwrites slices 32.. of the heap): 1 GiB read + 1 GiB written
-    cp_bytes = 1 << 31
-    base = L.osc_heap_base()
-    half = (P // 2) * L.osc_slice_bytes()
-    t = osc.Timer(stream)
-    L.osc_device_copy(base + half, base, cp_bytes // 2, stream)
-    h.sync()
-    t.start()
-    for _ in range(5):
-        L.osc_device_copy(base + half, base, cp_bytes // 2, stream)
-    t.stop()
-    copy_gbps = 5 * cp_bytes / (t.ms() / 1e3) / 1e9
+    log("ceilings")
+    ceil = ceilings(L, h, sink, stream)
+    traffic, traffic_note = None, None
+    if not args.no_traffic:
+        log("traffic: rocprofv3 counter passes")
+        try:
+            traffic, traffic_note = measured_traffic(dom, args.fp_mode)
+        except (RuntimeError, OSError, ValueError, KeyError) as e:
+            traffic_note = "not measured: %s" % e
 
     out = {
         "metric": "reduce HBM GB/s (% peak) at 64 PEs x 4M; p50/p99 latency "
@@ -588,25 +750,38 @@ def main():
                    "npes": P, "nreduce": N, "fp_mode": args.fp_mode,
                    "parallelism": "1 GPU, 64 emulated PEs"},
         "roofline": {"bound": "hbm",
-                     "kernel": kernel_key(CELLS[dom][0], CELLS[dom][1], mode),
+                     "kernel": " + ".join(cell_kernels[dom]),
+                     "cell": "%s %s" % (CELLS[dom][0], CELLS[dom][1]),
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
-                     "traffic": measured_traffic(
-                         kernel_key(CELLS[dom][0], CELLS[dom][1], mode)),
+                     "traffic": None if traffic is None else round(traffic),
+                     "traffic_ratio": None if traffic is None else
+                     round(traffic / dom_b, 4),
+                     "traffic_detail": traffic_note,
                      "algorithmic_bytes_per_launch": dom_b,
-                     "avg_launch_us": round(dom_avg * 1e6, 1)},
-        "copy_ceiling_GBps": round(copy_gbps, 1),
+                     "avg_launch_us": round(dom_avg * 1e6, 1),
+                     "launch": "one reduce call (its kernels back to back on "
+                               "the library stream), HIP events around it"},
+        "copy_ceiling_GBps": ceil["copy_1GiB_GBps"],
+        "ceilings": ceil,
+        "frac_of_read_then_write_bound": round(
+            value / ceil["read_then_write_GBps"], 4),
         "cells": cells,
         "accuracy": acc,
     }
     h.close()
+    log("movement")
     out["movement"] = None if args.no_movement else bench_movement()
+    log("batched")
     out["batched"] = None if args.no_batched else bench_batched()
+    log("served")
     out["served"] = None if args.no_served else bench_served()
     if not args.no_cpu:
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline_subprocess()
     else:
         out["cpu_baseline"] = None
+    log("done")
     print(json.dumps(out))
 
 

@@ -141,10 +141,11 @@ int osc_set_batch_branches(int n);
  * 8 or 16 bytes) run as a walk that stores member 0's result and a
  * replicate kernel that copies it to the other members, storing each loaded
  * piece to `group` members: -1 (default) takes 8 below 8 MiB per member and
- * from 8 MiB the faster of 4 and 8, timed on the first call of each team
- * size and power-of-two size class (that call waits for its stream); 1, 2,
- * 4, 8 or 16 fix it; 0 = one kernel that stores to every member.  Results
- * are identical either way.  OSC_REDUCE_SPLIT sets it at init. */
+ * from 8 MiB the faster of 4 and 8, timed (between events, without waiting)
+ * on the first two calls of each team size and power-of-two size class and
+ * fixed by a later one; 1, 2, 4, 8 or 16 fix it; 0 = one kernel that stores
+ * to every member.  Results are identical either way.  OSC_REDUCE_SPLIT
+ * sets it at init. */
 int osc_set_reduce_split(int group);
 /* The group the split uses for npes members and nbytes per member (0: not
  * timed yet). */
@@ -350,6 +351,17 @@ int osc_algo_resolve(const char *kind, const char *spec, int *fpmode);
 
 /* ---- raw device copy (the roofline calibration kernel) ------------------ */
 int osc_device_copy(void *dst, const void *src, size_t bytes, void *stream);
+/* The two halves of a collective's traffic alone, for measuring HBM
+ * ceilings: XOR of every 16-byte unit of [p, p + bytes) into the 16 bytes
+ * at sink16 (device memory, cleared first); and a fill of [p, p + bytes)
+ * with a 64-bit value.  p 16-byte aligned, bytes a multiple of 16. */
+int osc_device_xor_read(const void *p, size_t bytes, void *sink16,
+                        void *stream);
+int osc_device_fill64(void *p, size_t bytes, uint64_t value, void *stream);
+/* The kernels the calling thread's last reduction launched (the per-call
+ * path), named as rocprof names them and separated by ';' (e.g. the slice
+ * walk and the replicate of a split call); returns how many. */
+int osc_last_reduce_kernels(char *buf, size_t len);
 
 #ifdef __cplusplus
 }

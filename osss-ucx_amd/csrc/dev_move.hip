@@ -735,6 +735,105 @@ extern "C" int oscd_alltoall(const oscd_geom *g, uint64_t dest_off,
   return check_launch();
 }
 
+// ---- read-only and write-only streams -----------------------------------------
+// The two halves of every collective's traffic, alone, for measuring the
+// HBM ceilings beside the collectives.  k_xor_read folds every 16-byte unit
+// of a range into one value (each workgroup reads one contiguous 32 KiB
+// visit, the walk's read pattern, with non-temporal loads) and XORs its
+// result into a 16-byte sink with device atomics.  k_fill64 stores a 64-bit
+// value over a range in the replicate's interleaved write order.
+namespace osc {
+
+__global__ __launch_bounds__(kMoveBlock) void k_sink_clear(
+    unsigned long long *sink) {
+  if (threadIdx.x < 2) sink[threadIdx.x] = 0ull;
+}
+
+__global__ __launch_bounds__(kMoveBlock) void k_xor_read(
+    const char *__restrict__ p, const uint64_t nunits,
+    unsigned long long *sink) {
+  __shared__ unsigned long long red[2][kMoveBlock / 64];
+  typename Raw<16>::type a = {0u, 0u, 0u, 0u};
+  for (uint64_t c = blockIdx.x; c * kRepWgUnits < nunits; c += gridDim.x) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint64_t u = c * kRepWgUnits + (uint64_t)k * kMoveBlock + threadIdx.x;
+      if (u < nunits) a ^= ld_nt<16>(p + u * 16);
+    }
+  }
+  unsigned long long lo = ((unsigned long long)a.y << 32) | a.x;
+  unsigned long long hi = ((unsigned long long)a.w << 32) | a.z;
+  for (int d = 32; d > 0; d >>= 1) {
+    lo ^= __shfl_xor(lo, d);
+    hi ^= __shfl_xor(hi, d);
+  }
+  if (threadIdx.x % 64 == 0) {
+    red[0][threadIdx.x / 64] = lo;
+    red[1][threadIdx.x / 64] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    unsigned long long v = 0;
+    for (int w = 0; w < kMoveBlock / 64; ++w) v ^= red[threadIdx.x][w];
+    atomicXor(&sink[threadIdx.x], v);
+  }
+}
+
+__global__ __launch_bounds__(kMoveBlock) void k_fill64(
+    char *__restrict__ p, const uint64_t nunits, const uint64_t value,
+    const uint64_t total) {
+  const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
+  typename Raw<16>::type v;
+  v.x = (uint32_t)value;
+  v.y = (uint32_t)(value >> 32);
+  v.z = v.x;
+  v.w = v.y;
+  for (uint64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const uint64_t x = vb % kRepS, sc = vb / kRepS;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint64_t u = il_unit(sc, x, k, w, l);
+      if (u < nunits) st_nt<16>(p + u * 16, v);
+    }
+  }
+}
+
+}  // namespace osc
+
+extern "C" int oscd_xor_read(const void *p, uint64_t bytes, void *sink16,
+                             void *stream) {
+  using namespace osc;
+  if (p == nullptr || sink16 == nullptr || bytes % 16 != 0 ||
+      (uintptr_t)p % 16 != 0 || (uintptr_t)sink16 % 8 != 0)
+    return OSC_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_sink_clear, dim3(1), dim3(kMoveBlock), 0, s,
+                     (unsigned long long *)sink16);
+  if (bytes == 0) return check_launch();
+  const uint64_t nunits = bytes / 16;
+  // 2048 workgroups (8 per CU) loop over the 32 KiB visits: two sink
+  // atomics per workgroup, not per visit
+  const uint64_t visits = (nunits + kRepWgUnits - 1) / kRepWgUnits;
+  const unsigned grid = (unsigned)(visits < 2048 ? visits : 2048);
+  hipLaunchKernelGGL(k_xor_read, dim3(grid), dim3(kMoveBlock), 0, s,
+                     (const char *)p, nunits, (unsigned long long *)sink16);
+  return check_launch();
+}
+
+extern "C" int oscd_fill64(void *p, uint64_t bytes, uint64_t value,
+                           void *stream) {
+  using namespace osc;
+  if (p == nullptr || bytes % 16 != 0 || (uintptr_t)p % 16 != 0)
+    return OSC_ERR_ARG;
+  if (bytes == 0) return OSC_OK;
+  const uint64_t nunits = bytes / 16;
+  const uint64_t total = (nunits + kRepScUnits - 1) / kRepScUnits * kRepS;
+  const uint64_t grid = total < kRepMaxGrid ? total : kRepMaxGrid;
+  hipLaunchKernelGGL(k_fill64, dim3((unsigned)grid), dim3(kMoveBlock), 0,
+                     (hipStream_t)stream, (char *)p, nunits, value, total);
+  return check_launch();
+}
+
 extern "C" int oscd_copy(void *dst, const void *src, uint64_t bytes,
                          void *stream) {
   using namespace osc;

@@ -305,6 +305,36 @@ struct PolC64Acc {
   __device__ static cf64 out(Acc a) { return {a.re.hi, a.im.hi}; }
 };
 
+// ---- names of the kernels a call launched --------------------------------
+// Each launch records the address of a static string (two stores on the
+// calling thread), so measurement tools can name the kernels a reduce call
+// ran; oscd_last_reduce_kernels formats them as rocprof prints them
+// ("k_reduce<double, osc::PolF64Acc<5>, 2, 8, 1, 1, true>").  A name is the
+// __PRETTY_FUNCTION__ of a helper whose template parameters are the
+// kernel's, or a plain name.
+constexpr int kMaxLaunchNames = 8;
+struct LaunchNote {
+  const char *what;  // "k_reduce" ... or a plain name
+  const char *pretty;  // __PRETTY_FUNCTION__ of the helper, or nullptr
+};
+extern thread_local LaunchNote t_launch[kMaxLaunchNames];
+extern thread_local int t_nlaunch;
+inline void note_launch(const char *what, const char *pretty) {
+  if (t_nlaunch < kMaxLaunchNames) t_launch[t_nlaunch++] = {what, pretty};
+}
+template <class T, class Pol, int VEC, int UPT, int INFL, int MAXLV, bool ONE>
+const char *k_reduce_pf() {
+  return __PRETTY_FUNCTION__;
+}
+template <class T, class Pol, int G>
+const char *k_reduce_lanes_pf() {
+  return __PRETTY_FUNCTION__;
+}
+template <class T, class Pol>
+const char *k_reduce_gather_pf() {
+  return __PRETTY_FUNCTION__;
+}
+
 // ---- unit load / store ---------------------------------------------------
 template <class T, int VEC>
 __device__ __forceinline__ void load_unit(const char *p, T (&x)[VEC],
@@ -963,15 +993,19 @@ static int launch_lanes(const oscd_geom *g, uint64_t doff, uint64_t soff,
   const int G = lanes_group(P);
   const uint64_t per_block = (uint64_t)(kBlock / 64) * (64 / G);
   const unsigned grid = (unsigned)((n + per_block - 1) / per_block);
-  if (G == 64)
+  if (G == 64) {
     hipLaunchKernelGGL((k_reduce_lanes<T, Pol, 64>), dim3(grid), dim3(kBlock),
                        0, s, *g, doff, soff, n);
-  else if (G == 32)
+    note_launch("k_reduce_lanes", k_reduce_lanes_pf<T, Pol, 64>());
+  } else if (G == 32) {
     hipLaunchKernelGGL((k_reduce_lanes<T, Pol, 32>), dim3(grid), dim3(kBlock),
                        0, s, *g, doff, soff, n);
-  else
+    note_launch("k_reduce_lanes", k_reduce_lanes_pf<T, Pol, 32>());
+  } else {
     hipLaunchKernelGGL((k_reduce_lanes<T, Pol, 16>), dim3(grid), dim3(kBlock),
                        0, s, *g, doff, soff, n);
+    note_launch("k_reduce_lanes", k_reduce_lanes_pf<T, Pol, 16>());
+  }
   return hipGetLastError() == hipSuccess ? OSC_OK : OSC_ERR_HIP;
 }
 
@@ -1145,6 +1179,7 @@ static int launch_gather(const oscd_geom *g, int lg, uint64_t doff,
   const size_t lds = (size_t)(kBlock / 64) * (size_t)P * sizeof(T);
   hipLaunchKernelGGL((k_reduce_gather<T, Pol>), dim3(grid), dim3(kBlock), lds,
                      s, *g, doff, soff, n, lg);
+  note_launch("k_reduce_gather", k_reduce_gather_pf<T, Pol>());
   return hipGetLastError() == hipSuccess ? OSC_OK : OSC_ERR_HIP;
 }
 
@@ -1202,12 +1237,15 @@ static void launch_shape(const oscd_geom *g, int lg, uint64_t doff,
       hipLaunchKernelGGL((k_reduce<T, Pol, VEC, UPT, INFL, 13, ONE>),
                          dim3((unsigned)grid), dim3(kBlock), lds, s, *g, doff,
                          soff, n, lg, flags);
+      note_launch("k_reduce", k_reduce_pf<T, Pol, VEC, UPT, INFL, 13, ONE>());
       return;
     }
   }
   hipLaunchKernelGGL((k_reduce<T, Pol, VEC, UPT, INFL, Pol::TREE ? 8 : 1, ONE>),
                      dim3((unsigned)grid), dim3(kBlock), lds, s, *g, doff,
                      soff, n, lg, flags);
+  note_launch("k_reduce",
+              k_reduce_pf<T, Pol, VEC, UPT, INFL, Pol::TREE ? 8 : 1, ONE>());
 }
 
 // Large calls run in two kernels: the slice walk stores the result to

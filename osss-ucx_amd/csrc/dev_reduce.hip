@@ -21,11 +21,16 @@ int visit_reduce(int dtype, int op, int mode, PerCallV *pc, BatchV *bv,
 
 }  // namespace osc
 
+#include <string.h>
+
 #include <mutex>
+#include <string>
 #include <unordered_map>
 
 namespace osc {
 int g_split_group = -1;
+thread_local LaunchNote t_launch[kMaxLaunchNames];
+thread_local int t_nlaunch = 0;
 
 // Span flags of split float MIN/MAX reductions: one device buffer per
 // stream, since calls on different streams may run at the same time while
@@ -62,60 +67,102 @@ uint8_t *span_flags(hipStream_t s, uint64_t npieces) {
 // workgroup of the interleaved replicate differs between MI355X boxes (64
 // members x 32 MiB: 4 took 322-331 us where 8 took 345-355 on one box, and
 // 377-388 us on another where 8 took 329 us); within one box it held across
-// 16 placements of the arena.  So from 8 MiB per member, the first call of
-// each (team size, power-of-two size class) times groups 4 and 8 twice each
-// on its own destination -- a replicate is idempotent, the result is the
-// same after every run -- and later calls take the faster.  The first call
-// waits for its stream meanwhile.  Calls under graph capture, and calls
-// while the table is full, take group 4.
+// 16 placements of the arena.  So from 8 MiB per member, each (team size,
+// power-of-two size class) measures both on its first calls, without ever
+// waiting: its first call runs group 4 between two events, its second group
+// 8, and a later call that finds both pairs of events complete compares
+// them and fixes the group.  Until then (and under graph capture, and when
+// the table is full) calls take group 4.
 namespace {
 constexpr uint64_t kTuneMinBytes = 8ull << 20;
 constexpr int kTuneSlots = 64;
 struct TuneSlot {
   int npes;
   int cls;
-  int group;
+  int group;     // 0 while undecided
+  int recorded;  // timed runs recorded: 0, 1 (group 4), 2 (group 8)
+  hipEvent_t ev[4];
 };
 std::mutex g_tune_mu;
 TuneSlot g_tune[kTuneSlots];
 int g_tune_n = 0;
-hipEvent_t g_tune_ev[2] = {nullptr, nullptr};
 
 int size_class(uint64_t nbytes) { return 63 - __builtin_clzll(nbytes); }
 }  // namespace
 
+const char *replicate_name(int group) {
+  switch (group) {
+    case 1: return "k_replicate<1>";
+    case 2: return "k_replicate<2>";
+    case 4: return "k_replicate<4>";
+    case 16: return "k_replicate<16>";
+    default: return "k_replicate<8>";
+  }
+}
+
+int replicate_run(const oscd_geom *g, uint64_t off, uint64_t nbytes,
+                  int group, const uint8_t *skip, hipStream_t s) {
+  if (group < 0) group = nbytes >= kTuneMinBytes ? 4 : 8;
+  note_launch(replicate_name(group), nullptr);
+  return oscd_replicate(g, off, nbytes, group, skip, s);
+}
+
+static int replicate_timed(TuneSlot &t, int k, const oscd_geom *g,
+                           uint64_t off, uint64_t nbytes, const uint8_t *skip,
+                           hipStream_t s) {
+  const int group = k == 0 ? 4 : 8;
+  if (hipEventRecord(t.ev[2 * k], s) != hipSuccess) return OSC_ERR_HIP;
+  const int rc = replicate_run(g, off, nbytes, group, skip, s);
+  if (rc != OSC_OK) return rc;
+  if (hipEventRecord(t.ev[2 * k + 1], s) != hipSuccess) return OSC_ERR_HIP;
+  t.recorded = k + 1;
+  return OSC_OK;
+}
+
 int replicate_tuned(const oscd_geom *g, uint64_t off, uint64_t nbytes,
                     const uint8_t *skip, hipStream_t s) {
   if (g_split_group > 0 || nbytes < kTuneMinBytes)
-    return oscd_replicate(g, off, nbytes, g_split_group, skip, s);
+    return replicate_run(g, off, nbytes, g_split_group, skip, s);
   const int cls = size_class(nbytes);
   std::lock_guard<std::mutex> lk(g_tune_mu);
+  TuneSlot *t = nullptr;
   for (int i = 0; i < g_tune_n; ++i)
-    if (g_tune[i].npes == g->npes && g_tune[i].cls == cls)
-      return oscd_replicate(g, off, nbytes, g_tune[i].group, skip, s);
+    if (g_tune[i].npes == g->npes && g_tune[i].cls == cls) t = &g_tune[i];
+  if (t != nullptr && t->group != 0)
+    return replicate_run(g, off, nbytes, t->group, skip, s);
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
-  if (g_tune_n >= kTuneSlots || hipStreamIsCapturing(s, &st) != hipSuccess ||
+  if (hipStreamIsCapturing(s, &st) != hipSuccess ||
       st != hipStreamCaptureStatusNone)
-    return oscd_replicate(g, off, nbytes, 4, skip, s);
-  for (int e = 0; e < 2; ++e)
-    if (g_tune_ev[e] == nullptr && hipEventCreate(&g_tune_ev[e]) != hipSuccess)
-      return oscd_replicate(g, off, nbytes, 4, skip, s);
-  const int cand[2] = {4, 8};
-  float best[2] = {1e30f, 1e30f};
-  for (int rep = 0; rep < 2; ++rep)
-    for (int c = 0; c < 2; ++c) {
-      if (hipEventRecord(g_tune_ev[0], s) != hipSuccess) return OSC_ERR_HIP;
-      const int rc = oscd_replicate(g, off, nbytes, cand[c], skip, s);
-      if (rc != OSC_OK) return rc;
-      float ms = 0.0f;
-      if (hipEventRecord(g_tune_ev[1], s) != hipSuccess ||
-          hipEventSynchronize(g_tune_ev[1]) != hipSuccess ||
-          hipEventElapsedTime(&ms, g_tune_ev[0], g_tune_ev[1]) != hipSuccess)
-        return OSC_ERR_HIP;
-      if (ms < best[c]) best[c] = ms;
+    return replicate_run(g, off, nbytes, 4, skip, s);
+  if (t == nullptr) {
+    if (g_tune_n >= kTuneSlots)
+      return replicate_run(g, off, nbytes, 4, skip, s);
+    TuneSlot nt{g->npes, cls, 0, 0, {nullptr, nullptr, nullptr, nullptr}};
+    for (int e = 0; e < 4; ++e)
+      if (hipEventCreate(&nt.ev[e]) != hipSuccess) {
+        for (int f = 0; f < e; ++f) (void)hipEventDestroy(nt.ev[f]);
+        return replicate_run(g, off, nbytes, 4, skip, s);
+      }
+    g_tune[g_tune_n] = nt;
+    t = &g_tune[g_tune_n++];
+  }
+  if (t->recorded < 2) return replicate_timed(*t, t->recorded, g, off, nbytes, skip, s);
+  // both runs recorded: decide once both have completed
+  if (hipEventQuery(t->ev[1]) == hipSuccess &&
+      hipEventQuery(t->ev[3]) == hipSuccess) {
+    float a = 0.0f, b = 0.0f;
+    if (hipEventElapsedTime(&a, t->ev[0], t->ev[1]) == hipSuccess &&
+        hipEventElapsedTime(&b, t->ev[2], t->ev[3]) == hipSuccess)
+      t->group = b < a ? 8 : 4;
+    else
+      t->group = 4;
+    for (int e = 0; e < 4; ++e) {
+      (void)hipEventDestroy(t->ev[e]);
+      t->ev[e] = nullptr;
     }
-  g_tune[g_tune_n++] = {g->npes, cls, best[1] < best[0] ? 8 : 4};
-  return OSC_OK;
+    return replicate_run(g, off, nbytes, t->group, skip, s);
+  }
+  return replicate_run(g, off, nbytes, 4, skip, s);
 }
 
 int replicate_choice(int npes, uint64_t nbytes) {
@@ -130,12 +177,10 @@ int replicate_choice(int npes, uint64_t nbytes) {
 
 void replicate_tune_reset() {
   std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (int i = 0; i < g_tune_n; ++i)
+    for (int e = 0; e < 4; ++e)
+      if (g_tune[i].ev[e] != nullptr) (void)hipEventDestroy(g_tune[i].ev[e]);
   g_tune_n = 0;
-  for (int e = 0; e < 2; ++e)
-    if (g_tune_ev[e] != nullptr) {
-      (void)hipEventDestroy(g_tune_ev[e]);
-      g_tune_ev[e] = nullptr;
-    }
 }
 }  // namespace osc
 
@@ -179,10 +224,52 @@ extern "C" int oscd_reduce(const oscd_geom *g, int op, int dtype, int fpmode,
   if (g == nullptr || g->npes <= 0 || g->npes > 8191 || op < 0 ||
       op >= OSC_OP_COUNT)
     return OSC_ERR_ARG;
+  osc::t_nlaunch = 0;
   if (n == 0) return OSC_OK;
   PerCallV v{g, log2_floor(g->npes), dest_off, src_off, n, shape,
              (hipStream_t)stream};
   return visit_reduce(dtype, op, fpmode, &v, nullptr, nullptr);
+}
+
+// "const char *osc::k_reduce_pf() [T = double, Pol = osc::PolF64Acc<5>,
+// VEC = 2, ...]" -> "k_reduce<double, osc::PolF64Acc<5>, 2, ...>"
+static std::string kernel_name(const osc::LaunchNote &nt) {
+  if (nt.pretty == nullptr) return nt.what;
+  const std::string pf = nt.pretty;
+  const size_t a = pf.find('['), b = pf.rfind(']');
+  if (a == std::string::npos || b == std::string::npos || b < a) return nt.what;
+  std::string out = std::string(nt.what) + "<";
+  int depth = 0;
+  bool first = true;
+  size_t i = a + 1, item = i;
+  for (; i <= b; ++i) {
+    const char c = pf[i];
+    if (c == '<' || c == '(') ++depth;
+    if (c == '>' || c == ')') --depth;
+    if ((c == ',' && depth == 0) || i == b) {
+      const std::string kv = pf.substr(item, i - item);
+      const size_t eq = kv.find(" = ");
+      if (!first) out += ", ";
+      out += eq == std::string::npos ? kv : kv.substr(eq + 3);
+      first = false;
+      item = i + 2;  // skip ", "
+    }
+  }
+  return out + ">";
+}
+
+extern "C" int oscd_last_reduce_kernels(char *buf, size_t len) {
+  std::string all;
+  for (int i = 0; i < osc::t_nlaunch; ++i) {
+    if (i) all += ";";
+    all += kernel_name(osc::t_launch[i]);
+  }
+  if (buf != nullptr && len > 0) {
+    const size_t k = all.size() < len - 1 ? all.size() : len - 1;
+    memcpy(buf, all.data(), k);
+    buf[k] = '\0';
+  }
+  return osc::t_nlaunch;
 }
 
 extern "C" int oscd_batch_reduce(int dtype, int op, int fpmode, int shape,

@@ -327,6 +327,27 @@ int osc_device_copy(void *dst, const void *src, size_t bytes, void *stream) {
   return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "copy launch");
 }
 
+int osc_device_xor_read(const void *p, size_t bytes, void *sink16,
+                        void *stream) {
+  int rc = oscd_xor_read(p, bytes, sink16, stream ? stream : osc_g.stream);
+  if (rc == OSC_ERR_ARG)
+    return osc_set_error(rc, "xor read: %p (%zu bytes) needs 16-byte alignment",
+                         p, bytes);
+  return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "xor read launch");
+}
+
+int osc_device_fill64(void *p, size_t bytes, uint64_t value, void *stream) {
+  int rc = oscd_fill64(p, bytes, value, stream ? stream : osc_g.stream);
+  if (rc == OSC_ERR_ARG)
+    return osc_set_error(rc, "fill: %p (%zu bytes) needs 16-byte alignment",
+                         p, bytes);
+  return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "fill launch");
+}
+
+int osc_last_reduce_kernels(char *buf, size_t len) {
+  return oscd_last_reduce_kernels(buf, len);
+}
+
 /* "32M", "1G", "4096", "512k": binary multiples, as the reference's
  * SHMEM_SYMMETRIC_SIZE parser (src/shmemu/unitparse.c:72). */
 int osc_parse_size(const char *s, size_t *out) {

@@ -79,6 +79,14 @@ int oscd_reduce(const oscd_geom *g, int op, int dtype, int fpmode, int shape,
  * pieces with a nonzero byte are left alone. */
 int oscd_replicate(const oscd_geom *g, uint64_t off, uint64_t nbytes,
                    int group, const void *skip, void *stream);
+/* read-only / write-only streams (16-byte aligned, bytes a multiple of
+ * 16): XOR of every 16-byte unit into sink16 (cleared first); fill with a
+ * 64-bit value in the replicate's write order */
+int oscd_xor_read(const void *p, uint64_t bytes, void *sink16, void *stream);
+int oscd_fill64(void *p, uint64_t bytes, uint64_t value, void *stream);
+/* the kernels the calling thread's last oscd_reduce launched, as rocprof
+ * names them, ';'-separated; returns how many */
+int oscd_last_reduce_kernels(char *buf, size_t len);
 /* replicate group the reduce path uses for a team of npes members and
  * nbytes per member: the fixed group, 8 below 8 MiB, the measured one, or
  * 0 while no call of that size class has measured it yet */

@@ -102,6 +102,9 @@ def lib() -> C.CDLL:
         "osc_set_batch_branches": (i, [i]),
         "osc_set_reduce_split": (i, [i]),
         "osc_reduce_split_group": (i, [i, sz]),
+        "osc_device_xor_read": (i, [vp, sz, vp, vp]),
+        "osc_device_fill64": (i, [vp, sz, u64, vp]),
+        "osc_last_reduce_kernels": (i, [C.c_char_p, sz]),
         "osc_get_fp_mode": (i, []),
         "osc_strerror": (C.c_char_p, [i]),
         "osc_last_error": (C.c_char_p, []),
