@@ -352,8 +352,8 @@ def bench_movement():
             walls.append((time.perf_counter() - t0) * 1e6)
         walls.sort()
         out["int_sum_8pe_64K"] = {"kernel_us": round(kern * 1e3, 2),
-                                     "call_p50_us": round(walls[100], 2),
-                                     "call_p99_us": round(walls[198], 2)}
+                                     "python_call_p50_us": round(walls[100], 2),
+                                     "python_call_p99_us": round(walls[198], 2)}
     # small calls on large teams, 16 elements per PE: int64 (the cross-lane
     # butterfly kernel) and double (the gather kernel, team-order fold)
     n = 16
@@ -374,9 +374,28 @@ def bench_movement():
                     fn(h.world, d, s, n)
                     walls.append((time.perf_counter() - t0) * 1e6)
                 walls.sort()
-                out[f"{tn}_sum_{P}pe_16"] = {"kernel_us": round(kern * 1e3, 2),
-                                             "call_p50_us": round(walls[100], 2),
-                                             "call_p99_us": round(walls[198], 2)}
+                out[f"{tn}_sum_{P}pe_16"] = {
+                    "kernel_us": round(kern * 1e3, 2),
+                    "python_call_p50_us": round(walls[100], 2),
+                    "python_call_p99_us": round(walls[198], 2)}
+    # the same blocking calls timed from C (bin/latency: 2000 calls each,
+    # CLOCK_MONOTONIC), as a C program makes them: the Python loops above
+    # add the ctypes call (about 2 us)
+    import subprocess
+    exe = os.path.join(ROOT, "osss-ucx_amd", "bin", "latency")
+    try:
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+        if r.returncode != 0:
+            raise RuntimeError(r.stderr[-500:])
+        lat = json.loads(r.stdout.strip().splitlines()[-1])
+        for k, v in lat.items():
+            out.setdefault(k, {}).update({"call_p50_us": v["p50_us"],
+                                          "call_p90_us": v["p90_us"],
+                                          "call_p99_us": v["p99_us"]})
+        out["call_latency_source"] = "bin/latency (osss-ucx_amd/examples/" \
+                                     "latency.c), 2000 calls from C"
+    except (OSError, RuntimeError, ValueError, subprocess.TimeoutExpired) as e:
+        out["call_latency_source"] = "C timing failed: %s" % e
     return out
 
 
