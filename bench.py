@@ -588,9 +588,9 @@ def matrix_heap(L, only=None):
 
 
 def traffic_probe(L, cell, mode):
-    """Child process of measured_traffic (run under rocprofv3 --pmc): one
-    warm-up call of CELLS[cell], three 1 GiB copies (the counters'
-    calibration), three calls of the cell."""
+    """Child process of measured_traffic (run under rocprofv3 --pmc): ten
+    warm-up calls of CELLS[cell] (past the replicate layout timing), three
+    1 GiB copies (the counters' calibration), three calls of the cell."""
     op, dt, dist, a, b = CELLS[cell]
     h, src, dest, _ = matrix_heap(L, only=(dt, dist, a, b))
     osc.check(L.osc_set_fp_mode(mode), "fp mode")
@@ -599,6 +599,9 @@ def traffic_probe(L, cell, mode):
     def call():
         osc.check(L.osc_reduce(h.world, osc.OPS[op], osc.DTYPES[dt], dest,
                                src[(dt, dist, a, b)], N, st), "osc_reduce")
+    for _ in range(9):  # past the replicate layout timing
+        call()
+    h.sync()
     call()
     h.sync()
     base = L.osc_heap_base()
@@ -696,15 +699,17 @@ def main():
         if record:
             L.osc_event_record(evs[-1], stream)
 
-    # the kernels each cell launches (the first call of a size class also
-    # times the replicate groups)
+    # warm-up (the first calls of each size class also time the replicate
+    # layouts: at least 9 calls and a completed stream), then the kernels
+    # each cell launches from then on
+    for _ in range(max(args.warmup, 3)):
+        step(False)
+    h.sync()
     cell_kernels = []
     for i, (op, dt, dist, a, b) in enumerate(CELLS):
         osc.check(L.osc_reduce(h.world, osc.OPS[op], osc.DTYPES[dt], dest,
                                src[(dt, dist, a, b)], N, stream), "osc_reduce")
         cell_kernels.append(last_kernels(L))
-    for _ in range(args.warmup):
-        step(False)
     h.sync()
     log("matrix: %d timed steps" % args.steps)
     cell_ms = [0.0] * len(CELLS)

@@ -140,16 +140,18 @@ int osc_set_batch_branches(int n);
 /* Large reductions (1 MiB or more per member, 8+ members, elements of 2, 4,
  * 8 or 16 bytes) run as a walk that stores member 0's result and a
  * replicate kernel that copies it to the other members, storing each loaded
- * piece to `group` members: -1 (default) takes 8 below 8 MiB per member and
- * from 8 MiB the faster of 4 and 8, timed (between events, without waiting)
- * on the first two calls of each team size and power-of-two size class and
- * fixed by a later one; 1, 2, 4, 8 or 16 fix it; 0 = one kernel that stores
- * to every member.  Results are identical either way.  OSC_REDUCE_SPLIT
- * sets it at init. */
+ * piece to `group` members, the pieces interleaved over `width` workgroups.
+ * -1 (default): 8 members and width 16 below 8 MiB per member; from 8 MiB
+ * the fastest of (4, 16), (8, 16), (2, 32) and (4, 32) per byte written,
+ * timed between events (without waiting) on the first eight calls of each
+ * power-of-two size class and fixed by a later one.  1, 2, 4, 8 or 16 fix
+ * the group (width 16); 0 = one kernel that stores to every member.
+ * Results are identical either way.  OSC_REDUCE_SPLIT sets it at init. */
 int osc_set_reduce_split(int group);
 /* The group the split uses for npes members and nbytes per member (0: not
- * timed yet). */
-int osc_reduce_split_group(int npes, size_t nbytes);
+ * timed yet; the team size does not enter the choice); *width (may be
+ * NULL) receives the interleave width. */
+int osc_reduce_split_group(int npes, size_t nbytes, int *width);
 const char *osc_strerror(int code);
 const char *osc_last_error(void);
 int osc_last_error_code(void); /* code of the calling thread's last error */

@@ -52,14 +52,15 @@ constexpr int kRepS = 16;                         // workgroups per group
 constexpr uint64_t kRepPiece = 64;                // units (1 KiB) per piece
 constexpr uint64_t kRepWgUnits = kMoveBlock * 8;  // 2048 units per visit
 constexpr uint64_t kRepScUnits = kRepS * kRepWgUnits;  // per super-chunk
-constexpr uint64_t kRepMaxGrid = 1u << 20;        // a multiple of kRepS
+constexpr uint64_t kRepMaxGrid = 1u << 20;        // a multiple of every S
 
 // unit of the region that lane l of wave w stores as its k-th unit, for
-// workgroup x of the interleave group of super-chunk sc
+// workgroup x of the interleave group (S workgroups) of super-chunk sc
+template <int S = kRepS>
 __device__ __forceinline__ uint64_t il_unit(uint64_t sc, uint64_t x, int k,
                                             uint32_t w, uint32_t l) {
   const uint64_t q = (uint64_t)(k * 4) + w;
-  return sc * kRepScUnits + (q * kRepS + x) * kRepPiece + l;
+  return sc * (uint64_t)S * kRepWgUnits + (q * S + x) * kRepPiece + l;
 }
 
 // Gather, destination-major: member j's [doff + i*doff_step, +nbytes) =
@@ -481,7 +482,7 @@ extern "C" int oscd_collect(const oscd_geom *g, uint64_t dest_off,
 // fastest, 5.3-5.5 TB/s).
 namespace osc {
 
-template <int G>
+template <int G, int S>
 __global__ __launch_bounds__(kMoveBlock) void k_replicate(
     const oscd_geom g, const uint64_t off, const uint64_t nunits,
     const uint64_t ngroups, const uint64_t total,
@@ -489,16 +490,16 @@ __global__ __launch_bounds__(kMoveBlock) void k_replicate(
   const int P = g.npes;
   const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
   const char *src = member_ptr(g, 0, off);
-  // gridDim.x is a multiple of kRepS, so every visit of a workgroup keeps
-  // its place x in the interleave group
+  // gridDim.x is a multiple of S, so every visit of a workgroup keeps its
+  // place x in the interleave group
   for (uint64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
-    const uint64_t x = vb % kRepS, r = vb / kRepS;
+    const uint64_t x = vb % S, r = vb / S;
     const uint64_t gi = r % ngroups, sc = r / ngroups;
     uint64_t u[8];
     typename Raw<16>::type v[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      u[k] = il_unit(sc, x, k, w, l);
+      u[k] = il_unit<S>(sc, x, k, w, l);
       if (u[k] < nunits) v[k] = ld<16>(src + u[k] * 16);
     }
     if (skip != nullptr) {
@@ -508,7 +509,7 @@ __global__ __launch_bounds__(kMoveBlock) void k_replicate(
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const uint64_t pc = (uint64_t)__builtin_amdgcn_readfirstlane(
-            (uint32_t)(il_unit(sc, x, k, w, 0) / 256));
+            (uint32_t)(il_unit<S>(sc, x, k, w, 0) / 256));
         if (u[k] < nunits && skip[pc] != 0) u[k] = nunits;
       }
     }
@@ -524,16 +525,18 @@ __global__ __launch_bounds__(kMoveBlock) void k_replicate(
   }
 }
 
-template <int G>
+template <int G, int S>
 static int replicate_g(const oscd_geom *g, uint64_t off, uint64_t nbytes,
                        const uint8_t *skip, hipStream_t s) {
   const uint64_t nunits = nbytes / 16;
-  const uint64_t nsc = (nunits + kRepScUnits - 1) / kRepScUnits;
+  const uint64_t scu = (uint64_t)S * kRepWgUnits;
+  const uint64_t nsc = (nunits + scu - 1) / scu;
   const uint64_t ngroups = ((uint64_t)g->npes - 1 + G - 1) / G;
-  const uint64_t total = nsc * ngroups * kRepS;
+  const uint64_t total = nsc * ngroups * S;
   const uint64_t grid = total < kRepMaxGrid ? total : kRepMaxGrid;
-  hipLaunchKernelGGL((k_replicate<G>), dim3((unsigned)grid), dim3(kMoveBlock),
-                     0, s, *g, off, nunits, ngroups, total, skip);
+  hipLaunchKernelGGL((k_replicate<G, S>), dim3((unsigned)grid),
+                     dim3(kMoveBlock), 0, s, *g, off, nunits, ngroups, total,
+                     skip);
   return check_launch();
 }
 
@@ -548,19 +551,29 @@ static int replicate_auto_group(uint64_t nbytes) {
 }  // namespace osc
 
 extern "C" int oscd_replicate(const oscd_geom *g, uint64_t off, uint64_t nbytes,
-                              int group, const void *skip, void *stream) {
+                              int group, int width, const void *skip,
+                              void *stream) {
   using namespace osc;
   if (g == nullptr || g->npes <= 0 || nbytes % 16 != 0) return OSC_ERR_ARG;
   if (nbytes == 0 || g->npes == 1) return OSC_OK;
   hipStream_t s = (hipStream_t)stream;
   const uint8_t *sk = (const uint8_t *)skip;
   if (group < 0) group = replicate_auto_group(nbytes);
+  if (width == 32) {
+    switch (group) {
+      case 1: return replicate_g<1, 32>(g, off, nbytes, sk, s);
+      case 2: return replicate_g<2, 32>(g, off, nbytes, sk, s);
+      case 4: return replicate_g<4, 32>(g, off, nbytes, sk, s);
+      case 16: return replicate_g<16, 32>(g, off, nbytes, sk, s);
+      default: return replicate_g<8, 32>(g, off, nbytes, sk, s);
+    }
+  }
   switch (group) {
-    case 1: return replicate_g<1>(g, off, nbytes, sk, s);
-    case 2: return replicate_g<2>(g, off, nbytes, sk, s);
-    case 4: return replicate_g<4>(g, off, nbytes, sk, s);
-    case 16: return replicate_g<16>(g, off, nbytes, sk, s);
-    default: return replicate_g<8>(g, off, nbytes, sk, s);
+    case 1: return replicate_g<1, 16>(g, off, nbytes, sk, s);
+    case 2: return replicate_g<2, 16>(g, off, nbytes, sk, s);
+    case 4: return replicate_g<4, 16>(g, off, nbytes, sk, s);
+    case 16: return replicate_g<16, 16>(g, off, nbytes, sk, s);
+    default: return replicate_g<8, 16>(g, off, nbytes, sk, s);
   }
 }
 
@@ -792,7 +805,7 @@ __global__ __launch_bounds__(kMoveBlock) void k_fill64(
     const uint64_t x = vb % kRepS, sc = vb / kRepS;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const uint64_t u = il_unit(sc, x, k, w, l);
+      const uint64_t u = il_unit<kRepS>(sc, x, k, w, l);
       if (u < nunits) st_nt<16>(p + u * 16, v);
     }
   }

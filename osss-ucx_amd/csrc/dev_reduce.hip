@@ -63,25 +63,32 @@ uint8_t *span_flags(hipStream_t s, uint64_t npieces) {
   g_flag_bufs.emplace(s, (uint8_t *)p);
   return (uint8_t *)p;
 }
-// Replicate group by measurement.  The fastest number of members per
-// workgroup of the interleaved replicate differs between MI355X boxes (64
-// members x 32 MiB: 4 took 322-331 us where 8 took 345-355 on one box, and
-// 377-388 us on another where 8 took 329 us); within one box it held across
-// 16 placements of the arena.  So from 8 MiB per member, each (team size,
-// power-of-two size class) measures both on its first calls, without ever
-// waiting: its first call runs group 4 between two events, its second group
-// 8, and a later call that finds both pairs of events complete compares
-// them and fixes the group.  Until then (and under graph capture, and when
-// the table is full) calls take group 4.
+// Replicate layout by measurement.  The fastest replicate layout -- members
+// per workgroup G and workgroups per interleave group S -- differs between
+// MI355X boxes: at 64 members x 32 MiB, 232 MiB apart, one box took
+// 321-331 us with (G, S) = (4, 16), 345-355 with (8, 16) and 389-391 with
+// (4, 32); another took 370-374 us with (4, 16), 356-358 with (4, 32) and
+// 348-351 with (2, 32).  Within one box the order held across 16
+// placements of the arena, and the same candidate won at 16 and 32 MiB per
+// member.  So from 8 MiB per member, each power-of-two size class measures
+// four candidates on its first calls, without ever waiting: each of its
+// first eight calls (whatever their team size) runs the next candidate
+// between two events (two rounds), and a later call that finds every pair
+// of events complete fixes the candidate with the lowest time per byte
+// written.  Until then (and under graph capture, and when the table is
+// full) calls take (4, 16).
 namespace {
 constexpr uint64_t kTuneMinBytes = 8ull << 20;
 constexpr int kTuneSlots = 64;
+constexpr int kCand = 4, kRounds = 2;
+constexpr int kCandG[kCand] = {4, 8, 2, 4};
+constexpr int kCandS[kCand] = {16, 16, 32, 32};
 struct TuneSlot {
-  int npes;
   int cls;
-  int group;     // 0 while undecided
-  int recorded;  // timed runs recorded: 0, 1 (group 4), 2 (group 8)
-  hipEvent_t ev[4];
+  int choice;    // candidate index, -1 while undecided
+  int recorded;  // timed runs recorded, 0..kCand*kRounds
+  hipEvent_t ev[2 * kCand * kRounds];
+  double bytes[kCand * kRounds];  // bytes each timed run wrote
 };
 std::mutex g_tune_mu;
 TuneSlot g_tune[kTuneSlots];
@@ -90,102 +97,124 @@ int g_tune_n = 0;
 int size_class(uint64_t nbytes) { return 63 - __builtin_clzll(nbytes); }
 }  // namespace
 
-const char *replicate_name(int group) {
-  switch (group) {
-    case 1: return "k_replicate<1>";
-    case 2: return "k_replicate<2>";
-    case 4: return "k_replicate<4>";
-    case 16: return "k_replicate<16>";
-    default: return "k_replicate<8>";
-  }
+const char *replicate_name(int group, int width) {
+  static const char *const n16[] = {"k_replicate<1, 16>", "k_replicate<2, 16>",
+                                    "k_replicate<4, 16>", "k_replicate<8, 16>",
+                                    "k_replicate<16, 16>"};
+  static const char *const n32[] = {"k_replicate<1, 32>", "k_replicate<2, 32>",
+                                    "k_replicate<4, 32>", "k_replicate<8, 32>",
+                                    "k_replicate<16, 32>"};
+  const int i = group == 1 ? 0 : group == 2 ? 1 : group == 4 ? 2
+              : group == 16 ? 4 : 3;
+  return width == 32 ? n32[i] : n16[i];
 }
 
 int replicate_run(const oscd_geom *g, uint64_t off, uint64_t nbytes,
-                  int group, const uint8_t *skip, hipStream_t s) {
+                  int group, int width, const uint8_t *skip, hipStream_t s) {
   if (group < 0) group = nbytes >= kTuneMinBytes ? 4 : 8;
-  note_launch(replicate_name(group), nullptr);
-  return oscd_replicate(g, off, nbytes, group, skip, s);
-}
-
-static int replicate_timed(TuneSlot &t, int k, const oscd_geom *g,
-                           uint64_t off, uint64_t nbytes, const uint8_t *skip,
-                           hipStream_t s) {
-  const int group = k == 0 ? 4 : 8;
-  if (hipEventRecord(t.ev[2 * k], s) != hipSuccess) return OSC_ERR_HIP;
-  const int rc = replicate_run(g, off, nbytes, group, skip, s);
-  if (rc != OSC_OK) return rc;
-  if (hipEventRecord(t.ev[2 * k + 1], s) != hipSuccess) return OSC_ERR_HIP;
-  t.recorded = k + 1;
-  return OSC_OK;
+  note_launch(replicate_name(group, width), nullptr);
+  return oscd_replicate(g, off, nbytes, group, width, skip, s);
 }
 
 int replicate_tuned(const oscd_geom *g, uint64_t off, uint64_t nbytes,
                     const uint8_t *skip, hipStream_t s) {
   if (g_split_group > 0 || nbytes < kTuneMinBytes)
-    return replicate_run(g, off, nbytes, g_split_group, skip, s);
+    return replicate_run(g, off, nbytes, g_split_group, 16, skip, s);
   const int cls = size_class(nbytes);
   std::lock_guard<std::mutex> lk(g_tune_mu);
   TuneSlot *t = nullptr;
   for (int i = 0; i < g_tune_n; ++i)
-    if (g_tune[i].npes == g->npes && g_tune[i].cls == cls) t = &g_tune[i];
-  if (t != nullptr && t->group != 0)
-    return replicate_run(g, off, nbytes, t->group, skip, s);
+    if (g_tune[i].cls == cls) t = &g_tune[i];
+  if (t != nullptr && t->choice >= 0)
+    return replicate_run(g, off, nbytes, kCandG[t->choice],
+                         kCandS[t->choice], skip, s);
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &st) != hipSuccess ||
       st != hipStreamCaptureStatusNone)
-    return replicate_run(g, off, nbytes, 4, skip, s);
+    return replicate_run(g, off, nbytes, 4, 16, skip, s);
   if (t == nullptr) {
-    if (g_tune_n >= kTuneSlots)
-      return replicate_run(g, off, nbytes, 4, skip, s);
-    TuneSlot nt{g->npes, cls, 0, 0, {nullptr, nullptr, nullptr, nullptr}};
-    for (int e = 0; e < 4; ++e)
+    if (g_tune_n >= kTuneSlots) return replicate_run(g, off, nbytes, 4, 16, skip, s);
+    TuneSlot nt{};
+    nt.cls = cls;
+    nt.choice = -1;
+    for (int e = 0; e < 2 * kCand * kRounds; ++e)
       if (hipEventCreate(&nt.ev[e]) != hipSuccess) {
         for (int f = 0; f < e; ++f) (void)hipEventDestroy(nt.ev[f]);
-        return replicate_run(g, off, nbytes, 4, skip, s);
+        (void)hipGetLastError();
+        return replicate_run(g, off, nbytes, 4, 16, skip, s);
       }
     g_tune[g_tune_n] = nt;
     t = &g_tune[g_tune_n++];
   }
-  if (t->recorded < 2) return replicate_timed(*t, t->recorded, g, off, nbytes, skip, s);
-  // both runs recorded: decide once both have completed
-  if (hipEventQuery(t->ev[1]) == hipSuccess &&
-      hipEventQuery(t->ev[3]) == hipSuccess) {
-    float a = 0.0f, b = 0.0f;
-    if (hipEventElapsedTime(&a, t->ev[0], t->ev[1]) == hipSuccess &&
-        hipEventElapsedTime(&b, t->ev[2], t->ev[3]) == hipSuccess)
-      t->group = b < a ? 8 : 4;
-    else
-      t->group = 4;
-    for (int e = 0; e < 4; ++e) {
+  if (t->recorded < kCand * kRounds) {
+    const int r = t->recorded, c = r % kCand;
+    if (hipEventRecord(t->ev[2 * r], s) != hipSuccess) return OSC_ERR_HIP;
+    const int rc = replicate_run(g, off, nbytes, kCandG[c], kCandS[c], skip, s);
+    if (rc != OSC_OK) return rc;
+    if (hipEventRecord(t->ev[2 * r + 1], s) != hipSuccess) return OSC_ERR_HIP;
+    t->bytes[r] = (double)(g->npes - 1) * (double)nbytes;
+    t->recorded = r + 1;
+    return OSC_OK;
+  }
+  // every run recorded: decide once all of them have completed (calls of
+  // one size class may come on different streams)
+  bool done = true;
+  for (int r = 0; r < kCand * kRounds && done; ++r)
+    done = hipEventQuery(t->ev[2 * r + 1]) == hipSuccess;
+  (void)hipGetLastError();
+  if (done) {
+    float best[kCand];
+    for (int c = 0; c < kCand; ++c) best[c] = 1e30f;
+    for (int r = 0; r < kCand * kRounds; ++r) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, t->ev[2 * r], t->ev[2 * r + 1]) ==
+              hipSuccess &&
+          t->bytes[r] > 0.0) {
+        const float per_byte = (float)(ms / t->bytes[r]);
+        if (per_byte < best[r % kCand]) best[r % kCand] = per_byte;
+      }
+    }
+    int bc = 0;
+    for (int c = 1; c < kCand; ++c)
+      if (best[c] < best[bc]) bc = c;
+    t->choice = bc;
+    for (int e = 0; e < 2 * kCand * kRounds; ++e) {
       (void)hipEventDestroy(t->ev[e]);
       t->ev[e] = nullptr;
     }
-    return replicate_run(g, off, nbytes, t->group, skip, s);
+    (void)hipGetLastError();
+    return replicate_run(g, off, nbytes, kCandG[bc], kCandS[bc], skip, s);
   }
-  return replicate_run(g, off, nbytes, 4, skip, s);
+  return replicate_run(g, off, nbytes, 4, 16, skip, s);
 }
 
-int replicate_choice(int npes, uint64_t nbytes) {
+int replicate_choice(int npes, uint64_t nbytes, int *width) {
+  if (width != nullptr) *width = 16;
   if (g_split_group > 0) return g_split_group;
   if (nbytes < kTuneMinBytes) return 8;
   const int cls = size_class(nbytes);
   std::lock_guard<std::mutex> lk(g_tune_mu);
+  (void)npes;  // the size class alone keys the measurement
   for (int i = 0; i < g_tune_n; ++i)
-    if (g_tune[i].npes == npes && g_tune[i].cls == cls) return g_tune[i].group;
+    if (g_tune[i].cls == cls) {
+      if (g_tune[i].choice < 0) return 0;
+      if (width != nullptr) *width = kCandS[g_tune[i].choice];
+      return kCandG[g_tune[i].choice];
+    }
   return 0;
 }
 
 void replicate_tune_reset() {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   for (int i = 0; i < g_tune_n; ++i)
-    for (int e = 0; e < 4; ++e)
+    for (int e = 0; e < 2 * kCand * kRounds; ++e)
       if (g_tune[i].ev[e] != nullptr) (void)hipEventDestroy(g_tune[i].ev[e]);
   g_tune_n = 0;
 }
 }  // namespace osc
 
-extern "C" int oscd_replicate_group(int npes, uint64_t nbytes) {
-  return osc::replicate_choice(npes, nbytes);
+extern "C" int oscd_replicate_group(int npes, uint64_t nbytes, int *width) {
+  return osc::replicate_choice(npes, nbytes, width);
 }
 
 extern "C" void oscd_set_span_flag_capacity(uint64_t bytes_per_member) {

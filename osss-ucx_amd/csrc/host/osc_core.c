@@ -207,8 +207,8 @@ int osc_set_reduce_split(int group) {
   oscd_set_reduce_split(group != 0, group);
   return OSC_OK;
 }
-int osc_reduce_split_group(int npes, size_t nbytes) {
-  return oscd_replicate_group(npes, (uint64_t)nbytes);
+int osc_reduce_split_group(int npes, size_t nbytes, int *width) {
+  return oscd_replicate_group(npes, (uint64_t)nbytes, width);
 }
 int osc_npes(void) { return osc_g.npes; }
 size_t osc_slice_bytes(void) { return osc_g.slice; }

@@ -74,11 +74,12 @@ int oscd_reduce(const oscd_geom *g, int op, int dtype, int fpmode, int shape,
 
 /* replicate: member j's [off, off + nbytes) = member 0's, for every j
  * (nbytes a multiple of 16), each loaded piece stored to `group` (1, 2, 4,
- * 8 or 16; -1 picks by size) consecutive members; the second kernel of a
- * large reduction.  skip (device bytes, one per 4096-byte piece, or NULL):
+ * 8 or 16; -1 picks by size) consecutive members, the pieces interleaved
+ * over `width` (16 or 32) workgroups; the second kernel of a large
+ * reduction.  skip (device bytes, one per 4096-byte piece, or NULL):
  * pieces with a nonzero byte are left alone. */
 int oscd_replicate(const oscd_geom *g, uint64_t off, uint64_t nbytes,
-                   int group, const void *skip, void *stream);
+                   int group, int width, const void *skip, void *stream);
 /* read-only / write-only streams (16-byte aligned, bytes a multiple of
  * 16): XOR of every 16-byte unit into sink16 (cleared first); fill with a
  * 64-bit value in the replicate's write order */
@@ -89,8 +90,9 @@ int oscd_fill64(void *p, uint64_t bytes, uint64_t value, void *stream);
 int oscd_last_reduce_kernels(char *buf, size_t len);
 /* replicate group the reduce path uses for a team of npes members and
  * nbytes per member: the fixed group, 8 below 8 MiB, the measured one, or
- * 0 while no call of that size class has measured it yet */
-int oscd_replicate_group(int npes, uint64_t nbytes);
+ * 0 while no call of that size class has measured it yet; *width (may be
+ * NULL) receives its interleave width */
+int oscd_replicate_group(int npes, uint64_t nbytes, int *width);
 /* span flags of split float MIN/MAX reductions: per-stream device buffers
  * of one byte per 4096 bytes of the largest per-member call
  * (bytes_per_member, the heap slice); release frees them all */

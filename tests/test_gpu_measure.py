@@ -96,8 +96,9 @@ def test_kernel_names_of_a_split_call_and_a_small_call(heap):
         assert names[0].startswith("k_reduce<unsigned long, osc::PolSeq<"
                                    "unsigned long, 5, false>"), names
         assert names[0].endswith(", true>")  # stores member 0 only
-        assert names[1] in ("k_replicate<4>", "k_replicate<8>"), names
-        assert heap.L.osc_reduce_split_group(WORLD, n * 8) == 8
+        # below 8 MiB per member: 8 members per workgroup, width 16
+        assert names[1] == "k_replicate<8, 16>", names
+        assert heap.L.osc_reduce_split_group(WORLD, n * 8, None) == 8
         # one kernel when the split is off
         osc.check(heap.L.osc_set_reduce_split(0), "split off")
         osc.check(heap.L.osc_reduce(heap.world, osc.OPS["sum"],
@@ -117,12 +118,16 @@ def test_kernel_names_of_a_split_call_and_a_small_call(heap):
         heap.free(d)
 
 
-def test_timed_group_is_decided_without_waiting(heap):
-    """From 8 MiB per member the first call of a size class runs group 4
-    between events, the second group 8; a later call that finds both timed
-    runs complete fixes the faster one."""
+def test_timed_layout_is_decided_without_waiting(heap):
+    """From 8 MiB per member the first eight calls of a size class run the
+    four replicate layouts (members per workgroup, interleave width) twice,
+    each between events; a later call that finds every timed run complete
+    fixes the fastest, and later calls keep it."""
     n = (8 << 20) // 8 + 2 * 512  # 8 MiB per member and a bit: timed
     s, d = heap.malloc(n * 8), heap.malloc(n * 8)
+    cands = ["k_replicate<4, 16>", "k_replicate<8, 16>",
+             "k_replicate<2, 32>", "k_replicate<4, 32>"]
+    width = C.c_int(0)
     try:
         heap.fill(None, s, n, "u64", "bits", seed=4)
         st = heap.L.osc_default_stream()
@@ -132,15 +137,15 @@ def test_timed_group_is_decided_without_waiting(heap):
                                         osc.DTYPES["u64"], d, s, n, st),
                       "reduce")
             return last_kernels(heap)[-1]
-        assert heap.L.osc_reduce_split_group(WORLD, n * 8) == 0
-        assert call() == "k_replicate<4>"
-        assert call() == "k_replicate<8>"
-        assert heap.L.osc_reduce_split_group(WORLD, n * 8) == 0
+        assert heap.L.osc_reduce_split_group(WORLD, n * 8, None) == 0
+        assert [call() for _ in range(8)] == cands * 2
+        assert heap.L.osc_reduce_split_group(WORLD, n * 8, None) == 0
         heap.sync()
         last = call()
-        g = heap.L.osc_reduce_split_group(WORLD, n * 8)
-        assert g in (4, 8)
-        assert last == "k_replicate<%d>" % g
+        g = heap.L.osc_reduce_split_group(WORLD, n * 8, C.byref(width))
+        assert "k_replicate<%d, %d>" % (g, width.value) == last
+        assert last in cands
+        assert call() == last
         # the result: every member equals member 0, which is the XOR
         heap.sync()
         m0 = heap.get(d, n, np.uint64, 0)
