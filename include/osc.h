@@ -142,9 +142,9 @@ int osc_set_batch_branches(int n);
  * replicate kernel that copies it to the other members, storing each loaded
  * piece to `group` members, the pieces interleaved over `width` workgroups.
  * -1 (default): 8 members and width 16 below 8 MiB per member; from 8 MiB
- * the fastest of (4, 16), (8, 16), (2, 32) and (4, 32) per byte written,
- * timed between events (without waiting) on the first eight calls of each
- * power-of-two size class and fixed by a later one.  1, 2, 4, 8 or 16 fix
+ * the fastest of (4, 16), (8, 16), (2, 32), (4, 32), (1, 32) and (1, 64)
+ * per byte written, timed between events (without waiting) on the first
+ * twelve calls of each power-of-two size class and fixed by a later one.  1, 2, 4, 8 or 16 fix
  * the group (width 16); 0 = one kernel that stores to every member.
  * Results are identical either way.  OSC_REDUCE_SPLIT sets it at init. */
 int osc_set_reduce_split(int group);

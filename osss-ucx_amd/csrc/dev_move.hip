@@ -559,6 +559,15 @@ extern "C" int oscd_replicate(const oscd_geom *g, uint64_t off, uint64_t nbytes,
   hipStream_t s = (hipStream_t)stream;
   const uint8_t *sk = (const uint8_t *)skip;
   if (group < 0) group = replicate_auto_group(nbytes);
+  if (width == 64) {
+    switch (group) {
+      case 1: return replicate_g<1, 64>(g, off, nbytes, sk, s);
+      case 2: return replicate_g<2, 64>(g, off, nbytes, sk, s);
+      case 4: return replicate_g<4, 64>(g, off, nbytes, sk, s);
+      case 16: return replicate_g<16, 64>(g, off, nbytes, sk, s);
+      default: return replicate_g<8, 64>(g, off, nbytes, sk, s);
+    }
+  }
   if (width == 32) {
     switch (group) {
       case 1: return replicate_g<1, 32>(g, off, nbytes, sk, s);

@@ -68,21 +68,26 @@ uint8_t *span_flags(hipStream_t s, uint64_t npieces) {
 // MI355X boxes: at 64 members x 32 MiB, 232 MiB apart, one box took
 // 321-331 us with (G, S) = (4, 16), 345-355 with (8, 16) and 389-391 with
 // (4, 32); another took 370-374 us with (4, 16), 356-358 with (4, 32) and
-// 348-351 with (2, 32).  Within one box the order held across 16
-// placements of the arena, and the same candidate won at 16 and 32 MiB per
-// member.  So from 8 MiB per member, each power-of-two size class measures
-// four candidates on its first calls, without ever waiting: each of its
-// first eight calls (whatever their team size) runs the next candidate
-// between two events (two rounds), and a later call that finds every pair
-// of events complete fixes the candidate with the lowest time per byte
-// written.  Until then (and under graph capture, and when the table is
-// full) calls take (4, 16).
+// 348-351 with (2, 32); a third took 368 us with (4, 16), 346 with
+// (2, 32), 338 with (1, 64) and 332 with (1, 32) against 328 us for one
+// plain 2 GiB fill.  (With one member per workgroup, the 63 workgroup
+// groups of a super-chunk run together and workgroup x of each runs on the
+// same XCD, so the 63 reads of a piece after the first come from that
+// XCD's L2.)  Within one box the order held across 16 placements of the
+// arena, and the same candidate won at 16 and 32 MiB per member.  So from
+// 8 MiB per member, each power-of-two size class measures six candidates
+// on its first calls, without ever waiting: each of its first twelve calls
+// (whatever their team size) runs the next candidate between two events
+// (two rounds), and a later call that finds every pair of events complete
+// fixes the candidate with the lowest time per byte written.  Until then
+// (and under graph capture, and when the table is full) calls take
+// (4, 16).
 namespace {
 constexpr uint64_t kTuneMinBytes = 8ull << 20;
 constexpr int kTuneSlots = 64;
-constexpr int kCand = 4, kRounds = 2;
-constexpr int kCandG[kCand] = {4, 8, 2, 4};
-constexpr int kCandS[kCand] = {16, 16, 32, 32};
+constexpr int kCand = 6, kRounds = 2;
+constexpr int kCandG[kCand] = {4, 8, 2, 4, 1, 1};
+constexpr int kCandS[kCand] = {16, 16, 32, 32, 32, 64};
 struct TuneSlot {
   int cls;
   int choice;    // candidate index, -1 while undecided
@@ -104,9 +109,12 @@ const char *replicate_name(int group, int width) {
   static const char *const n32[] = {"k_replicate<1, 32>", "k_replicate<2, 32>",
                                     "k_replicate<4, 32>", "k_replicate<8, 32>",
                                     "k_replicate<16, 32>"};
+  static const char *const n64[] = {"k_replicate<1, 64>", "k_replicate<2, 64>",
+                                    "k_replicate<4, 64>", "k_replicate<8, 64>",
+                                    "k_replicate<16, 64>"};
   const int i = group == 1 ? 0 : group == 2 ? 1 : group == 4 ? 2
               : group == 16 ? 4 : 3;
-  return width == 32 ? n32[i] : n16[i];
+  return width == 64 ? n64[i] : width == 32 ? n32[i] : n16[i];
 }
 
 int replicate_run(const oscd_geom *g, uint64_t off, uint64_t nbytes,

@@ -75,7 +75,7 @@ int oscd_reduce(const oscd_geom *g, int op, int dtype, int fpmode, int shape,
 /* replicate: member j's [off, off + nbytes) = member 0's, for every j
  * (nbytes a multiple of 16), each loaded piece stored to `group` (1, 2, 4,
  * 8 or 16; -1 picks by size) consecutive members, the pieces interleaved
- * over `width` (16 or 32) workgroups; the second kernel of a large
+ * over `width` (16, 32 or 64) workgroups; the second kernel of a large
  * reduction.  skip (device bytes, one per 4096-byte piece, or NULL):
  * pieces with a nonzero byte are left alone. */
 int oscd_replicate(const oscd_geom *g, uint64_t off, uint64_t nbytes,

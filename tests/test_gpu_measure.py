@@ -119,14 +119,15 @@ def test_kernel_names_of_a_split_call_and_a_small_call(heap):
 
 
 def test_timed_layout_is_decided_without_waiting(heap):
-    """From 8 MiB per member the first eight calls of a size class run the
-    four replicate layouts (members per workgroup, interleave width) twice,
+    """From 8 MiB per member the first twelve calls of a size class run the
+    six replicate layouts (members per workgroup, interleave width) twice,
     each between events; a later call that finds every timed run complete
     fixes the fastest, and later calls keep it."""
     n = (8 << 20) // 8 + 2 * 512  # 8 MiB per member and a bit: timed
     s, d = heap.malloc(n * 8), heap.malloc(n * 8)
     cands = ["k_replicate<4, 16>", "k_replicate<8, 16>",
-             "k_replicate<2, 32>", "k_replicate<4, 32>"]
+             "k_replicate<2, 32>", "k_replicate<4, 32>",
+             "k_replicate<1, 32>", "k_replicate<1, 64>"]
     width = C.c_int(0)
     try:
         heap.fill(None, s, n, "u64", "bits", seed=4)
@@ -138,7 +139,22 @@ def test_timed_layout_is_decided_without_waiting(heap):
                       "reduce")
             return last_kernels(heap)[-1]
         assert heap.L.osc_reduce_split_group(WORLD, n * 8, None) == 0
-        assert [call() for _ in range(8)] == cands * 2
+        src = np.stack([heap.get(s, n, np.uint64, k) for k in range(WORLD)])
+        want = np.bitwise_xor.reduce(src, axis=0)
+        sl = heap.L.osc_slice_bytes()
+        ran = []
+        for _ in range(12):
+            # every layout writes every member in full: poison first
+            for k in range(WORLD):
+                osc.check(heap.L.osc_device_fill64(d + k * sl, n * 8,
+                                                   0xA5A5A5A5A5A5A5A5, st),
+                          "poison")
+            ran.append(call())
+            heap.sync()
+            for k in range(WORLD):
+                assert np.array_equal(heap.get(d, n, np.uint64, k), want), \
+                    (ran[-1], k)
+        assert ran == cands * 2
         assert heap.L.osc_reduce_split_group(WORLD, n * 8, None) == 0
         heap.sync()
         last = call()
@@ -146,13 +162,9 @@ def test_timed_layout_is_decided_without_waiting(heap):
         assert "k_replicate<%d, %d>" % (g, width.value) == last
         assert last in cands
         assert call() == last
-        # the result: every member equals member 0, which is the XOR
         heap.sync()
-        m0 = heap.get(d, n, np.uint64, 0)
-        src = np.stack([heap.get(s, n, np.uint64, k) for k in range(WORLD)])
-        assert np.array_equal(m0, np.bitwise_xor.reduce(src, axis=0))
-        for k in (1, 7, WORLD - 1):
-            assert np.array_equal(heap.get(d, n, np.uint64, k), m0)
+        for k in (0, 1, 7, WORLD - 1):
+            assert np.array_equal(heap.get(d, n, np.uint64, k), want)
     finally:
         heap.free(s)
         heap.free(d)
