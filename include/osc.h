@@ -141,10 +141,10 @@ int osc_set_batch_branches(int n);
  * 8 or 16 bytes) run as a walk that stores member 0's result and a
  * replicate kernel that copies it to the other members, storing each loaded
  * piece to `group` members, the pieces interleaved over `width` workgroups.
- * -1 (default): 8 members and width 16 below 8 MiB per member; from 8 MiB
- * the fastest of (4, 16), (8, 16), (2, 32), (4, 32), (1, 32) and (1, 64)
+ * -1 (default): one member per workgroup, width 32, below 8 MiB per
+ * member; from 8 MiB the fastest of (1, 16), (1, 32), (1, 64) and (2, 32)
  * per byte written, timed between events (without waiting) on the first
- * twelve calls of each power-of-two size class and fixed by a later one.  1, 2, 4, 8 or 16 fix
+ * eight calls of each power-of-two size class and fixed by a later one.  1, 2, 4, 8 or 16 fix
  * the group (width 16); 0 = one kernel that stores to every member.
  * Results are identical either way.  OSC_REDUCE_SPLIT sets it at init. */
 int osc_set_reduce_split(int group);
@@ -354,9 +354,11 @@ int osc_algo_resolve(const char *kind, const char *spec, int *fpmode);
 /* ---- raw device copy (the roofline calibration kernel) ------------------ */
 int osc_device_copy(void *dst, const void *src, size_t bytes, void *stream);
 /* The two halves of a collective's traffic alone, for measuring HBM
- * ceilings: XOR of every 16-byte unit of [p, p + bytes) into the 16 bytes
- * at sink16 (device memory, cleared first); and a fill of [p, p + bytes)
- * with a 64-bit value.  p 16-byte aligned, bytes a multiple of 16. */
+ * ceilings: the XOR of every 16-byte unit of [p, p + bytes) stored to the
+ * 16 bytes at sink16 (device memory); and a fill of [p, p + bytes) with a
+ * 64-bit value.  p 16-byte aligned, bytes a multiple of 16.  The XOR read
+ * keeps per-workgroup partial values in a 32 KiB device buffer allocated
+ * on its first call on a device: calls that overlap on one device race. */
 int osc_device_xor_read(const void *p, size_t bytes, void *sink16,
                         void *stream);
 int osc_device_fill64(void *p, size_t bytes, uint64_t value, void *stream);

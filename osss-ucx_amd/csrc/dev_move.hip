@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/osc.h"
 #include "dev_common.h"
 #include "osc_dev.h"
@@ -38,29 +40,41 @@ constexpr int kMoveBlock = 256;
 // 2 GiB of non-temporal 16-byte stores, each workgroup writing its own
 // contiguous 32 KiB, run at 5.2-5.4 TB/s; S consecutive workgroups that
 // interleave 1 KiB pieces (workgroup x of the group writes pieces x, x+S,
-// x+2S, ... of an S*32 KiB super-chunk) run at 6.3-6.5 TB/s, and
-// destinations that change every 16-32 KiB stay at 5.5-5.8 TB/s whatever
-// the piece size.  Reads gain nothing from the interleave.  A piece is one
-// wave's 64 lanes x 16 bytes; wave w's k-th piece is k*4 + w.  S is a
-// multiple of the 8 XCDs, so workgroup x of every interleave group runs on
-// the same XCD.  (The same order applied to fcollect at 256 members x 1 MiB,
-// 4, 8 or 16 destinations per workgroup, was measured against k_gather at
-// 16 placements of the arena: within +-3% either way, so fcollect keeps
+// x+2S, ... of its super-chunk) run at 6.6-7.0 TB/s (S = 64, 8 pieces per
+// wave, the fastest fill measured; S = 16: 6.5-6.6), and destinations that
+// change every 16-32 KiB stay at 5.5-5.8 TB/s whatever the piece size.
+// Reads gain nothing from the interleave.  A piece is one wave's 64 lanes x
+// 16 bytes; wave w's k-th piece is k*4 + w, so a workgroup covers 256*U
+// units of the super-chunk's S*256*U.  S is a multiple of the 8 XCDs, so
+// workgroup x of every interleave group runs on the same XCD.  (The same
+// order applied to fcollect at 256 members x 1 MiB, 4, 8 or 16
+// destinations per workgroup, was measured against k_gather at 16
+// placements of the arena: within +-3% either way, so fcollect keeps
 // k_gather.  Both move by up to 30% with the arena's physical placement:
 // 9.8-13.0 ms for the same layout and kernel.)
-constexpr int kRepS = 16;                         // workgroups per group
-constexpr uint64_t kRepPiece = 64;                // units (1 KiB) per piece
-constexpr uint64_t kRepWgUnits = kMoveBlock * 8;  // 2048 units per visit
-constexpr uint64_t kRepScUnits = kRepS * kRepWgUnits;  // per super-chunk
-constexpr uint64_t kRepMaxGrid = 1u << 20;        // a multiple of every S
+//
+// Code shape: a bounds test per unit puts every load and store in its own
+// branch, and the compiler then waits for all outstanding memory operations
+// before each store (stores count in the same counter as loads), so a wave
+// has one store in flight.  Visits that lie wholly in range therefore run a
+// branch-free body: all loads issue back to back and each store waits only
+// for its own data.  At 64 members x 32 MiB that took the replicate from
+// 332-368 us to 308-310 us (6.8 TB/s of writes).
+constexpr uint64_t kRepPiece = 64;  // units (1 KiB) per piece
+constexpr int kRepUpt = 16;         // replicate: units per thread per visit
+constexpr uint64_t kRepWgUnits = (uint64_t)kMoveBlock * kRepUpt;  // 64 KiB
+constexpr int kFillS = 64, kFillUpt = 8;  // the write-ceiling fill
+constexpr int kReadUpt = 8;  // the read ceiling: 32 KiB contiguous visits
+constexpr uint64_t kRepMaxGrid = 1u << 20;  // a multiple of every S
 
 // unit of the region that lane l of wave w stores as its k-th unit, for
-// workgroup x of the interleave group (S workgroups) of super-chunk sc
-template <int S = kRepS>
+// workgroup x of the interleave group (S workgroups, U units per thread) of
+// super-chunk sc
+template <int S, int U>
 __device__ __forceinline__ uint64_t il_unit(uint64_t sc, uint64_t x, int k,
                                             uint32_t w, uint32_t l) {
   const uint64_t q = (uint64_t)(k * 4) + w;
-  return sc * (uint64_t)S * kRepWgUnits + (q * S + x) * kRepPiece + l;
+  return sc * (uint64_t)S * (kMoveBlock * U) + (q * S + x) * kRepPiece + l;
 }
 
 // Gather, destination-major: member j's [doff + i*doff_step, +nbytes) =
@@ -482,12 +496,56 @@ extern "C" int oscd_collect(const oscd_geom *g, uint64_t dest_off,
 // fastest, 5.3-5.5 TB/s).
 namespace osc {
 
-template <int G, int S>
+// One visit of k_replicate: the wave's kRepUpt pieces of super-chunk sc,
+// loaded from member 0 and stored to members d0 .. d0+G-1 (those below P).
+// FULL: the super-chunk lies wholly in range (no unit is tested).  skip
+// (or null): one flag per 256 units, set where the walk already stored
+// every member; a piece of 64 units lies in one flag's span.  Lane k
+// fetches the flag of the wave's k-th piece in the same batch as the data
+// loads, and the flags become a wave-uniform mask once the data loads are
+// in flight.
+template <int G, int S, bool FULL, bool SKIP>
+__device__ __forceinline__ void rep_visit(const oscd_geom &g, uint64_t off,
+                                          const char *src, uint64_t nunits,
+                                          uint64_t sc, uint64_t x, int d0,
+                                          uint32_t w, uint32_t l,
+                                          const uint8_t *skip) {
+  // lanes >= kRepUpt repeat the first kRepUpt lanes' pieces
+  uint8_t f = 0;
+  bool fin = true;
+  if (SKIP) {
+    const uint64_t uf = il_unit<S, kRepUpt>(sc, x, (int)(l % kRepUpt), w, 0);
+    fin = FULL || uf < nunits;
+    f = skip[(fin ? uf : nunits - 1) / 256];
+  }
+  uint64_t u[kRepUpt];
+  typename Raw<16>::type v[kRepUpt];
+#pragma unroll
+  for (int k = 0; k < kRepUpt; ++k) {
+    u[k] = il_unit<S, kRepUpt>(sc, x, k, w, l);
+    if (FULL || u[k] < nunits) v[k] = ld<16>(src + u[k] * 16);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  // bit k set: the wave stores its k-th piece
+  const uint32_t keep =
+      SKIP ? (uint32_t)__ballot(l < kRepUpt && fin && f == 0)
+           : (1u << kRepUpt) - 1;
+#pragma unroll
+  for (int q = 0; q < G; ++q) {
+    if (d0 + q >= g.npes) break;  // the last group may be short
+    char *dst = member_ptr(g, d0 + q, off);
+#pragma unroll
+    for (int k = 0; k < kRepUpt; ++k)
+      if (((keep >> k) & 1u) != 0 && (FULL || u[k] < nunits))
+        st_nt<16>(dst + u[k] * 16, v[k]);
+  }
+}
+
+template <int G, int S, bool SKIP>
 __global__ __launch_bounds__(kMoveBlock) void k_replicate(
     const oscd_geom g, const uint64_t off, const uint64_t nunits,
     const uint64_t ngroups, const uint64_t total,
     const uint8_t *__restrict__ skip) {
-  const int P = g.npes;
   const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
   const char *src = member_ptr(g, 0, off);
   // gridDim.x is a multiple of S, so every visit of a workgroup keeps its
@@ -495,33 +553,11 @@ __global__ __launch_bounds__(kMoveBlock) void k_replicate(
   for (uint64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
     const uint64_t x = vb % S, r = vb / S;
     const uint64_t gi = r % ngroups, sc = r / ngroups;
-    uint64_t u[8];
-    typename Raw<16>::type v[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      u[k] = il_unit<S>(sc, x, k, w, l);
-      if (u[k] < nunits) v[k] = ld<16>(src + u[k] * 16);
-    }
-    if (skip != nullptr) {
-      // pieces of 256 units that the walk already stored to every member;
-      // a wave's 64 units lie in one such piece, so the flag is read once
-      // per wave, after the data loads are in flight
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const uint64_t pc = (uint64_t)__builtin_amdgcn_readfirstlane(
-            (uint32_t)(il_unit<S>(sc, x, k, w, 0) / 256));
-        if (u[k] < nunits && skip[pc] != 0) u[k] = nunits;
-      }
-    }
     const int d0 = 1 + (int)gi * G;  // member 0 already holds the result
-#pragma unroll
-    for (int q = 0; q < G; ++q) {
-      if (d0 + q >= P) break;  // the last group may be short
-      char *dst = member_ptr(g, d0 + q, off);
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (u[k] < nunits) st_nt<16>(dst + u[k] * 16, v[k]);
-    }
+    if ((sc + 1) * (uint64_t)S * kRepWgUnits <= nunits)
+      rep_visit<G, S, true, SKIP>(g, off, src, nunits, sc, x, d0, w, l, skip);
+    else
+      rep_visit<G, S, false, SKIP>(g, off, src, nunits, sc, x, d0, w, l, skip);
   }
 }
 
@@ -534,18 +570,25 @@ static int replicate_g(const oscd_geom *g, uint64_t off, uint64_t nbytes,
   const uint64_t ngroups = ((uint64_t)g->npes - 1 + G - 1) / G;
   const uint64_t total = nsc * ngroups * S;
   const uint64_t grid = total < kRepMaxGrid ? total : kRepMaxGrid;
-  hipLaunchKernelGGL((k_replicate<G, S>), dim3((unsigned)grid),
-                     dim3(kMoveBlock), 0, s, *g, off, nunits, ngroups, total,
-                     skip);
+  if (skip != nullptr)
+    hipLaunchKernelGGL((k_replicate<G, S, true>), dim3((unsigned)grid),
+                       dim3(kMoveBlock), 0, s, *g, off, nunits, ngroups,
+                       total, skip);
+  else
+    hipLaunchKernelGGL((k_replicate<G, S, false>), dim3((unsigned)grid),
+                       dim3(kMoveBlock), 0, s, *g, off, nunits, ngroups,
+                       total, skip);
   return check_launch();
 }
 
-// Members per workgroup for group -1: 8 below 8 MiB per member (at 4 MiB
-// 8 is ahead: 47 vs 50-52 us), else 4.  From 8 MiB the reduce path picks
-// between 4 and 8 by measurement instead (replicate_tuned): the faster one
-// differs between boxes.
+// Members per workgroup for group -1: one.  64 members, width 32, per
+// member 1 / 2 / 4 / 8 / 16 / 32 MiB: 13.1 / 22.7 / 43.0 / 83.5 / 162.5 /
+// 320.5 us, against 13.0 / 23.2 / 47.3 / 90.8 / 181.3 / 359.5 us with 8
+// members per workgroup.  From 8 MiB the reduce path picks the layout by
+// measurement instead (replicate_tuned).
 static int replicate_auto_group(uint64_t nbytes) {
-  return nbytes >= (8ull << 20) ? 4 : 8;
+  (void)nbytes;
+  return 1;
 }
 
 }  // namespace osc
@@ -760,27 +803,40 @@ extern "C" int oscd_alltoall(const oscd_geom *g, uint64_t dest_off,
 // ---- read-only and write-only streams -----------------------------------------
 // The two halves of every collective's traffic, alone, for measuring the
 // HBM ceilings beside the collectives.  k_xor_read folds every 16-byte unit
-// of a range into one value (each workgroup reads one contiguous 32 KiB
-// visit, the walk's read pattern, with non-temporal loads) and XORs its
-// result into a 16-byte sink with device atomics.  k_fill64 stores a 64-bit
-// value over a range in the replicate's interleaved write order.
+// of a range into one value per workgroup (each workgroup reads contiguous
+// 32 KiB visits, the walk's read pattern, with non-temporal loads) and
+// k_xor_fold folds those into a 16-byte sink.  (Device atomics into the
+// sink instead cost 50 us: the 2048 workgroups finish together and their
+// atomics on one address queue up, 353-358 us against 302-306 us.)  k_fill64 stores a 64-bit
+// value over a range in the interleaved write order (64 workgroups per
+// group, 8 pieces per wave).  Both are the fastest shapes measured on
+// MI355X for 2 GiB: reads 300-303 us (7.1 TB/s; 16 units per thread 310 us,
+// one load in flight per wave 308-312 us), writes 304-310 us.
 namespace osc {
-
-__global__ __launch_bounds__(kMoveBlock) void k_sink_clear(
-    unsigned long long *sink) {
-  if (threadIdx.x < 2) sink[threadIdx.x] = 0ull;
-}
 
 __global__ __launch_bounds__(kMoveBlock) void k_xor_read(
     const char *__restrict__ p, const uint64_t nunits,
-    unsigned long long *sink) {
+    unsigned long long *__restrict__ part) {
   __shared__ unsigned long long red[2][kMoveBlock / 64];
+  constexpr uint64_t visit = (uint64_t)kMoveBlock * kReadUpt;
   typename Raw<16>::type a = {0u, 0u, 0u, 0u};
-  for (uint64_t c = blockIdx.x; c * kRepWgUnits < nunits; c += gridDim.x) {
+  for (uint64_t c = blockIdx.x; c * visit < nunits; c += gridDim.x) {
+    const uint64_t u0 = c * visit + threadIdx.x;
+    if ((c + 1) * visit <= nunits) {
+      // a whole visit: every load in flight before the first XOR waits
+      typename Raw<16>::type t[kReadUpt];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const uint64_t u = c * kRepWgUnits + (uint64_t)k * kMoveBlock + threadIdx.x;
-      if (u < nunits) a ^= ld_nt<16>(p + u * 16);
+      for (int k = 0; k < kReadUpt; ++k)
+        t[k] = ld_nt<16>(p + (u0 + (uint64_t)k * kMoveBlock) * 16);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < kReadUpt; ++k) a ^= t[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < kReadUpt; ++k) {
+        const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
+        if (u < nunits) a ^= ld_nt<16>(p + u * 16);
+      }
     }
   }
   unsigned long long lo = ((unsigned long long)a.y << 32) | a.x;
@@ -797,13 +853,40 @@ __global__ __launch_bounds__(kMoveBlock) void k_xor_read(
   if (threadIdx.x < 2) {
     unsigned long long v = 0;
     for (int w = 0; w < kMoveBlock / 64; ++w) v ^= red[threadIdx.x][w];
-    atomicXor(&sink[threadIdx.x], v);
+    part[2 * blockIdx.x + threadIdx.x] = v;
+  }
+}
+
+// XOR of the nblk workgroups' partial values into the 16-byte sink.
+__global__ __launch_bounds__(kMoveBlock) void k_xor_fold(
+    const unsigned long long *__restrict__ part, const uint32_t nblk,
+    unsigned long long *sink) {
+  __shared__ unsigned long long red[2][kMoveBlock / 64];
+  unsigned long long lo = 0, hi = 0;
+  for (uint32_t b = threadIdx.x; b < nblk; b += kMoveBlock) {
+    lo ^= part[2 * b];
+    hi ^= part[2 * b + 1];
+  }
+  for (int d = 32; d > 0; d >>= 1) {
+    lo ^= __shfl_xor(lo, d);
+    hi ^= __shfl_xor(hi, d);
+  }
+  if (threadIdx.x % 64 == 0) {
+    red[0][threadIdx.x / 64] = lo;
+    red[1][threadIdx.x / 64] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    unsigned long long v = 0;
+    for (int w = 0; w < kMoveBlock / 64; ++w) v ^= red[threadIdx.x][w];
+    sink[threadIdx.x] = v;
   }
 }
 
 __global__ __launch_bounds__(kMoveBlock) void k_fill64(
     char *__restrict__ p, const uint64_t nunits, const uint64_t value,
     const uint64_t total) {
+  constexpr uint64_t sc_units = (uint64_t)kFillS * kMoveBlock * kFillUpt;
   const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
   typename Raw<16>::type v;
   v.x = (uint32_t)value;
@@ -811,15 +894,43 @@ __global__ __launch_bounds__(kMoveBlock) void k_fill64(
   v.z = v.x;
   v.w = v.y;
   for (uint64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
-    const uint64_t x = vb % kRepS, sc = vb / kRepS;
+    const uint64_t x = vb % kFillS, sc = vb / kFillS;
+    const bool full = (sc + 1) * sc_units <= nunits;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const uint64_t u = il_unit<kRepS>(sc, x, k, w, l);
-      if (u < nunits) st_nt<16>(p + u * 16, v);
+    for (int k = 0; k < kFillUpt; ++k) {
+      const uint64_t u = il_unit<kFillS, kFillUpt>(sc, x, k, w, l);
+      if (full || u < nunits) st_nt<16>(p + u * 16, v);
     }
   }
 }
 
+}  // namespace osc
+
+// Per-device buffer of the read ceiling's per-workgroup partial values,
+// allocated on first use and kept until the process ends.  Calls that
+// overlap on one device (on different streams) share it and race.
+namespace osc {
+constexpr unsigned kReadGrid = 2048;
+static std::mutex g_part_mu;
+static unsigned long long *g_part[kMaxDevices];
+static unsigned long long *read_partials() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_part_mu);
+  if (g_part[dev] == nullptr) {
+    void *p = nullptr;
+    if (hipMalloc(&p, 2 * sizeof(unsigned long long) * kReadGrid) !=
+        hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    g_part[dev] = (unsigned long long *)p;
+  }
+  return g_part[dev];
+}
 }  // namespace osc
 
 extern "C" int oscd_xor_read(const void *p, uint64_t bytes, void *sink16,
@@ -829,16 +940,19 @@ extern "C" int oscd_xor_read(const void *p, uint64_t bytes, void *sink16,
       (uintptr_t)p % 16 != 0 || (uintptr_t)sink16 % 8 != 0)
     return OSC_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_sink_clear, dim3(1), dim3(kMoveBlock), 0, s,
-                     (unsigned long long *)sink16);
-  if (bytes == 0) return check_launch();
+  unsigned long long *part = read_partials();
+  if (part == nullptr) return OSC_ERR_NOMEM;
   const uint64_t nunits = bytes / 16;
-  // 2048 workgroups (8 per CU) loop over the 32 KiB visits: two sink
-  // atomics per workgroup, not per visit
-  const uint64_t visits = (nunits + kRepWgUnits - 1) / kRepWgUnits;
-  const unsigned grid = (unsigned)(visits < 2048 ? visits : 2048);
-  hipLaunchKernelGGL(k_xor_read, dim3(grid), dim3(kMoveBlock), 0, s,
-                     (const char *)p, nunits, (unsigned long long *)sink16);
+  // 2048 workgroups (8 per CU) loop over the 32 KiB visits
+  const uint64_t visit = (uint64_t)kMoveBlock * kReadUpt;
+  const uint64_t visits = (nunits + visit - 1) / visit;
+  const unsigned grid = (unsigned)(visits < kReadGrid ? visits : kReadGrid);
+  if (grid > 0)
+    hipLaunchKernelGGL(k_xor_read, dim3(grid), dim3(kMoveBlock), 0, s,
+                       (const char *)p, nunits, part);
+  hipLaunchKernelGGL(k_xor_fold, dim3(1), dim3(kMoveBlock), 0, s,
+                     (const unsigned long long *)part, grid,
+                     (unsigned long long *)sink16);
   return check_launch();
 }
 
@@ -849,7 +963,8 @@ extern "C" int oscd_fill64(void *p, uint64_t bytes, uint64_t value,
     return OSC_ERR_ARG;
   if (bytes == 0) return OSC_OK;
   const uint64_t nunits = bytes / 16;
-  const uint64_t total = (nunits + kRepScUnits - 1) / kRepScUnits * kRepS;
+  constexpr uint64_t sc_units = (uint64_t)kFillS * kMoveBlock * kFillUpt;
+  const uint64_t total = (nunits + sc_units - 1) / sc_units * kFillS;
   const uint64_t grid = total < kRepMaxGrid ? total : kRepMaxGrid;
   hipLaunchKernelGGL(k_fill64, dim3((unsigned)grid), dim3(kMoveBlock), 0,
                      (hipStream_t)stream, (char *)p, nunits, value, total);

@@ -231,8 +231,11 @@ struct PolF64Acc {
   static constexpr bool TREE = false;
   // The accurate sum keeps 2 doubles per element: at 8x2 it takes 180
   // VGPRs (2 waves per SIMD).  8x1 held to 4 waves per SIMD measured 3%
-  // faster at 64 PEs x 4M.
+  // faster at 64 PEs x 4M when the walk stored every member; the walk that
+  // stores member 0 only (split calls) is 1.5% faster at 8x2 (663 vs 674
+  // us per call).
   static constexpr int SHAPE = OP == OSC_OP_SUM ? 1 : 0;  // default shape
+  static constexpr int SHAPE_ONE = 0;  // default shape of a split call
   static constexpr int WAVES81 = OP == OSC_OP_SUM ? 4 : 0;
   static constexpr bool SPECIAL = false;
   static constexpr bool BINOM = false;
@@ -1285,6 +1288,17 @@ uint8_t *span_flags(hipStream_t s, uint64_t npieces);
 int replicate_tuned(const oscd_geom *g, uint64_t off, uint64_t nbytes,
                     const uint8_t *skip, hipStream_t s);
 
+// Default shape of the walk of a split call: Pol::SHAPE_ONE where the
+// policy names one, else Pol::SHAPE.
+template <class Pol, class = void>
+struct shape_one {
+  static constexpr int v = Pol::SHAPE;
+};
+template <class Pol>
+struct shape_one<Pol, std::void_t<decltype(Pol::SHAPE_ONE)>> {
+  static constexpr int v = Pol::SHAPE_ONE;
+};
+
 // Default shape per policy (Pol::SHAPE), from the 64-PE x 4M sweeps on
 // MI355X: 8x2 for the order-free kernels, 8x1 for FP32 accumulated in FP64
 // (its FP64 accumulators make 8x2 too register-heavy), 2x4 for the
@@ -1315,6 +1329,7 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
   // grid within 2^23 workgroups (the reference itself takes nreduce < 2^31)
   if ((n + (vec_ok ? VB : 1) - 1) / (vec_ok ? VB : 1) > (1ull << 31))
     return OSC_ERR_ARG;
+  const bool chosen = shape < 0;  // the shape is the library's to pick
   if (shape < 0) {
     if constexpr (lanes_ok<T, Pol>()) {
       if (lanes_pays<T>(g, doff, soff, n))
@@ -1332,6 +1347,8 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
   }
   shape = clamp_shape<Pol>(shape);
   if (vec_ok && split_pays<T, Pol>(g, n)) {
+    if (chosen && shape == default_shape<Pol>())
+      shape = clamp_shape<Pol>(shape_one<Pol>::v);
     uint8_t *fl = nullptr;
     if constexpr (Pol::SPECIAL)
       fl = span_flags(s, (n * sizeof(T) / 16 + kBlock - 1) / kBlock);

@@ -65,29 +65,28 @@ uint8_t *span_flags(hipStream_t s, uint64_t npieces) {
 }
 // Replicate layout by measurement.  The fastest replicate layout -- members
 // per workgroup G and workgroups per interleave group S -- differs between
-// MI355X boxes: at 64 members x 32 MiB, 232 MiB apart, one box took
-// 321-331 us with (G, S) = (4, 16), 345-355 with (8, 16) and 389-391 with
-// (4, 32); another took 370-374 us with (4, 16), 356-358 with (4, 32) and
-// 348-351 with (2, 32); a third took 368 us with (4, 16), 346 with
-// (2, 32), 338 with (1, 64) and 332 with (1, 32) against 328 us for one
-// plain 2 GiB fill.  (With one member per workgroup, the 63 workgroup
-// groups of a super-chunk run together and workgroup x of each runs on the
-// same XCD, so the 63 reads of a piece after the first come from that
-// XCD's L2.)  Within one box the order held across 16 placements of the
-// arena, and the same candidate won at 16 and 32 MiB per member.  So from
-// 8 MiB per member, each power-of-two size class measures six candidates
-// on its first calls, without ever waiting: each of its first twelve calls
-// (whatever their team size) runs the next candidate between two events
-// (two rounds), and a later call that finds every pair of events complete
-// fixes the candidate with the lowest time per byte written.  Until then
-// (and under graph capture, and when the table is full) calls take
-// (4, 16).
+// MI355X boxes.  With the branch-free visit body, one member per workgroup
+// leads everywhere measured: at 64 members x 32 MiB (232 MiB apart) one box
+// took 316 / 321 / 337 us with (1, 16) / (1, 32) / (1, 64), 347 with
+// (2, 32) and 352-375 with 4 or 8 members; another took 308-310 us with
+// (1, 32) and 313-315 with (1, 16) and (1, 64).  (With one member per
+// workgroup the 63 groups of a super-chunk run together and workgroup x of
+// each runs on the same XCD, so the 63 reads of a piece after the first
+// come from that XCD's L2.)  So from 8 MiB per member, each power-of-two
+// size class measures four candidates on its first calls, without ever
+// waiting: each of its first eight calls (whatever their team size) runs
+// the next candidate between two events (two rounds), and a later call that
+// finds every pair of events complete fixes the candidate with the lowest
+// time per byte written.  Below 8 MiB, until then, under graph capture and
+// when the table is full, calls take (1, 32).
 namespace {
 constexpr uint64_t kTuneMinBytes = 8ull << 20;
 constexpr int kTuneSlots = 64;
-constexpr int kCand = 6, kRounds = 2;
-constexpr int kCandG[kCand] = {4, 8, 2, 4, 1, 1};
-constexpr int kCandS[kCand] = {16, 16, 32, 32, 32, 64};
+constexpr int kCand = 4, kRounds = 2;
+constexpr int kCandG[kCand] = {1, 1, 1, 2};
+constexpr int kCandS[kCand] = {16, 32, 64, 32};
+// below kTuneMinBytes, under graph capture and while undecided
+constexpr int kFixedG = 1, kFixedS = 32;
 struct TuneSlot {
   int cls;
   int choice;    // candidate index, -1 while undecided
@@ -102,25 +101,31 @@ int g_tune_n = 0;
 int size_class(uint64_t nbytes) { return 63 - __builtin_clzll(nbytes); }
 }  // namespace
 
-const char *replicate_name(int group, int width) {
-  static const char *const n16[] = {"k_replicate<1, 16>", "k_replicate<2, 16>",
-                                    "k_replicate<4, 16>", "k_replicate<8, 16>",
-                                    "k_replicate<16, 16>"};
-  static const char *const n32[] = {"k_replicate<1, 32>", "k_replicate<2, 32>",
-                                    "k_replicate<4, 32>", "k_replicate<8, 32>",
-                                    "k_replicate<16, 32>"};
-  static const char *const n64[] = {"k_replicate<1, 64>", "k_replicate<2, 64>",
-                                    "k_replicate<4, 64>", "k_replicate<8, 64>",
-                                    "k_replicate<16, 64>"};
+// The replicate kernel's name as rocprof prints it: k_replicate<G, S, SKIP>.
+const char *replicate_name(int group, int width, bool skip) {
+#define OSC_REP_NAMES(S, B)                                         \
+  {"k_replicate<1, " #S ", " #B ">", "k_replicate<2, " #S ", " #B ">", \
+   "k_replicate<4, " #S ", " #B ">", "k_replicate<8, " #S ", " #B ">", \
+   "k_replicate<16, " #S ", " #B ">"}
+  static const char *const names[2][3][5] = {
+      {OSC_REP_NAMES(16, false), OSC_REP_NAMES(32, false),
+       OSC_REP_NAMES(64, false)},
+      {OSC_REP_NAMES(16, true), OSC_REP_NAMES(32, true),
+       OSC_REP_NAMES(64, true)}};
+#undef OSC_REP_NAMES
   const int i = group == 1 ? 0 : group == 2 ? 1 : group == 4 ? 2
               : group == 16 ? 4 : 3;
-  return width == 64 ? n64[i] : width == 32 ? n32[i] : n16[i];
+  const int j = width == 64 ? 2 : width == 32 ? 1 : 0;
+  return names[skip ? 1 : 0][j][i];
 }
 
 int replicate_run(const oscd_geom *g, uint64_t off, uint64_t nbytes,
                   int group, int width, const uint8_t *skip, hipStream_t s) {
-  if (group < 0) group = nbytes >= kTuneMinBytes ? 4 : 8;
-  note_launch(replicate_name(group, width), nullptr);
+  if (group < 0) {
+    group = kFixedG;
+    width = kFixedS;
+  }
+  note_launch(replicate_name(group, width, skip != nullptr), nullptr);
   return oscd_replicate(g, off, nbytes, group, width, skip, s);
 }
 
@@ -139,9 +144,9 @@ int replicate_tuned(const oscd_geom *g, uint64_t off, uint64_t nbytes,
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &st) != hipSuccess ||
       st != hipStreamCaptureStatusNone)
-    return replicate_run(g, off, nbytes, 4, 16, skip, s);
+    return replicate_run(g, off, nbytes, kFixedG, kFixedS, skip, s);
   if (t == nullptr) {
-    if (g_tune_n >= kTuneSlots) return replicate_run(g, off, nbytes, 4, 16, skip, s);
+    if (g_tune_n >= kTuneSlots) return replicate_run(g, off, nbytes, kFixedG, kFixedS, skip, s);
     TuneSlot nt{};
     nt.cls = cls;
     nt.choice = -1;
@@ -149,7 +154,7 @@ int replicate_tuned(const oscd_geom *g, uint64_t off, uint64_t nbytes,
       if (hipEventCreate(&nt.ev[e]) != hipSuccess) {
         for (int f = 0; f < e; ++f) (void)hipEventDestroy(nt.ev[f]);
         (void)hipGetLastError();
-        return replicate_run(g, off, nbytes, 4, 16, skip, s);
+        return replicate_run(g, off, nbytes, kFixedG, kFixedS, skip, s);
       }
     g_tune[g_tune_n] = nt;
     t = &g_tune[g_tune_n++];
@@ -193,13 +198,15 @@ int replicate_tuned(const oscd_geom *g, uint64_t off, uint64_t nbytes,
     (void)hipGetLastError();
     return replicate_run(g, off, nbytes, kCandG[bc], kCandS[bc], skip, s);
   }
-  return replicate_run(g, off, nbytes, 4, 16, skip, s);
+  return replicate_run(g, off, nbytes, kFixedG, kFixedS, skip, s);
 }
 
 int replicate_choice(int npes, uint64_t nbytes, int *width) {
   if (width != nullptr) *width = 16;
   if (g_split_group > 0) return g_split_group;
-  if (nbytes < kTuneMinBytes) return 8;
+  if (width != nullptr) *width = kFixedS;
+  if (nbytes < kTuneMinBytes) return kFixedG;
+  if (width != nullptr) *width = 16;
   const int cls = size_class(nbytes);
   std::lock_guard<std::mutex> lk(g_tune_mu);
   (void)npes;  // the size class alone keys the measurement

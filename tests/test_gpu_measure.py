@@ -96,9 +96,19 @@ def test_kernel_names_of_a_split_call_and_a_small_call(heap):
         assert names[0].startswith("k_reduce<unsigned long, osc::PolSeq<"
                                    "unsigned long, 5, false>"), names
         assert names[0].endswith(", true>")  # stores member 0 only
-        # below 8 MiB per member: 8 members per workgroup, width 16
-        assert names[1] == "k_replicate<8, 16>", names
-        assert heap.L.osc_reduce_split_group(WORLD, n * 8, None) == 8
+        # below 8 MiB per member: one member per workgroup, width 32
+        assert names[1] == "k_replicate<1, 32, false>", names
+        width = C.c_int(0)
+        assert heap.L.osc_reduce_split_group(WORLD, n * 8,
+                                             C.byref(width)) == 1
+        assert width.value == 32
+        # float MIN/MAX: the replicate skips the spans the walk stored
+        osc.check(heap.L.osc_reduce(heap.world, osc.OPS["min"],
+                                    osc.DTYPES["f64"], d, s, n, st), "reduce")
+        names = last_kernels(heap)
+        assert names[0].startswith("k_reduce<double, osc::PolSeq<double, 3, "
+                                   "true>"), names
+        assert names[1] == "k_replicate<1, 32, true>", names
         # one kernel when the split is off
         osc.check(heap.L.osc_set_reduce_split(0), "split off")
         osc.check(heap.L.osc_reduce(heap.world, osc.OPS["sum"],
@@ -119,15 +129,14 @@ def test_kernel_names_of_a_split_call_and_a_small_call(heap):
 
 
 def test_timed_layout_is_decided_without_waiting(heap):
-    """From 8 MiB per member the first twelve calls of a size class run the
-    six replicate layouts (members per workgroup, interleave width) twice,
+    """From 8 MiB per member the first eight calls of a size class run the
+    four replicate layouts (members per workgroup, interleave width) twice,
     each between events; a later call that finds every timed run complete
     fixes the fastest, and later calls keep it."""
     n = (8 << 20) // 8 + 2 * 512  # 8 MiB per member and a bit: timed
     s, d = heap.malloc(n * 8), heap.malloc(n * 8)
-    cands = ["k_replicate<4, 16>", "k_replicate<8, 16>",
-             "k_replicate<2, 32>", "k_replicate<4, 32>",
-             "k_replicate<1, 32>", "k_replicate<1, 64>"]
+    cands = ["k_replicate<1, 16, false>", "k_replicate<1, 32, false>",
+             "k_replicate<1, 64, false>", "k_replicate<2, 32, false>"]
     width = C.c_int(0)
     try:
         heap.fill(None, s, n, "u64", "bits", seed=4)
@@ -143,7 +152,7 @@ def test_timed_layout_is_decided_without_waiting(heap):
         want = np.bitwise_xor.reduce(src, axis=0)
         sl = heap.L.osc_slice_bytes()
         ran = []
-        for _ in range(12):
+        for _ in range(8):
             # every layout writes every member in full: poison first
             for k in range(WORLD):
                 osc.check(heap.L.osc_device_fill64(d + k * sl, n * 8,
@@ -159,7 +168,7 @@ def test_timed_layout_is_decided_without_waiting(heap):
         heap.sync()
         last = call()
         g = heap.L.osc_reduce_split_group(WORLD, n * 8, C.byref(width))
-        assert "k_replicate<%d, %d>" % (g, width.value) == last
+        assert "k_replicate<%d, %d, false>" % (g, width.value) == last
         assert last in cands
         assert call() == last
         heap.sync()
