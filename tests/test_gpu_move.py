@@ -45,6 +45,28 @@ def test_fcollect(heap, nb):
         assert np.array_equal(got, O.fcollect(src)), P
 
 
+@pytest.mark.parametrize("nb", [(512 << 10) - 16, 512 << 10, (512 << 10) + 16,
+                                (1 << 20) + 48, 3 << 19])
+def test_fcollect_interleaved_blocks(heap, nb):
+    """Blocks of 512 KiB and more (16-byte units) take the interleaved
+    gather: whole super-chunks without bounds tests, then the ragged rest."""
+    rng = np.random.default_rng(nb)
+    for P in (2, 5, 13):
+        src = rng.integers(0, 256, (P, nb), dtype=np.uint8)
+        got = move(heap, "fcollect", src, nb, team=team_of(heap, P))
+        assert np.array_equal(got, O.fcollect(src)), P
+
+
+@pytest.mark.parametrize("root", [0, 6])
+@pytest.mark.parametrize("nb", [512 << 10, (1 << 20) + 16])
+def test_broadcast_interleaved_blocks(heap, root, nb):
+    rng = np.random.default_rng(root + nb)
+    P = 9
+    src = rng.integers(0, 256, (P, nb), dtype=np.uint8)
+    got = move(heap, "broadcast", src, nb, team=team_of(heap, P), root=root)
+    assert np.array_equal(got, O.broadcast(src, root))
+
+
 @pytest.mark.parametrize("nb", SIZES)
 def test_alltoall(heap, nb):
     rng = np.random.default_rng(1000 + nb)
