@@ -77,6 +77,26 @@ __device__ __forceinline__ uint64_t il_unit(uint64_t sc, uint64_t x, int k,
   return sc * (uint64_t)S * (kMoveBlock * U) + (q * S + x) * kRepPiece + l;
 }
 
+// Copy of UPT units per thread, units u0 + k*256: FULL (the whole chunk in
+// range) takes no per-unit test, so every load is in flight before the
+// first store; otherwise each unit is tested, and the compiler then waits
+// for all outstanding memory operations before each store.
+template <int W, int UPT, bool FULL>
+__device__ __forceinline__ void move_chunk(const char *src, char *dst,
+                                           uint64_t u0, uint64_t nunits) {
+  typename Raw<W>::type v[UPT];
+#pragma unroll
+  for (int k = 0; k < UPT; ++k) {
+    const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
+    if (FULL || u < nunits) v[k] = ld<W>(src + u * W);
+  }
+#pragma unroll
+  for (int k = 0; k < UPT; ++k) {
+    const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
+    if (FULL || u < nunits) st_nt<W>(dst + u * W, v[k]);
+  }
+}
+
 // Gather, destination-major: member j's [doff + i*doff_step, +nbytes) =
 // source member i's [soff, +nbytes), for every source i (fcollect,
 // src_member < 0) or for i = src_member only (broadcast, doff_step = 0).
@@ -101,20 +121,10 @@ __global__ __launch_bounds__(kMoveBlock) void k_gather(
     const char *src = member_ptr(g, i, soff);
     char *dst = member_ptr(g, j, doff + (uint64_t)i * doff_step);
     const uint64_t u0 = cc * (uint64_t)(kMoveBlock * UPT) + threadIdx.x;
-    // a chunk wholly in range takes no per-unit test (which would make each
-    // store wait for every outstanding load)
-    const bool full = (cc + 1) * (uint64_t)(kMoveBlock * UPT) <= nunits;
-    typename Raw<W>::type v[UPT];
-#pragma unroll
-    for (int k = 0; k < UPT; ++k) {
-      const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
-      if (full || u < nunits) v[k] = ld<W>(src + u * W);
-    }
-#pragma unroll
-    for (int k = 0; k < UPT; ++k) {
-      const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
-      if (full || u < nunits) st_nt<W>(dst + u * W, v[k]);
-    }
+    if ((cc + 1) * (uint64_t)(kMoveBlock * UPT) <= nunits)
+      move_chunk<W, UPT, true>(src, dst, u0, nunits);
+    else
+      move_chunk<W, UPT, false>(src, dst, u0, nunits);
   }
 }
 
@@ -172,17 +182,10 @@ __global__ __launch_bounds__(kMoveBlock) void k_collect(
     const char *src = member_ptr(g, i, soff);
     char *dst = member_ptr(g, j, doff + tab[2 * i]);
     const uint64_t u0 = c0 + threadIdx.x;
-    typename Raw<W>::type v[UPT];
-#pragma unroll
-    for (int k = 0; k < UPT; ++k) {
-      const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
-      if (u < nunits) v[k] = ld<W>(src + u * W);
-    }
-#pragma unroll
-    for (int k = 0; k < UPT; ++k) {
-      const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
-      if (u < nunits) st_nt<W>(dst + u * W, v[k]);
-    }
+    if (c0 + (uint64_t)(kMoveBlock * UPT) <= nunits)
+      move_chunk<W, UPT, true>(src, dst, u0, nunits);
+    else
+      move_chunk<W, UPT, false>(src, dst, u0, nunits);
   }
 }
 
@@ -260,17 +263,10 @@ __global__ __launch_bounds__(kMoveBlock) void k_a2a_big(
   const char *src = member_ptr(g, i, soff) + (uint64_t)j * nbytes;
   char *dst = member_ptr(g, j, doff) + (uint64_t)i * nbytes;
   const uint64_t u0 = (uint64_t)chunk * (kMoveBlock * UPT) + threadIdx.x;
-  typename Raw<W>::type v[UPT];
-#pragma unroll
-  for (int k = 0; k < UPT; ++k) {
-    const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
-    if (u < nunits) v[k] = ld<W>(src + u * W);
-  }
-#pragma unroll
-  for (int k = 0; k < UPT; ++k) {
-    const uint64_t u = u0 + (uint64_t)k * kMoveBlock;
-    if (u < nunits) st_nt<W>(dst + u * W, v[k]);
-  }
+  if ((uint64_t)(chunk + 1) * (kMoveBlock * UPT) <= nunits)
+    move_chunk<W, UPT, true>(src, dst, u0, nunits);
+  else
+    move_chunk<W, UPT, false>(src, dst, u0, nunits);
 }
 
 // Alltoall, small blocks (nbytes <= 64): a 16x16 tile of (source i,
