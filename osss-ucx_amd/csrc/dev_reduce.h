@@ -404,6 +404,28 @@ __device__ __forceinline__ cf64 pick(bool c, const cf64 &a, const cf64 &b) {
   r.im = c ? a.im : b.im;
   return r;
 }
+// the accumulators' own component-wise selects: a select of the whole
+// struct kept the FP64 walk's accumulators in scratch memory
+__device__ __forceinline__ dd pick(bool c, const dd &a, const dd &b) {
+  dd r;
+  r.hi = c ? a.hi : b.hi;
+  r.lo = c ? a.lo : b.lo;
+  return r;
+}
+__device__ __forceinline__ c64acc pick(bool c, const c64acc &a,
+                                       const c64acc &b) {
+  c64acc r;
+  r.re = c ? a.re : b.re;
+  r.im = c ? a.im : b.im;
+  return r;
+}
+__device__ __forceinline__ cddacc pick(bool c, const cddacc &a,
+                                       const cddacc &b) {
+  cddacc r;
+  r.re = pick(c, a.re, b.re);
+  r.im = pick(c, a.im, b.im);
+  return r;
+}
 
 // Team index of leaf k's member: ceil(k*P / 2^lg).
 __device__ __forceinline__ int member_of(int k, int P, int lg) {
@@ -577,31 +599,25 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
   if (any) {
     Acc acc[UPT][VEC];
     if constexpr (!Pol::TREE) {
-      // Slice 0 seeds the accumulator; then groups of INFL slices whose
-      // loads are unconditional (so the compiler's wait counts stay exact
-      // and the next group's loads are not held back by a drained queue),
-      // then the < INFL leftover slices one at a time.
+      // Slices in batches of INFL whose loads are unconditional (so the
+      // compiler's wait counts stay exact and the next batch's loads are
+      // not held back by a drained queue).  The first batch holds slice 0,
+      // which seeds the accumulator; the last holds the P % INFL leftover
+      // slices.  In those two an index past P-1 re-reads slice P-1 and its
+      // value is selected away, so every load still issues unconditionally
+      // and a call makes ceil(P / INFL) dependent round trips: seeding from
+      // slice 0 alone and taking the leftover slices one at a time made a
+      // 1x8 call on 8 members take 8 round trips instead of one.
       uint32_t nanm = 0;
-      {
-        T buf[UPT][VEC];
-#pragma unroll
-        for (int k = 0; k < UPT; ++k)
-          if (FAST || cnt[k] > 0)
-            load_unit<T, VEC>(s0 + loff[k], buf[k], FAST || full[k], cnt[k]);
-#pragma unroll
-        for (int k = 0; k < UPT; ++k)
-#pragma unroll
-          for (int v = 0; v < VEC; ++v) {
-            if (Pol::SPECIAL && is_nan(buf[k][v])) nanm |= 1u << (k * VEC + v);
-            acc[k][v] = Pol::lift(buf[k][v]);
-          }
-      }
-      int i = 1;
-      for (; i + INFL <= P; i += INFL) {
+      // slices i .. i+INFL-1, those below P combined in order; FIRST: slice
+      // i (= 0) seeds the accumulator
+      auto batch = [&](const int i, const bool first, const bool tail)
+                       __attribute__((always_inline)) {
         T buf[INFL][UPT][VEC];
 #pragma unroll
         for (int j = 0; j < INFL; ++j) {
-          const char *sp = s0 + (i + j) * step;
+          const int m = (!tail || i + j < P) ? i + j : P - 1;
+          const char *sp = s0 + m * step;
 #pragma unroll
           for (int k = 0; k < UPT; ++k)
             if (FAST || cnt[k] > 0)
@@ -609,31 +625,27 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
                                 cnt[k]);
         }
 #pragma unroll
-        for (int j = 0; j < INFL; ++j)
+        for (int j = 0; j < INFL; ++j) {
+          const bool in = !tail || i + j < P;
 #pragma unroll
           for (int k = 0; k < UPT; ++k)
 #pragma unroll
             for (int v = 0; v < VEC; ++v) {
-              if (Pol::SPECIAL && is_nan(buf[j][k][v]))
+              if (Pol::SPECIAL && in && is_nan(buf[j][k][v]))
                 nanm |= 1u << (k * VEC + v);
-              acc[k][v] = Pol::seq(acc[k][v], buf[j][k][v]);
+              if (first && j == 0) {
+                acc[k][v] = Pol::lift(buf[j][k][v]);
+              } else {
+                acc[k][v] = pick(in, Pol::seq(acc[k][v], buf[j][k][v]),
+                                 acc[k][v]);
+              }
             }
-      }
-      for (; i < P; ++i) {
-        T buf[UPT][VEC];
-        const char *sp = s0 + i * step;
-#pragma unroll
-        for (int k = 0; k < UPT; ++k)
-          if (FAST || cnt[k] > 0)
-            load_unit<T, VEC>(sp + loff[k], buf[k], FAST || full[k], cnt[k]);
-#pragma unroll
-        for (int k = 0; k < UPT; ++k)
-#pragma unroll
-          for (int v = 0; v < VEC; ++v) {
-            if (Pol::SPECIAL && is_nan(buf[k][v])) nanm |= 1u << (k * VEC + v);
-            acc[k][v] = Pol::seq(acc[k][v], buf[k][v]);
-          }
-      }
+        }
+      };
+      batch(0, true, INFL > P);
+      int i = INFL;
+      for (; i + INFL <= P; i += INFL) batch(i, false, false);
+      if (i < P) batch(i, false, true);
 #pragma unroll
       for (int k = 0; k < UPT; ++k)
 #pragma unroll
