@@ -623,9 +623,11 @@ __global__ __launch_bounds__(kMoveBlock) void k_gather_il(
   }
 }
 
-// 16-byte units and at least one full super-chunk per block
+// On unless turned off (oscd_set_gather_interleave): 16-byte units and at
+// least one full super-chunk per block
+static bool g_gather_il = true;
 static bool gather_il_pays(int W, uint64_t nunits) {
-  return W == 16 && nunits >= (uint64_t)kGatherS * kRepWgUnits;
+  return g_gather_il && W == 16 && nunits >= (uint64_t)kGatherS * kRepWgUnits;
 }
 
 static int gather_il(const oscd_geom *g, int src_member, uint64_t soff,
@@ -652,6 +654,10 @@ static int replicate_auto_group(uint64_t nbytes) {
 }
 
 }  // namespace osc
+
+extern "C" void oscd_set_gather_interleave(int on) {
+  osc::g_gather_il = on != 0;
+}
 
 extern "C" int oscd_replicate(const oscd_geom *g, uint64_t off, uint64_t nbytes,
                               int group, int width, const void *skip,

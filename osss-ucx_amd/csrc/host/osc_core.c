@@ -90,6 +90,10 @@ int osc_init(int npes, size_t heap_bytes_per_pe, int device) {
       osc_set_reduce_split(-1);
   }
   {
+    const char *gi = getenv("OSC_GATHER_INTERLEAVE");
+    oscd_set_gather_interleave(gi == NULL || gi[0] == '\0' || atoi(gi) != 0);
+  }
+  {
     const char *br = getenv("OSC_BATCH_BRANCHES");
     osc_g.batch_branches = (br != NULL && br[0] != '\0') ? atoi(br) : 4;
     if (osc_g.batch_branches < 1) osc_g.batch_branches = 1;
@@ -205,6 +209,12 @@ int osc_set_reduce_split(int group) {
     return osc_set_error(OSC_ERR_ARG,
                          "replicate group %d (-1, 0, 1, 2, 4, 8, 16)", group);
   oscd_set_reduce_split(group != 0, group);
+  return OSC_OK;
+}
+int osc_set_gather_interleave(int on) {
+  if (on != 0 && on != 1)
+    return osc_set_error(OSC_ERR_ARG, "gather interleave %d (0 or 1)", on);
+  oscd_set_gather_interleave(on);
   return OSC_OK;
 }
 int osc_reduce_split_group(int npes, size_t nbytes, int *width) {

@@ -45,16 +45,22 @@ def test_fcollect(heap, nb):
         assert np.array_equal(got, O.fcollect(src)), P
 
 
+@pytest.mark.parametrize("il", [1, 0])
 @pytest.mark.parametrize("nb", [(512 << 10) - 16, 512 << 10, (512 << 10) + 16,
                                 (1 << 20) + 48, 3 << 19])
-def test_fcollect_interleaved_blocks(heap, nb):
+def test_fcollect_interleaved_blocks(heap, nb, il):
     """Blocks of 512 KiB and more (16-byte units) take the interleaved
-    gather: whole super-chunks without bounds tests, then the ragged rest."""
+    gather: whole super-chunks without bounds tests, then the ragged rest;
+    with the interleave off, the destination-major chunks."""
     rng = np.random.default_rng(nb)
-    for P in (2, 5, 13):
-        src = rng.integers(0, 256, (P, nb), dtype=np.uint8)
-        got = move(heap, "fcollect", src, nb, team=team_of(heap, P))
-        assert np.array_equal(got, O.fcollect(src)), P
+    osc.check(heap.L.osc_set_gather_interleave(il), "gather order")
+    try:
+        for P in (2, 5, 13):
+            src = rng.integers(0, 256, (P, nb), dtype=np.uint8)
+            got = move(heap, "fcollect", src, nb, team=team_of(heap, P))
+            assert np.array_equal(got, O.fcollect(src)), P
+    finally:
+        heap.L.osc_set_gather_interleave(1)
 
 
 @pytest.mark.parametrize("root", [0, 6])

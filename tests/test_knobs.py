@@ -1,0 +1,19 @@
+"""The library's tuning switches reject values outside their range and take
+the valid ones without a device (they only set host-side state)."""
+import osc
+
+
+def test_gather_interleave_switch():
+    L = osc.lib()
+    assert L.osc_set_gather_interleave(2) != 0
+    assert L.osc_set_gather_interleave(-1) != 0
+    assert L.osc_set_gather_interleave(0) == 0
+    assert L.osc_set_gather_interleave(1) == 0
+
+
+def test_reduce_split_switch():
+    L = osc.lib()
+    for bad in (3, 5, 32, -2):
+        assert L.osc_set_reduce_split(bad) != 0
+    for good in (0, 1, 2, 4, 8, 16, -1):
+        assert L.osc_set_reduce_split(good) == 0
