@@ -1,7 +1,7 @@
 """Large reductions of 2-, 4-, 8- and 16-byte elements run in two kernels:
 the slice walk stores member 0's result only, then a replicate kernel copies
 it to the other members, each loaded piece to a group of consecutive
-members, in 1 KiB pieces interleaved over 16 workgroups.  Checked against
+members, in 1 KiB pieces interleaved over 16, 32 or 64 workgroups.  Checked against
 the oracle just above and below the size where the split starts (1 MiB per
 member, at least 8 members, a multiple of 16 bytes), on power-of-two and
 odd teams, a strided team, in place, in both float modes, with the bytes
@@ -194,3 +194,29 @@ def test_split_minmax_special_spans(heap, dtype, op, team):
     finally:
         heap.L.osc_set_reduce_split(-1)
     assert bit_equal(one, want, dtype)
+
+
+@pytest.mark.parametrize("group", [-1, 2])
+@pytest.mark.parametrize("dtype,op", [("f32", "max"), ("f64", "min")])
+@pytest.mark.parametrize("team", [(0, 1, 16), (3, 1, 29)])
+def test_split_minmax_special_spans_whole_super_chunks(heap, dtype, op, team,
+                                                       group):
+    """4 MiB per member and a ragged rest: the replicate's branch-free body
+    for whole super-chunks (2 MiB at the default width 32, 1 MiB at width
+    16 with 2 members per workgroup) reads the skip flags as a wave mask;
+    the special elements planted inside those super-chunks and in the tail
+    keep their per-member results."""
+    start, stride, P = team
+    pes = [start + k * stride for k in range(P)]
+    t = heap.split(start, stride, P)
+    es = 4 if dtype == "f32" else 8
+    n = (4 << 20) // es + 1004  # a multiple of 16 bytes: split
+    rng = np.random.default_rng(zlib.crc32(repr((dtype, op, team)).encode()))
+    src = planted(rng, dtype, P, n)
+    want = O.rec_dbl(op, dtype, src)
+    osc.check(heap.L.osc_set_reduce_split(group), "split group")
+    try:
+        got = reduce_checked(heap, t, pes, op, dtype, src, osc.FP_ACCURATE)
+    finally:
+        heap.L.osc_set_reduce_split(-1)
+    assert bit_equal(got, want, dtype)
