@@ -152,7 +152,7 @@ int osc_set_reduce_split(int group);
  * timed yet; the team size does not enter the choice); *width (may be
  * NULL) receives the interleave width. */
 int osc_reduce_split_group(int npes, size_t nbytes, int *width);
-/* fcollect and broadcast blocks of 16-byte units from 512 KiB per member:
+/* fcollect and broadcast blocks of 16-byte units from 64 KiB per member:
  * 1 (default) = the replicate's interleaved write order, 8 workgroups per
  * super-chunk; 0 = destination-major chunks, the order smaller blocks
  * take.  Results are identical either way.  OSC_GATHER_INTERLEAVE sets it

@@ -511,38 +511,37 @@ namespace osc {
 // fetches the flag of the wave's k-th piece in the same batch as the data
 // loads, and the flags become a wave-uniform mask once the data loads are
 // in flight.
-template <int G, int S, bool FULL, bool SKIP>
+template <int G, int S, int U, bool FULL, bool SKIP>
 __device__ __forceinline__ void rep_visit(const oscd_geom &g, uint64_t off,
                                           const char *src, uint64_t nunits,
                                           uint64_t sc, uint64_t x, int d0,
                                           uint32_t w, uint32_t l,
                                           const uint8_t *skip) {
-  // lanes >= kRepUpt repeat the first kRepUpt lanes' pieces
+  // lanes >= U repeat the first U lanes' pieces
   uint8_t f = 0;
   bool fin = true;
   if (SKIP) {
-    const uint64_t uf = il_unit<S, kRepUpt>(sc, x, (int)(l % kRepUpt), w, 0);
+    const uint64_t uf = il_unit<S, U>(sc, x, (int)(l % U), w, 0);
     fin = FULL || uf < nunits;
     f = skip[(fin ? uf : nunits - 1) / 256];
   }
-  uint64_t u[kRepUpt];
-  typename Raw<16>::type v[kRepUpt];
+  uint64_t u[U];
+  typename Raw<16>::type v[U];
 #pragma unroll
-  for (int k = 0; k < kRepUpt; ++k) {
-    u[k] = il_unit<S, kRepUpt>(sc, x, k, w, l);
+  for (int k = 0; k < U; ++k) {
+    u[k] = il_unit<S, U>(sc, x, k, w, l);
     if (FULL || u[k] < nunits) v[k] = ld<16>(src + u[k] * 16);
   }
   __builtin_amdgcn_sched_barrier(0);
   // bit k set: the wave stores its k-th piece
   const uint32_t keep =
-      SKIP ? (uint32_t)__ballot(l < kRepUpt && fin && f == 0)
-           : (1u << kRepUpt) - 1;
+      SKIP ? (uint32_t)__ballot(l < U && fin && f == 0) : (1u << U) - 1;
 #pragma unroll
   for (int q = 0; q < G; ++q) {
     if (d0 + q >= g.npes) break;  // the last group may be short
     char *dst = member_ptr(g, d0 + q, off);
 #pragma unroll
-    for (int k = 0; k < kRepUpt; ++k)
+    for (int k = 0; k < U; ++k)
       if (((keep >> k) & 1u) != 0 && (FULL || u[k] < nunits))
         st_nt<16>(dst + u[k] * 16, v[k]);
   }
@@ -562,9 +561,11 @@ __global__ __launch_bounds__(kMoveBlock) void k_replicate(
     const uint64_t gi = r % ngroups, sc = r / ngroups;
     const int d0 = 1 + (int)gi * G;  // member 0 already holds the result
     if ((sc + 1) * (uint64_t)S * kRepWgUnits <= nunits)
-      rep_visit<G, S, true, SKIP>(g, off, src, nunits, sc, x, d0, w, l, skip);
+      rep_visit<G, S, kRepUpt, true, SKIP>(g, off, src, nunits, sc, x, d0, w,
+                                           l, skip);
     else
-      rep_visit<G, S, false, SKIP>(g, off, src, nunits, sc, x, d0, w, l, skip);
+      rep_visit<G, S, kRepUpt, false, SKIP>(g, off, src, nunits, sc, x, d0, w,
+                                            l, skip);
   }
 }
 
@@ -589,16 +590,20 @@ static int replicate_g(const oscd_geom *g, uint64_t off, uint64_t nbytes,
 }
 
 // Gather in the replicate's interleaved write order: the S workgroups of a
-// group share one super-chunk (S*kRepWgUnits units) of one (source,
-// destination) pair.  Virtual block vb -> x = vb % S, then destination
-// (fastest, so the source super-chunk is read from HBM once and from the
-// XCD's L2 by the other destinations), then super-chunk, then source.
-// fcollect at 256 members x 1 MiB: 11.1-11.3 ms with S = 8 against
-// 11.9-12.1 ms for k_gather's order (S = 16: 12.6 ms; 2, 4 or 8
-// destinations per workgroup: 11.3-12.0 ms).
+// group share one super-chunk (S*256*U units) of one (source, destination)
+// pair.  Virtual block vb -> x = vb % S, then destination (fastest, so the
+// source super-chunk is read from HBM once and from the XCD's L2 by the
+// other destinations), then super-chunk, then source.  U, the units per
+// thread, is the largest of 16, 8, 4, 2 whose super-chunk fits in a block.
+// Measured on MI355X against k_gather's order (destination-major chunks,
+// which fetch each source chunk once per XCD): fcollect 256 x 1 MiB 11.1-11.3
+// vs 11.9-12.1 ms (U = 16; S = 16: 12.6 ms; 2, 4 or 8 destinations per
+// workgroup: 11.3-12.0 ms); 256 x 256 KiB 2.69 vs 3.07-3.12 ms; 128 x 256 KiB
+// 0.70 vs 0.98-1.05 ms; 128 x 128 KiB 338 vs 361-400 us; 64 x 64 KiB 45 vs
+// 45-51 us.
 constexpr int kGatherS = 8;
 
-template <int S>
+template <int S, int U>
 __global__ __launch_bounds__(kMoveBlock) void k_gather_il(
     const oscd_geom g, const int src_member, const uint64_t soff,
     const uint64_t doff, const uint64_t doff_step, const uint64_t nunits,
@@ -614,33 +619,52 @@ __global__ __launch_bounds__(kMoveBlock) void k_gather_il(
     const int i = src_member >= 0 ? src_member : (int)(t / nsc);
     const char *src = member_ptr(g, i, soff);
     const uint64_t off = doff + (uint64_t)i * doff_step;
-    if ((sc + 1) * (uint64_t)S * kRepWgUnits <= nunits)
-      rep_visit<1, S, true, false>(g, off, src, nunits, sc, x, j, w, l,
-                                   nullptr);
+    if ((sc + 1) * (uint64_t)S * kMoveBlock * U <= nunits)
+      rep_visit<1, S, U, true, false>(g, off, src, nunits, sc, x, j, w, l,
+                                      nullptr);
     else
-      rep_visit<1, S, false, false>(g, off, src, nunits, sc, x, j, w, l,
-                                    nullptr);
+      rep_visit<1, S, U, false, false>(g, off, src, nunits, sc, x, j, w, l,
+                                       nullptr);
   }
 }
 
-// On unless turned off (oscd_set_gather_interleave): 16-byte units and at
-// least one full super-chunk per block
+// On unless turned off (oscd_set_gather_interleave): 16-byte units and a
+// block of at least one super-chunk at 2 units per thread (64 KiB)
 static bool g_gather_il = true;
 static bool gather_il_pays(int W, uint64_t nunits) {
-  return g_gather_il && W == 16 && nunits >= (uint64_t)kGatherS * kRepWgUnits;
+  return g_gather_il && W == 16 &&
+         nunits >= (uint64_t)kGatherS * kMoveBlock * 2;
+}
+
+template <int U>
+static int gather_il_u(const oscd_geom *g, int src_member, uint64_t soff,
+                       uint64_t doff, uint64_t doff_step, uint64_t nunits,
+                       uint64_t nsrc, int skip, hipStream_t s) {
+  const uint64_t scu = (uint64_t)kGatherS * kMoveBlock * U;
+  const uint64_t nsc = (nunits + scu - 1) / scu;
+  const uint64_t total = (uint64_t)g->npes * nsrc * nsc * kGatherS;
+  const uint64_t grid = total < kRepMaxGrid ? total : kRepMaxGrid;
+  hipLaunchKernelGGL((k_gather_il<kGatherS, U>), dim3((unsigned)grid),
+                     dim3(kMoveBlock), 0, s, *g, src_member, soff, doff,
+                     doff_step, nunits, nsc, skip, total);
+  return check_launch();
 }
 
 static int gather_il(const oscd_geom *g, int src_member, uint64_t soff,
                      uint64_t doff, uint64_t doff_step, uint64_t nunits,
                      uint64_t nsrc, int skip, hipStream_t s) {
-  const uint64_t scu = (uint64_t)kGatherS * kRepWgUnits;
-  const uint64_t nsc = (nunits + scu - 1) / scu;
-  const uint64_t total = (uint64_t)g->npes * nsrc * nsc * kGatherS;
-  const uint64_t grid = total < kRepMaxGrid ? total : kRepMaxGrid;
-  hipLaunchKernelGGL((k_gather_il<kGatherS>), dim3((unsigned)grid),
-                     dim3(kMoveBlock), 0, s, *g, src_member, soff, doff,
-                     doff_step, nunits, nsc, skip, total);
-  return check_launch();
+  constexpr uint64_t per = (uint64_t)kGatherS * kMoveBlock;  // units at U=1
+  if (nunits >= per * 16)
+    return gather_il_u<16>(g, src_member, soff, doff, doff_step, nunits, nsrc,
+                           skip, s);
+  if (nunits >= per * 8)
+    return gather_il_u<8>(g, src_member, soff, doff, doff_step, nunits, nsrc,
+                          skip, s);
+  if (nunits >= per * 4)
+    return gather_il_u<4>(g, src_member, soff, doff, doff_step, nunits, nsrc,
+                          skip, s);
+  return gather_il_u<2>(g, src_member, soff, doff, doff_step, nunits, nsrc,
+                        skip, s);
 }
 
 // Members per workgroup for group -1: one.  64 members, width 32, per

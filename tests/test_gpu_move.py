@@ -46,12 +46,15 @@ def test_fcollect(heap, nb):
 
 
 @pytest.mark.parametrize("il", [1, 0])
-@pytest.mark.parametrize("nb", [(512 << 10) - 16, 512 << 10, (512 << 10) + 16,
-                                (1 << 20) + 48, 3 << 19])
+@pytest.mark.parametrize("nb", [(64 << 10) - 16, 64 << 10, (64 << 10) + 16,
+                                (128 << 10) - 16, (128 << 10) + 48,
+                                (256 << 10) + 16, (512 << 10) - 16, 512 << 10,
+                                (512 << 10) + 16, (1 << 20) + 48, 3 << 19])
 def test_fcollect_interleaved_blocks(heap, nb, il):
-    """Blocks of 512 KiB and more (16-byte units) take the interleaved
-    gather: whole super-chunks without bounds tests, then the ragged rest;
-    with the interleave off, the destination-major chunks."""
+    """Blocks of 64 KiB and more (16-byte units) take the interleaved
+    gather, 2, 4, 8 or 16 units per thread by block size: whole
+    super-chunks without bounds tests, then the ragged rest; with the
+    interleave off, the destination-major chunks."""
     rng = np.random.default_rng(nb)
     osc.check(heap.L.osc_set_gather_interleave(il), "gather order")
     try:
@@ -64,7 +67,7 @@ def test_fcollect_interleaved_blocks(heap, nb, il):
 
 
 @pytest.mark.parametrize("root", [0, 6])
-@pytest.mark.parametrize("nb", [512 << 10, (1 << 20) + 16])
+@pytest.mark.parametrize("nb", [(96 << 10) + 16, 512 << 10, (1 << 20) + 16])
 def test_broadcast_interleaved_blocks(heap, root, nb):
     rng = np.random.default_rng(root + nb)
     P = 9
