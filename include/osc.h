@@ -152,12 +152,13 @@ int osc_set_reduce_split(int group);
  * timed yet; the team size does not enter the choice); *width (may be
  * NULL) receives the interleave width. */
 int osc_reduce_split_group(int npes, size_t nbytes, int *width);
-/* fcollect and broadcast blocks of 16-byte units from 64 KiB per member:
- * 1 (default) = the replicate's interleaved write order, 8 workgroups per
- * super-chunk; 0 = destination-major chunks, the order smaller blocks
- * take.  Results are identical either way.  OSC_GATHER_INTERLEAVE sets it
- * at init. */
-int osc_set_gather_interleave(int on);
+/* fcollect, broadcast and alltoall blocks of 16-byte units from 64 KiB per
+ * member: 1 (default) = the replicate's interleaved write order, 8
+ * workgroups per super-chunk; 0 = the chunked orders smaller blocks take
+ * (destination-major for fcollect/broadcast, pair-major for alltoall).
+ * Results are identical either way.  OSC_MOVE_INTERLEAVE sets it at
+ * init. */
+int osc_set_move_interleave(int on);
 const char *osc_strerror(int code);
 const char *osc_last_error(void);
 int osc_last_error_code(void); /* code of the calling thread's last error */

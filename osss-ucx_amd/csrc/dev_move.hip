@@ -511,7 +511,7 @@ namespace osc {
 // fetches the flag of the wave's k-th piece in the same batch as the data
 // loads, and the flags become a wave-uniform mask once the data loads are
 // in flight.
-template <int G, int S, int U, bool FULL, bool SKIP>
+template <int G, int S, int U, bool FULL, bool SKIP, bool NTL = false>
 __device__ __forceinline__ void rep_visit(const oscd_geom &g, uint64_t off,
                                           const char *src, uint64_t nunits,
                                           uint64_t sc, uint64_t x, int d0,
@@ -530,7 +530,8 @@ __device__ __forceinline__ void rep_visit(const oscd_geom &g, uint64_t off,
 #pragma unroll
   for (int k = 0; k < U; ++k) {
     u[k] = il_unit<S, U>(sc, x, k, w, l);
-    if (FULL || u[k] < nunits) v[k] = ld<16>(src + u[k] * 16);
+    if (FULL || u[k] < nunits)
+      v[k] = NTL ? ld_nt<16>(src + u[k] * 16) : ld<16>(src + u[k] * 16);
   }
   __builtin_amdgcn_sched_barrier(0);
   // bit k set: the wave stores its k-th piece
@@ -628,11 +629,11 @@ __global__ __launch_bounds__(kMoveBlock) void k_gather_il(
   }
 }
 
-// On unless turned off (oscd_set_gather_interleave): 16-byte units and a
+// On unless turned off (oscd_set_move_interleave): 16-byte units and a
 // block of at least one super-chunk at 2 units per thread (64 KiB)
-static bool g_gather_il = true;
+static bool g_move_il = true;
 static bool gather_il_pays(int W, uint64_t nunits) {
-  return g_gather_il && W == 16 &&
+  return g_move_il && W == 16 &&
          nunits >= (uint64_t)kGatherS * kMoveBlock * 2;
 }
 
@@ -679,8 +680,8 @@ static int replicate_auto_group(uint64_t nbytes) {
 
 }  // namespace osc
 
-extern "C" void oscd_set_gather_interleave(int on) {
-  osc::g_gather_il = on != 0;
+extern "C" void oscd_set_move_interleave(int on) {
+  osc::g_move_il = on != 0;
 }
 
 extern "C" int oscd_replicate(const oscd_geom *g, uint64_t off, uint64_t nbytes,
@@ -852,6 +853,45 @@ extern "C" int oscd_signal(uint64_t *sig, uint64_t value, int op,
   return check_launch();
 }
 
+// Alltoall of 16-byte blocks of 64 KiB or more in the interleaved write
+// order: the 8 workgroups of a group share one super-chunk (8*256*2 units)
+// of one (source i, destination j) pair; virtual block vb -> x = vb % 8,
+// then super-chunk, then pair with the source fastest, so consecutive
+// groups fill one destination's region.  Streaming loads when the call's
+// footprint exceeds the 256 MB Infinity Cache, cached loads when it fits.
+// Measured on MI355X against the pair-major chunks of k_a2a_big: 128 PEs x
+// 256 KiB 1.28-1.29 vs 1.55-1.59 ms; 128 x 64 KiB 323-327 vs 394-408 us;
+// 64 x 1 MiB 1.33 vs 1.56-1.57 ms; 64 x 64 KiB (256 MiB, cached loads)
+// 77.6-77.7 vs 78.4-79.4 us.
+namespace osc {
+
+constexpr int kA2aS = 8, kA2aU = 2;
+
+template <int S, int U, bool NTL>
+__global__ __launch_bounds__(kMoveBlock) void k_a2a_il(
+    const oscd_geom g, const uint64_t doff, const uint64_t soff,
+    const uint64_t nbytes, const uint64_t nsc, const uint64_t total) {
+  const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const uint64_t P = (uint64_t)g.npes;
+  const uint64_t nunits = nbytes / 16;
+  // gridDim.x is a multiple of S: every visit keeps its place x
+  for (uint64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const uint64_t x = vb % S, r = vb / S;
+    const uint64_t sc = r % nsc, pair = r / nsc;
+    const int i = (int)(pair % P), j = (int)(pair / P);
+    const char *src = member_ptr(g, i, soff) + (uint64_t)j * nbytes;
+    const uint64_t off = doff + (uint64_t)i * nbytes;
+    if ((sc + 1) * (uint64_t)S * kMoveBlock * U <= nunits)
+      rep_visit<1, S, U, true, false, NTL>(g, off, src, nunits, sc, x, j, w,
+                                           l, nullptr);
+    else
+      rep_visit<1, S, U, false, false, NTL>(g, off, src, nunits, sc, x, j, w,
+                                            l, nullptr);
+  }
+}
+
+}  // namespace osc
+
 extern "C" int oscd_alltoall(const oscd_geom *g, uint64_t dest_off,
                              uint64_t src_off, uint64_t nbytes, void *stream) {
   using namespace osc;
@@ -866,6 +906,21 @@ extern "C" int oscd_alltoall(const oscd_geom *g, uint64_t dest_off,
     OSC_W_DISPATCH(W, hipLaunchKernelGGL((k_a2a_small<WW>), dim3(t, t),
                                          dim3(kMoveBlock), 0, s, *g, dest_off,
                                          src_off, nbytes));
+    return check_launch();
+  }
+  if (g_move_il && W == 16 && nunits >= (uint64_t)kA2aS * kMoveBlock * kA2aU) {
+    const uint64_t scu = (uint64_t)kA2aS * kMoveBlock * kA2aU;
+    const uint64_t nsc = (nunits + scu - 1) / scu;
+    const uint64_t total = P * P * nsc * kA2aS;
+    const uint64_t grid = total < kRepMaxGrid ? total : kRepMaxGrid;
+    if (P * P * nbytes > (256ull << 20))
+      hipLaunchKernelGGL((k_a2a_il<kA2aS, kA2aU, true>), dim3((unsigned)grid),
+                         dim3(kMoveBlock), 0, s, *g, dest_off, src_off, nbytes,
+                         nsc, total);
+    else
+      hipLaunchKernelGGL((k_a2a_il<kA2aS, kA2aU, false>), dim3((unsigned)grid),
+                         dim3(kMoveBlock), 0, s, *g, dest_off, src_off, nbytes,
+                         nsc, total);
     return check_launch();
   }
   constexpr int UPT = 4;
@@ -897,7 +952,8 @@ extern "C" int oscd_alltoall(const oscd_geom *g, uint64_t dest_off,
 // 32 KiB visits, the walk's read pattern, with non-temporal loads) and
 // k_xor_fold folds those into a 16-byte sink.  (Device atomics into the
 // sink instead cost 50 us: the 2048 workgroups finish together and their
-// atomics on one address queue up, 353-358 us against 302-306 us.)  k_fill64 stores a 64-bit
+// atomics on one address queue up, 353-358 us against 302-306 us.)
+// k_fill64 stores a 64-bit
 // value over a range in the interleaved write order (64 workgroups per
 // group, 8 pieces per wave).  Both are the fastest shapes measured on
 // MI355X for 2 GiB: reads 300-303 us (7.1 TB/s; 16 units per thread 310 us,

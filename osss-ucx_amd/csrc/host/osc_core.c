@@ -90,8 +90,8 @@ int osc_init(int npes, size_t heap_bytes_per_pe, int device) {
       osc_set_reduce_split(-1);
   }
   {
-    const char *gi = getenv("OSC_GATHER_INTERLEAVE");
-    oscd_set_gather_interleave(gi == NULL || gi[0] == '\0' || atoi(gi) != 0);
+    const char *gi = getenv("OSC_MOVE_INTERLEAVE");
+    oscd_set_move_interleave(gi == NULL || gi[0] == '\0' || atoi(gi) != 0);
   }
   {
     const char *br = getenv("OSC_BATCH_BRANCHES");
@@ -211,10 +211,10 @@ int osc_set_reduce_split(int group) {
   oscd_set_reduce_split(group != 0, group);
   return OSC_OK;
 }
-int osc_set_gather_interleave(int on) {
+int osc_set_move_interleave(int on) {
   if (on != 0 && on != 1)
-    return osc_set_error(OSC_ERR_ARG, "gather interleave %d (0 or 1)", on);
-  oscd_set_gather_interleave(on);
+    return osc_set_error(OSC_ERR_ARG, "move interleave %d (0 or 1)", on);
+  oscd_set_move_interleave(on);
   return OSC_OK;
 }
 int osc_reduce_split_group(int npes, size_t nbytes, int *width) {

@@ -80,9 +80,9 @@ int oscd_reduce(const oscd_geom *g, int op, int dtype, int fpmode, int shape,
  * pieces with a nonzero byte are left alone. */
 int oscd_replicate(const oscd_geom *g, uint64_t off, uint64_t nbytes,
                    int group, int width, const void *skip, void *stream);
-/* fcollect/broadcast blocks of 16-byte units from 64 KiB in the
- * interleaved order (on, the default) or destination-major chunks (off) */
-void oscd_set_gather_interleave(int on);
+/* fcollect/broadcast/alltoall blocks of 16-byte units from 64 KiB in the
+ * interleaved order (on, the default) or the chunked orders (off) */
+void oscd_set_move_interleave(int on);
 /* read-only / write-only streams (16-byte aligned, bytes a multiple of
  * 16): XOR of every 16-byte unit into sink16 (cleared first); fill with a
  * 64-bit value in the replicate's write order */

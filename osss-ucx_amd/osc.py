@@ -101,7 +101,7 @@ def lib() -> C.CDLL:
         "osc_get_reduce_shape": (i, []),
         "osc_set_batch_branches": (i, [i]),
         "osc_set_reduce_split": (i, [i]),
-        "osc_set_gather_interleave": (i, [i]),
+        "osc_set_move_interleave": (i, [i]),
         "osc_reduce_split_group": (i, [i, sz, C.POINTER(C.c_int)]),
         "osc_device_xor_read": (i, [vp, sz, vp, vp]),
         "osc_device_fill64": (i, [vp, sz, u64, vp]),

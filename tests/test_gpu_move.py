@@ -56,14 +56,14 @@ def test_fcollect_interleaved_blocks(heap, nb, il):
     super-chunks without bounds tests, then the ragged rest; with the
     interleave off, the destination-major chunks."""
     rng = np.random.default_rng(nb)
-    osc.check(heap.L.osc_set_gather_interleave(il), "gather order")
+    osc.check(heap.L.osc_set_move_interleave(il), "move order")
     try:
         for P in (2, 5, 13):
             src = rng.integers(0, 256, (P, nb), dtype=np.uint8)
             got = move(heap, "fcollect", src, nb, team=team_of(heap, P))
             assert np.array_equal(got, O.fcollect(src)), P
     finally:
-        heap.L.osc_set_gather_interleave(1)
+        heap.L.osc_set_move_interleave(1)
 
 
 @pytest.mark.parametrize("root", [0, 6])
@@ -74,6 +74,28 @@ def test_broadcast_interleaved_blocks(heap, root, nb):
     src = rng.integers(0, 256, (P, nb), dtype=np.uint8)
     got = move(heap, "broadcast", src, nb, team=team_of(heap, P), root=root)
     assert np.array_equal(got, O.broadcast(src, root))
+
+
+@pytest.mark.parametrize("il", [1, 0])
+@pytest.mark.parametrize("nb", [(64 << 10) - 16, 64 << 10, (64 << 10) + 16,
+                                (96 << 10) + 48, (192 << 10) + 16, 256 << 10])
+def test_alltoall_interleaved_blocks(heap, nb, il):
+    """Blocks of 64 KiB and more (16-byte units) take the interleaved
+    alltoall: whole super-chunks without bounds tests, then the ragged
+    rest.  P^2 x block above 256 MiB (40 members x 192 or 256 KiB) takes
+    streaming loads, below it cached loads; with the interleave off, the
+    pair-major chunks."""
+    rng = np.random.default_rng(7 + nb)
+    osc.check(heap.L.osc_set_move_interleave(il), "move order")
+    try:
+        for P in (2, 5, 13, 40):
+            if P * nb > (12 << 20):
+                continue
+            src = rng.integers(0, 256, (P, P * nb), dtype=np.uint8)
+            got = move(heap, "alltoall", src, nb, team=team_of(heap, P))
+            assert np.array_equal(got, O.alltoall(src)), P
+    finally:
+        heap.L.osc_set_move_interleave(1)
 
 
 @pytest.mark.parametrize("nb", SIZES)

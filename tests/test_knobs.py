@@ -3,12 +3,12 @@ the valid ones without a device (they only set host-side state)."""
 import osc
 
 
-def test_gather_interleave_switch():
+def test_move_interleave_switch():
     L = osc.lib()
-    assert L.osc_set_gather_interleave(2) != 0
-    assert L.osc_set_gather_interleave(-1) != 0
-    assert L.osc_set_gather_interleave(0) == 0
-    assert L.osc_set_gather_interleave(1) == 0
+    assert L.osc_set_move_interleave(2) != 0
+    assert L.osc_set_move_interleave(-1) != 0
+    assert L.osc_set_move_interleave(0) == 0
+    assert L.osc_set_move_interleave(1) == 0
 
 
 def test_reduce_split_switch():
