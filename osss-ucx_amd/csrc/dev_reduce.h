@@ -599,53 +599,117 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
   if (any) {
     Acc acc[UPT][VEC];
     if constexpr (!Pol::TREE) {
-      // Slices in batches of INFL whose loads are unconditional (so the
-      // compiler's wait counts stay exact and the next batch's loads are
-      // not held back by a drained queue).  The first batch holds slice 0,
-      // which seeds the accumulator; the last holds the P % INFL leftover
-      // slices.  In those two an index past P-1 re-reads slice P-1 and its
-      // value is selected away, so every load still issues unconditionally
-      // and a call makes ceil(P / INFL) dependent round trips: seeding from
-      // slice 0 alone and taking the leftover slices one at a time made a
-      // 1x8 call on 8 members take 8 round trips instead of one.
       uint32_t nanm = 0;
-      // slices i .. i+INFL-1, those below P combined in order; FIRST: slice
-      // i (= 0) seeds the accumulator
-      auto batch = [&](const int i, const bool first, const bool tail)
-                       __attribute__((always_inline)) {
-        T buf[INFL][UPT][VEC];
-#pragma unroll
-        for (int j = 0; j < INFL; ++j) {
-          const int m = (!tail || i + j < P) ? i + j : P - 1;
-          const char *sp = s0 + m * step;
-#pragma unroll
+      // The small-call shapes (2x4, 1x8) load the seed slice and the leftover
+      // slices in full batches, so a call makes ceil(P / INFL) dependent
+      // round trips.  The wide shapes (8x2, 8x1) keep the seed slice and the
+      // leftover slices apart: at 64 members x 4M the batched form ran the
+      // matrix 0.7-2.7% slower (one box, alternating runs).
+      if constexpr (INFL >= 4) {
+        // Slices in batches of INFL whose loads are unconditional (so the
+        // compiler's wait counts stay exact and the next batch's loads are
+        // not held back by a drained queue).  The first batch holds slice 0,
+        // which seeds the accumulator; the last holds the P % INFL leftover
+        // slices.  In those two an index past P-1 re-reads slice P-1 and its
+        // value is selected away, so every load still issues unconditionally
+        // and a call makes ceil(P / INFL) dependent round trips: seeding from
+        // slice 0 alone and taking the leftover slices one at a time made a
+        // 1x8 call on 8 members take 8 round trips instead of one.
+        // slices i .. i+INFL-1, those below P combined in order; FIRST: slice
+        // i (= 0) seeds the accumulator
+        auto batch = [&](const int i, const bool first, const bool tail)
+                         __attribute__((always_inline)) {
+          T buf[INFL][UPT][VEC];
+  #pragma unroll
+          for (int j = 0; j < INFL; ++j) {
+            const int m = (!tail || i + j < P) ? i + j : P - 1;
+            const char *sp = s0 + m * step;
+  #pragma unroll
+            for (int k = 0; k < UPT; ++k)
+              if (FAST || cnt[k] > 0)
+                load_unit<T, VEC>(sp + loff[k], buf[j][k], FAST || full[k],
+                                  cnt[k]);
+          }
+  #pragma unroll
+          for (int j = 0; j < INFL; ++j) {
+            const bool in = !tail || i + j < P;
+  #pragma unroll
+            for (int k = 0; k < UPT; ++k)
+  #pragma unroll
+              for (int v = 0; v < VEC; ++v) {
+                if (Pol::SPECIAL && in && is_nan(buf[j][k][v]))
+                  nanm |= 1u << (k * VEC + v);
+                if (first && j == 0) {
+                  acc[k][v] = Pol::lift(buf[j][k][v]);
+                } else {
+                  acc[k][v] = pick(in, Pol::seq(acc[k][v], buf[j][k][v]),
+                                   acc[k][v]);
+                }
+              }
+          }
+        };
+        batch(0, true, INFL > P);
+        int i = INFL;
+        for (; i + INFL <= P; i += INFL) batch(i, false, false);
+        if (i < P) batch(i, false, true);
+      } else {
+        // Slice 0 seeds the accumulator; then groups of INFL slices whose
+        // loads are unconditional (so the compiler's wait counts stay exact
+        // and the next group's loads are not held back by a drained queue),
+        // then the < INFL leftover slices one at a time.
+        {
+          T buf[UPT][VEC];
+  #pragma unroll
           for (int k = 0; k < UPT; ++k)
             if (FAST || cnt[k] > 0)
-              load_unit<T, VEC>(sp + loff[k], buf[j][k], FAST || full[k],
-                                cnt[k]);
-        }
-#pragma unroll
-        for (int j = 0; j < INFL; ++j) {
-          const bool in = !tail || i + j < P;
-#pragma unroll
+              load_unit<T, VEC>(s0 + loff[k], buf[k], FAST || full[k], cnt[k]);
+  #pragma unroll
           for (int k = 0; k < UPT; ++k)
-#pragma unroll
+  #pragma unroll
             for (int v = 0; v < VEC; ++v) {
-              if (Pol::SPECIAL && in && is_nan(buf[j][k][v]))
-                nanm |= 1u << (k * VEC + v);
-              if (first && j == 0) {
-                acc[k][v] = Pol::lift(buf[j][k][v]);
-              } else {
-                acc[k][v] = pick(in, Pol::seq(acc[k][v], buf[j][k][v]),
-                                 acc[k][v]);
-              }
+              if (Pol::SPECIAL && is_nan(buf[k][v])) nanm |= 1u << (k * VEC + v);
+              acc[k][v] = Pol::lift(buf[k][v]);
             }
         }
-      };
-      batch(0, true, INFL > P);
-      int i = INFL;
-      for (; i + INFL <= P; i += INFL) batch(i, false, false);
-      if (i < P) batch(i, false, true);
+        int i = 1;
+        for (; i + INFL <= P; i += INFL) {
+          T buf[INFL][UPT][VEC];
+  #pragma unroll
+          for (int j = 0; j < INFL; ++j) {
+            const char *sp = s0 + (i + j) * step;
+  #pragma unroll
+            for (int k = 0; k < UPT; ++k)
+              if (FAST || cnt[k] > 0)
+                load_unit<T, VEC>(sp + loff[k], buf[j][k], FAST || full[k],
+                                  cnt[k]);
+          }
+  #pragma unroll
+          for (int j = 0; j < INFL; ++j)
+  #pragma unroll
+            for (int k = 0; k < UPT; ++k)
+  #pragma unroll
+              for (int v = 0; v < VEC; ++v) {
+                if (Pol::SPECIAL && is_nan(buf[j][k][v]))
+                  nanm |= 1u << (k * VEC + v);
+                acc[k][v] = Pol::seq(acc[k][v], buf[j][k][v]);
+              }
+        }
+        for (; i < P; ++i) {
+          T buf[UPT][VEC];
+          const char *sp = s0 + i * step;
+  #pragma unroll
+          for (int k = 0; k < UPT; ++k)
+            if (FAST || cnt[k] > 0)
+              load_unit<T, VEC>(sp + loff[k], buf[k], FAST || full[k], cnt[k]);
+  #pragma unroll
+          for (int k = 0; k < UPT; ++k)
+  #pragma unroll
+            for (int v = 0; v < VEC; ++v) {
+              if (Pol::SPECIAL && is_nan(buf[k][v])) nanm |= 1u << (k * VEC + v);
+              acc[k][v] = Pol::seq(acc[k][v], buf[k][v]);
+            }
+        }
+      }
 #pragma unroll
       for (int k = 0; k < UPT; ++k)
 #pragma unroll
