@@ -913,7 +913,8 @@ extern "C" int oscd_alltoall(const oscd_geom *g, uint64_t dest_off,
     const uint64_t nsc = (nunits + scu - 1) / scu;
     const uint64_t total = P * P * nsc * kA2aS;
     const uint64_t grid = total < kRepMaxGrid ? total : kRepMaxGrid;
-    if (P * P * nbytes > (256ull << 20))
+    // the footprint in double: P^2 x nbytes can exceed 2^64
+    if ((double)P * (double)P * (double)nbytes > (double)(256ull << 20))
       hipLaunchKernelGGL((k_a2a_il<kA2aS, kA2aU, true>), dim3((unsigned)grid),
                          dim3(kMoveBlock), 0, s, *g, dest_off, src_off, nbytes,
                          nsc, total);
