@@ -630,7 +630,9 @@ __global__ __launch_bounds__(kMoveBlock) void k_gather_il(
 }
 
 // On unless turned off (oscd_set_move_interleave): 16-byte units and a
-// block of at least one super-chunk at 2 units per thread (64 KiB)
+// block of at least one super-chunk at 2 units per thread (64 KiB).  (One
+// unit per thread from 32 KiB measured slower than the destination-major
+// chunks: 256 PEs x 40-56 KiB blocks 752-795 vs 533-648 us.)
 static bool g_move_il = true;
 static bool gather_il_pays(int W, uint64_t nunits) {
   return g_move_il && W == 16 &&
@@ -866,6 +868,20 @@ extern "C" int oscd_signal(uint64_t *sig, uint64_t value, int op,
 namespace osc {
 
 constexpr int kA2aS = 8, kA2aU = 2;
+// Blocks of 24 KiB to 64 KiB take the interleaved order too when the loads
+// stream (the footprint exceeds the Infinity Cache).  Measured on MI355X
+// through osc_alltoall, interleaved vs pair-major chunks: 128 PEs x 24, 32,
+// 40, 48, 56, 60 KiB 145 / 191 / 243 / 264 / 297 / 335 vs 155 / 215 / 254 /
+// 311 / 358 / 384 us; 96 PEs x 40-60 KiB 134-190 vs 144-221 us (but x 32
+// KiB 98 vs 87 us).  With cached loads (64 and 96 PEs below 40 KiB) and
+// below 24 KiB (one partial super-chunk per pair) the pair-major chunks
+// were as fast or faster: 96 PEs x 20-24 KiB 67-89 vs 62-77 us.
+constexpr uint64_t kA2aMidUnits = 1536;
+
+static bool a2a_streams(uint64_t P, uint64_t nbytes) {
+  // the footprint in double: P^2 x nbytes can exceed 2^64
+  return (double)P * (double)P * (double)nbytes > (double)(256ull << 20);
+}
 
 template <int S, int U, bool NTL>
 __global__ __launch_bounds__(kMoveBlock) void k_a2a_il(
@@ -890,6 +906,24 @@ __global__ __launch_bounds__(kMoveBlock) void k_a2a_il(
   }
 }
 
+static int a2a_il(const oscd_geom *g, uint64_t dest_off, uint64_t src_off,
+                  uint64_t nbytes, hipStream_t s) {
+  const uint64_t P = (uint64_t)g->npes, nunits = nbytes / 16;
+  const uint64_t scu = (uint64_t)kA2aS * kMoveBlock * kA2aU;
+  const uint64_t nsc = (nunits + scu - 1) / scu;
+  const uint64_t total = P * P * nsc * kA2aS;
+  const uint64_t grid = total < kRepMaxGrid ? total : kRepMaxGrid;
+  if (a2a_streams(P, nbytes))
+    hipLaunchKernelGGL((k_a2a_il<kA2aS, kA2aU, true>), dim3((unsigned)grid),
+                       dim3(kMoveBlock), 0, s, *g, dest_off, src_off, nbytes,
+                       nsc, total);
+  else
+    hipLaunchKernelGGL((k_a2a_il<kA2aS, kA2aU, false>), dim3((unsigned)grid),
+                       dim3(kMoveBlock), 0, s, *g, dest_off, src_off, nbytes,
+                       nsc, total);
+  return check_launch();
+}
+
 }  // namespace osc
 
 extern "C" int oscd_alltoall(const oscd_geom *g, uint64_t dest_off,
@@ -908,22 +942,10 @@ extern "C" int oscd_alltoall(const oscd_geom *g, uint64_t dest_off,
                                          src_off, nbytes));
     return check_launch();
   }
-  if (g_move_il && W == 16 && nunits >= (uint64_t)kA2aS * kMoveBlock * kA2aU) {
-    const uint64_t scu = (uint64_t)kA2aS * kMoveBlock * kA2aU;
-    const uint64_t nsc = (nunits + scu - 1) / scu;
-    const uint64_t total = P * P * nsc * kA2aS;
-    const uint64_t grid = total < kRepMaxGrid ? total : kRepMaxGrid;
-    // the footprint in double: P^2 x nbytes can exceed 2^64
-    if ((double)P * (double)P * (double)nbytes > (double)(256ull << 20))
-      hipLaunchKernelGGL((k_a2a_il<kA2aS, kA2aU, true>), dim3((unsigned)grid),
-                         dim3(kMoveBlock), 0, s, *g, dest_off, src_off, nbytes,
-                         nsc, total);
-    else
-      hipLaunchKernelGGL((k_a2a_il<kA2aS, kA2aU, false>), dim3((unsigned)grid),
-                         dim3(kMoveBlock), 0, s, *g, dest_off, src_off, nbytes,
-                         nsc, total);
-    return check_launch();
-  }
+  if (g_move_il && W == 16 &&
+      (nunits >= (uint64_t)kA2aS * kMoveBlock * kA2aU ||
+       (nunits >= kA2aMidUnits && a2a_streams(P, nbytes))))
+    return a2a_il(g, dest_off, src_off, nbytes, s);
   constexpr int UPT = 4;
   constexpr uint64_t kChunk = kMoveBlock * UPT;
   if (nunits < kChunk) {

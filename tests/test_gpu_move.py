@@ -77,14 +77,17 @@ def test_broadcast_interleaved_blocks(heap, root, nb):
 
 
 @pytest.mark.parametrize("il", [1, 0])
-@pytest.mark.parametrize("nb", [(64 << 10) - 16, 64 << 10, (64 << 10) + 16,
+@pytest.mark.parametrize("nb", [(16 << 10) - 16, 16 << 10, (20 << 10) + 48,
+                                (24 << 10) - 16, 24 << 10, (40 << 10) + 16,
+                                (64 << 10) - 16, 64 << 10, (64 << 10) + 16,
                                 (96 << 10) + 48, (192 << 10) + 16, 256 << 10])
 def test_alltoall_interleaved_blocks(heap, nb, il):
     """Blocks of 64 KiB and more (16-byte units) take the interleaved
     alltoall: whole super-chunks without bounds tests, then the ragged
     rest.  P^2 x block above 256 MiB (40 members x 192 or 256 KiB) takes
-    streaming loads, below it cached loads; with the interleave off, the
-    pair-major chunks."""
+    streaming loads, below it cached loads; with the interleave off, and
+    for smaller blocks at these footprints, the pair-major chunks (below
+    16 KiB, several pairs per workgroup)."""
     rng = np.random.default_rng(7 + nb)
     osc.check(heap.L.osc_set_move_interleave(il), "move order")
     try:
@@ -96,6 +99,20 @@ def test_alltoall_interleaved_blocks(heap, nb, il):
             assert np.array_equal(got, O.alltoall(src)), P
     finally:
         heap.L.osc_set_move_interleave(1)
+
+
+@pytest.mark.parametrize("nb", [23536, 24576, 24592, (32 << 10) + 48])
+def test_alltoall_streaming_mid_blocks(nb):
+    """112 members, blocks around 24 KiB and 32 KiB: P^2 x block above
+    256 MiB, so blocks of 24 KiB to 64 KiB take the interleaved alltoall
+    with streaming loads (one ragged super-chunk per pair) and 23,536
+    bytes the pair-major chunks."""
+    P = 112
+    rng = np.random.default_rng(nb)
+    src = rng.integers(0, 256, (P, P * nb), dtype=np.uint8)
+    with osc.Heap(npes=P, heap_bytes=12 << 20) as h:
+        got = move(h, "alltoall", src, nb, team=h.world)
+    assert np.array_equal(got, O.alltoall(src))
 
 
 @pytest.mark.parametrize("nb", SIZES)
