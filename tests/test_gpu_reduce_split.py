@@ -75,6 +75,10 @@ CASES = [  # (dtype, op, n): at, above and off the 1 MiB/member threshold
     # 8 MiB per member (the size-picked group of 4) and a size that leaves
     # the last interleave group of 1 KiB pieces partly empty
     ("f64", "sum", 1 << 20), ("i64", "xor", (1 << 20) + 131072 + 16 * 5),
+    # 2-8 MiB per member: 1x8 walks below 8 MiB, 8x2 at 8 MiB
+    ("i64", "max", 1 << 18), ("f32", "sum", (3 << 20) // 4 + 4),
+    ("f64", "sum", (3 << 20) // 8 + 2), ("f64", "prod", (5 << 20) // 8 + 6),
+    ("f64", "max", (1 << 20) + 6),
 ]
 
 
@@ -196,21 +200,22 @@ def test_split_minmax_special_spans(heap, dtype, op, team):
     assert bit_equal(one, want, dtype)
 
 
+@pytest.mark.parametrize("mib", [4, 8])
 @pytest.mark.parametrize("group", [-1, 2])
 @pytest.mark.parametrize("dtype,op", [("f32", "max"), ("f64", "min")])
 @pytest.mark.parametrize("team", [(0, 1, 16), (3, 1, 29)])
 def test_split_minmax_special_spans_whole_super_chunks(heap, dtype, op, team,
-                                                       group):
-    """4 MiB per member and a ragged rest: the replicate's branch-free body
-    for whole super-chunks (2 MiB at the default width 32, 1 MiB at width
-    16 with 2 members per workgroup) reads the skip flags as a wave mask;
-    the special elements planted inside those super-chunks and in the tail
-    keep their per-member results."""
+                                                       group, mib):
+    """4 and 8 MiB per member and a ragged rest (1x8 and 8x2 walks): the
+    replicate's branch-free body for whole super-chunks (2 MiB at the default width 32, 1 MiB at width 16 with 2
+    members per workgroup) reads the skip flags as a wave mask; the special
+    elements planted inside those super-chunks and in the tail keep their
+    per-member results."""
     start, stride, P = team
     pes = [start + k * stride for k in range(P)]
     t = heap.split(start, stride, P)
     es = 4 if dtype == "f32" else 8
-    n = (4 << 20) // es + 1004  # a multiple of 16 bytes: split
+    n = (mib << 20) // es + 1004  # a multiple of 16 bytes: split
     rng = np.random.default_rng(zlib.crc32(repr((dtype, op, team)).encode()))
     src = planted(rng, dtype, P, n)
     want = O.rec_dbl(op, dtype, src)
