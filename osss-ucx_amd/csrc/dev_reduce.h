@@ -875,6 +875,19 @@ __device__ __forceinline__ void reduce_block(const oscd_geom &g,
     }
   };
   if (any) store_members(0, ONE ? 1 : P);
+  if constexpr (ONE && !FAST) {
+    // The call's last unit, when it holds fewer than VEC elements, goes to
+    // every member from here: the replicate copies whole 16-byte units
+    // only.  One thread of the grid has it.
+#pragma unroll
+    for (int k = 0; k < UPT; ++k)
+      if (cnt[k] > 0 && !full[k])
+        for (int j = 1; j < P; ++j)
+          store_unit<T, VEC>(d0 + j * step + loff[k], r[k], false, cnt[k],
+                             Pol::SPECIAL ? (spec >> (k * VEC)) &
+                                                ((1u << VEC) - 1u)
+                                          : 0u);
+  }
 
   // Special elements last: the stores above skip them, so their inputs
   // are still intact (also when dest == source), and the block-wide vote
@@ -1337,8 +1350,10 @@ static void launch_shape(const oscd_geom *g, int lg, uint64_t doff,
 // members writes at 5.1-5.3 TB/s and a fused u64 cell takes 735 us; the
 // replicate writes in an interleaved order that reaches 6.3-6.8 TB/s.
 // Every type whose 16-byte unit holds whole elements (2, 4, 8 and 16
-// bytes) takes it, from 1 MiB per member and 8 members.  The split costs
-// one extra launch and (P-1)/G cached re-reads of member 0's result.
+// bytes) takes it, from 1 MiB per member and 8 members.  The replicate
+// copies whole units; a last unit with fewer elements is stored to every
+// member by the walk.  The split costs one extra launch and (P-1)/G cached
+// re-reads of member 0's result.
 // Float MIN/MAX (SPECIAL) split too: the walk flags every 256-unit piece of
 // a span that holds a special element (whose members get per-member
 // results), stores those spans to every member itself, and the replicate
@@ -1349,11 +1364,14 @@ constexpr int kSplitMinPes = 8;
 constexpr uint64_t kSplitMinBytes = 1ull << 20;  // per member
 // 0: no split; -1: replicate group picked by size; else the group size
 extern int g_split_group;
+// false: calls whose bytes per member are not a multiple of 16 run as one
+// kernel (oscd_set_split_partial)
+extern bool g_split_partial;
 template <class T, class Pol>
 static bool split_pays(const oscd_geom *g, uint64_t n) {
   return g_split_group != 0 && vec_of<T>() * sizeof(T) == 16 &&
          g->npes >= kSplitMinPes && n * sizeof(T) >= kSplitMinBytes &&
-         (n * sizeof(T)) % 16 == 0;
+         (g_split_partial || (n * sizeof(T)) % 16 == 0);
 }
 // One flag byte per 256-unit piece for a split float MIN/MAX call on stream
 // s, or nullptr when the stream has none to give (under graph capture, more
@@ -1392,6 +1410,43 @@ constexpr int clamp_shape(int shape) {
              : (Pol::TREE ? (shape == 3 ? 3 : 2) : shape);
 }
 
+// Walk shape of a split call when the library picks it, by bytes per
+// member m (g_split_walk_rule, on by default; off: the policy's split shape
+// from 8 MiB, 1x8 below).  The walk's grid is one workgroup per span, so
+// the 8-unit shapes (32 KiB of each slice per workgroup) launch m / 32 KiB
+// workgroups: 256 at 8 MiB, one per CU.  Measured on MI355X, every variant
+// alternated in one process (a rebuilt library moves the heap's physical
+// pages, which alone moves these rates by a few percent): on heaps whose
+// members lie 6-36 MiB apart, 32-128 members, whole calls with 2x4 against
+// the 1x8 pick at m = 4 MiB: i64 sum 5.24-6.14 vs 4.90-5.77 TB/s, f64 sum
+// 5.38-6.38 vs 5.10-5.70, f32 sum 5.23-6.05 vs 4.94-5.77, f64 min
+// 5.14-6.06 vs 4.97-5.77; at 2 MiB +5-8% except FP64 Sum2 (1x8 stays:
+// 5.09-5.33 vs 4.51-5.00); at 8 MiB i64 sum +3-6% over 8x2, while the
+// FP32/FP64 accumulated sums keep 8x2/8x1 (2x4: -3-6%) and f64 min/max
+// takes 8x1 (+3%).  On the
+// batched workload's world (128 members about 1 GiB apart) the same calls
+// gain 0.7% overall (f32 at 2-4 MiB 3-9%), the rest within 1.5%.
+extern bool g_split_walk_rule;
+template <class T, class Pol>
+static int split_walk_shape(uint64_t n, int picked) {
+  if constexpr (Pol::TREE || Pol::RAB || !std::is_arithmetic<T>::value) {
+    return picked;
+  } else {
+    if (!g_split_walk_rule) return picked;
+    // FP64 SUM/PROD in twice the working precision
+    constexpr bool dd = std::is_same<T, double>::value &&
+                        !std::is_same<typename Pol::Acc, double>::value;
+    const uint64_t m = n * sizeof(T);
+    if (m < (2ull << 20)) return 3;
+    if (m < (4ull << 20)) return dd ? 3 : 2;
+    if (m < (8ull << 20)) return 2;
+    if (m < (16ull << 20) && std::is_integral<T>::value) return 2;
+    if (m < (16ull << 20) && Pol::SPECIAL && std::is_same<T, double>::value)
+      return 1;
+    return clamp_shape<Pol>(shape_one<Pol>::v);
+  }
+}
+
 template <class T, class Pol>
 static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
                   uint64_t n, int shape, hipStream_t s) {
@@ -1425,9 +1480,10 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
   if (vec_ok && split_pays<T, Pol>(g, n)) {
     if (chosen && shape == default_shape<Pol>())
       shape = clamp_shape<Pol>(shape_one<Pol>::v);
+    if (chosen) shape = split_walk_shape<T, Pol>(n, shape);
     uint8_t *fl = nullptr;
     if constexpr (Pol::SPECIAL)
-      fl = span_flags(s, (n * sizeof(T) / 16 + kBlock - 1) / kBlock);
+      fl = span_flags(s, ((n * sizeof(T) + 15) / 16 + kBlock - 1) / kBlock);
     if (!Pol::SPECIAL || fl != nullptr) {
       if constexpr (Pol::TREE) {
         if (shape == 3)
@@ -1443,7 +1499,7 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
         }
       }
       if (hipGetLastError() != hipSuccess) return OSC_ERR_HIP;
-      return replicate_tuned(g, doff, n * sizeof(T), fl, s);
+      return replicate_tuned(g, doff, n * sizeof(T) / 16 * 16, fl, s);
     }
   }
   if (!vec_ok) {

@@ -29,6 +29,8 @@ int visit_reduce(int dtype, int op, int mode, PerCallV *pc, BatchV *bv,
 
 namespace osc {
 int g_split_group = -1;
+bool g_split_partial = true;
+bool g_split_walk_rule = true;
 thread_local LaunchNote t_launch[kMaxLaunchNames];
 thread_local int t_nlaunch = 0;
 
@@ -245,6 +247,14 @@ extern "C" void oscd_release_span_flags(void) {
     osc::g_flag_cap = 0;
   }
   osc::replicate_tune_reset();
+}
+
+extern "C" void oscd_set_split_partial(int on) {
+  osc::g_split_partial = on != 0;
+}
+
+extern "C" void oscd_set_split_walk_rule(int on) {
+  osc::g_split_walk_rule = on != 0;
 }
 
 extern "C" void oscd_set_reduce_split(int enabled, int group) {

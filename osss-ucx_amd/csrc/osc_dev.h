@@ -101,6 +101,12 @@ int oscd_replicate_group(int npes, uint64_t nbytes, int *width);
  * (bytes_per_member, the heap slice); release frees them all */
 void oscd_set_span_flag_capacity(uint64_t bytes_per_member);
 void oscd_release_span_flags(void);
+/* split calls whose bytes per member are not a multiple of 16 (1, the
+ * default) or run them as one kernel (0) */
+void oscd_set_split_partial(int on);
+/* walk shape of split calls the library picks: 1 (the default) = by bytes
+ * per member, 0 = the policy's split shape from 8 MiB, 1x8 below */
+void oscd_set_split_walk_rule(int on);
 /* large-reduction split: 0 = one kernel (every workgroup stores to all
  * members), else the walk to member 0 + oscd_replicate with that group
  * size (-1, the default: picked by size) */

@@ -1,9 +1,10 @@
 """Large reductions of 2-, 4-, 8- and 16-byte elements run in two kernels:
-the slice walk stores member 0's result only, then a replicate kernel copies
-it to the other members, each loaded piece to a group of consecutive
+the slice walk stores member 0's result only (and a last 16-byte unit that
+holds fewer elements to every member), then a replicate kernel copies the
+whole units to the other members, each loaded piece to a group of consecutive
 members, in 1 KiB pieces interleaved over 16, 32 or 64 workgroups.  Checked against
 the oracle just above and below the size where the split starts (1 MiB per
-member, at least 8 members, a multiple of 16 bytes), on power-of-two and
+member, at least 8 members), with whole and partial last units, on power-of-two and
 odd teams, a strided team, in place, in both float modes, with the bytes
 after each destination untouched; 1-byte types (one kernel) at the same
 sizes; sizes that leave the last interleave group of pieces partly empty;
@@ -75,6 +76,12 @@ CASES = [  # (dtype, op, n): at, above and off the 1 MiB/member threshold
     # 8 MiB per member (the size-picked group of 4) and a size that leaves
     # the last interleave group of 1 KiB pieces partly empty
     ("f64", "sum", 1 << 20), ("i64", "xor", (1 << 20) + 131072 + 16 * 5),
+    # a last 16-byte unit with fewer elements: the walk stores it to every
+    # member, the replicate copies the whole units
+    ("i64", "sum", (1 << 18) + 1), ("f32", "sum", (1 << 19) + 3),
+    ("i16", "xor", (1 << 20) + 5), ("f64", "prod", 131072 + 777),
+    ("u32", "min", (1 << 18) + 2), ("f32", "min", 300001),
+    ("f64", "max", (1 << 20) + 131072 + 3),
     # 2-8 MiB per member: 1x8 walks below 8 MiB, 8x2 at 8 MiB
     ("i64", "max", 1 << 18), ("f32", "sum", (3 << 20) // 4 + 4),
     ("f64", "sum", (3 << 20) // 8 + 2), ("f64", "prod", (5 << 20) // 8 + 6),
@@ -122,7 +129,9 @@ def test_split_reduce_under_each_algorithm(heap):
 @pytest.mark.parametrize("dtype,op,n", [("f64", "sum", 200002),
                                         ("i64", "prod", 131072),
                                         ("c64", "prod", 65536 + 2),
-                                        ("f32", "sum", 262144 + 4 * 33)])
+                                        ("f32", "sum", 262144 + 4 * 33),
+                                        ("i64", "sum", 131072 + 1),
+                                        ("f32", "prod", 262144 + 3)])
 @pytest.mark.parametrize("team", [(0, 1, 8), (1, 2, 13), (0, 1, 64),
                                   (5, 1, 37)])
 def test_split_matches_one_kernel_for_every_group(heap, dtype, op, n, team):
