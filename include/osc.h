@@ -137,9 +137,11 @@ int osc_get_reduce_shape(void);
 /* Parallel branches per wave in a batch's captured graph, 1..8 (default 4,
  * OSC_BATCH_BRANCHES sets it at init); takes effect at the next capture. */
 int osc_set_batch_branches(int n);
-/* Large reductions (1 MiB or more per member, 8+ members, elements of 2, 4,
- * 8 or 16 bytes) run as a walk that stores member 0's result and a
- * replicate kernel that copies it to the other members, storing each loaded
+/* Large reductions (1 MiB or more per member on 32+ members, 3 MiB or more
+ * on 8 to 31; elements of 2, 4, 8 or 16 bytes) run as a walk that stores
+ * member 0's result and a replicate kernel that copies it to the other
+ * members in whole 16-byte units (a partial last unit is stored to every
+ * member by the walk), storing each loaded
  * piece to `group` members, the pieces interleaved over `width` workgroups.
  * -1 (default): one member per workgroup, width 32, below 8 MiB per
  * member; from 8 MiB the fastest of (1, 16), (1, 32), (1, 64) and (2, 32)

@@ -1350,7 +1350,8 @@ static void launch_shape(const oscd_geom *g, int lg, uint64_t doff,
 // members writes at 5.1-5.3 TB/s and a fused u64 cell takes 735 us; the
 // replicate writes in an interleaved order that reaches 6.3-6.8 TB/s.
 // Every type whose 16-byte unit holds whole elements (2, 4, 8 and 16
-// bytes) takes it, from 1 MiB per member and 8 members.  The replicate
+// bytes) takes it, from 1 MiB per member and 32 members (3 MiB per member
+// and 8 members).  The replicate
 // copies whole units; a last unit with fewer elements is stored to every
 // member by the walk.  The split costs one extra launch and (P-1)/G cached
 // re-reads of member 0's result.
@@ -1360,8 +1361,17 @@ static void launch_shape(const oscd_geom *g, int lg, uint64_t doff,
 // skips the flagged pieces.  The flags live in a per-stream device buffer
 // (span_flags); without one (a stream under graph capture) the call runs
 // as one kernel.
-constexpr int kSplitMinPes = 8;
+// Teams of fewer than kSplitWidePes members split from kSplitNarrowBytes
+// per member: below it their walk and replicate are too short to pay for
+// the second launch.  Measured on MI355X per call, split vs one kernel
+// alternated in one process (the batched workload's world, 128 members
+// about 1 GiB apart): 8-31 members at 1, 1.5 and 2-3 MiB per member
+// 2.7-3.1 vs 3.3-3.5, 3.4-4.0 vs 4.3-4.8 and 3.9-4.0 vs 4.1-4.3 TB/s, at
+// 3-4 MiB even; 32 members and more at 1 MiB 4.0-4.9 vs 4.0-4.6 and at
+// 2-3 MiB 5.0-5.6 vs 4.9-5.2 TB/s.
+constexpr int kSplitMinPes = 8, kSplitWidePes = 32;
 constexpr uint64_t kSplitMinBytes = 1ull << 20;  // per member
+constexpr uint64_t kSplitNarrowBytes = 3ull << 20;
 // 0: no split; -1: replicate group picked by size; else the group size
 extern int g_split_group;
 // false: calls whose bytes per member are not a multiple of 16 run as one
@@ -1371,6 +1381,7 @@ template <class T, class Pol>
 static bool split_pays(const oscd_geom *g, uint64_t n) {
   return g_split_group != 0 && vec_of<T>() * sizeof(T) == 16 &&
          g->npes >= kSplitMinPes && n * sizeof(T) >= kSplitMinBytes &&
+         (g->npes >= kSplitWidePes || n * sizeof(T) >= kSplitNarrowBytes) &&
          (g_split_partial || (n * sizeof(T)) % 16 == 0);
 }
 // One flag byte per 256-unit piece for a split float MIN/MAX call on stream

@@ -83,7 +83,7 @@ def test_misaligned_ranges_are_rejected(heap):
 
 
 def test_kernel_names_of_a_split_call_and_a_small_call(heap):
-    n = (2 << 20) // 8  # 2 MiB per member: split into walk + replicate
+    n = (4 << 20) // 8  # 4 MiB per member on 16 members: walk + replicate
     s, d = heap.malloc(n * 8), heap.malloc(n * 8)
     try:
         heap.fill(None, s, n, "i64", "int_range", -100, 100, seed=3)
@@ -109,6 +109,12 @@ def test_kernel_names_of_a_split_call_and_a_small_call(heap):
         assert names[0].startswith("k_reduce<double, osc::PolSeq<double, 3, "
                                    "true>"), names
         assert names[1] == "k_replicate<1, 32, true>", names
+        # one kernel below 3 MiB per member on fewer than 32 members
+        osc.check(heap.L.osc_reduce(heap.world, osc.OPS["sum"],
+                                    osc.DTYPES["i64"], d, s, n // 2, st),
+                  "reduce")
+        names = last_kernels(heap)
+        assert len(names) == 1 and names[0].endswith(", false>"), names
         # one kernel when the split is off
         osc.check(heap.L.osc_set_reduce_split(0), "split off")
         osc.check(heap.L.osc_reduce(heap.world, osc.OPS["sum"],

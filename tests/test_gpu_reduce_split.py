@@ -3,8 +3,8 @@ the slice walk stores member 0's result only (and a last 16-byte unit that
 holds fewer elements to every member), then a replicate kernel copies the
 whole units to the other members, each loaded piece to a group of consecutive
 members, in 1 KiB pieces interleaved over 16, 32 or 64 workgroups.  Checked against
-the oracle just above and below the size where the split starts (1 MiB per
-member, at least 8 members), with whole and partial last units, on power-of-two and
+the oracle just above and below the sizes where the split starts (1 MiB per
+member on 32 members or more, 3 MiB on 8 to 31), with whole and partial last units, on power-of-two and
 odd teams, a strided team, in place, in both float modes, with the bytes
 after each destination untouched; 1-byte types (one kernel) at the same
 sizes; sizes that leave the last interleave group of pieces partly empty;
@@ -126,17 +126,18 @@ def test_split_reduce_under_each_algorithm(heap):
             assert bit_equal(got, ref("sum", dtype, src), dtype), (mode, dtype)
 
 
-@pytest.mark.parametrize("dtype,op,n", [("f64", "sum", 200002),
-                                        ("i64", "prod", 131072),
-                                        ("c64", "prod", 65536 + 2),
-                                        ("f32", "sum", 262144 + 4 * 33),
-                                        ("i64", "sum", 131072 + 1),
-                                        ("f32", "prod", 262144 + 3)])
+@pytest.mark.parametrize("dtype,op,n", [("f64", "sum", 400002),
+                                        ("i64", "prod", 393216),
+                                        ("c64", "prod", 196608 + 2),
+                                        ("f32", "sum", 786432 + 4 * 33),
+                                        ("i64", "sum", 393216 + 1),
+                                        ("f32", "prod", 786432 + 3)])
 @pytest.mark.parametrize("team", [(0, 1, 8), (1, 2, 13), (0, 1, 64),
                                   (5, 1, 37)])
 def test_split_matches_one_kernel_for_every_group(heap, dtype, op, n, team):
     """Every replicate group size gives the one-kernel path's bits, on teams
-    of 8, 13, 37 and 64 members (partial last groups of 1..8 members)."""
+    of 8, 13, 37 and 64 members (partial last groups of 1..8 members), at
+    3 MiB per member, where teams of fewer than 32 members split too."""
     start, stride, P = team
     pes = [start + k * stride for k in range(P)]
     t = heap.world if P == WORLD else heap.split(start, stride, P)
