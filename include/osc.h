@@ -137,6 +137,13 @@ int osc_get_reduce_shape(void);
 /* Parallel branches per wave in a batch's captured graph, 1..8 (default 4,
  * OSC_BATCH_BRANCHES sets it at init); takes effect at the next capture. */
 int osc_set_batch_branches(int n);
+/* 1 (default): each wave of a batch's graph runs in two phases, the walks
+ * of split reductions (and the reductions that stay one kernel) first,
+ * then the replicates of the split ones with the data movement, so each
+ * phase's HBM traffic runs mostly one way; 0: one phase per wave.  Results
+ * are identical either way; takes effect at the next capture.
+ * OSC_BATCH_PHASES sets it at init. */
+int osc_set_batch_phases(int on);
 /* Large reductions (1 MiB or more per member on 32+ members, 3 MiB or more
  * on 8 to 31; elements of 2, 4, 8 or 16 bytes) run as a walk that stores
  * member 0's result and a replicate kernel that copies it to the other

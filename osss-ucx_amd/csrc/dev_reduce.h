@@ -322,6 +322,12 @@ struct LaunchNote {
 };
 extern thread_local LaunchNote t_launch[kMaxLaunchNames];
 extern thread_local int t_nlaunch;
+// Which part of a reduction this thread's next launches issue
+// (oscd_set_split_phase): -1 all of it; 0 the walk of a split call, or the
+// whole call when it does not split; 1 the replicate of a split call and
+// nothing else.  Float MIN/MAX do not split under 0 and 1 (their span flags
+// belong to one stream).
+extern thread_local int t_split_phase;
 inline void note_launch(const char *what, const char *pretty) {
   if (t_nlaunch < kMaxLaunchNames) t_launch[t_nlaunch++] = {what, pretty};
 }
@@ -1472,13 +1478,15 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
   if ((n + (vec_ok ? VB : 1) - 1) / (vec_ok ? VB : 1) > (1ull << 31))
     return OSC_ERR_ARG;
   const bool chosen = shape < 0;  // the shape is the library's to pick
+  const int phase = t_split_phase;
   if (shape < 0) {
     if constexpr (lanes_ok<T, Pol>()) {
       if (lanes_pays<T>(g, doff, soff, n))
-        return launch_lanes<T, Pol>(g, doff, soff, n, s);
+        return phase == 1 ? OSC_OK : launch_lanes<T, Pol>(g, doff, soff, n, s);
     } else if constexpr (gather_ok<T, Pol>()) {
       if (gather_pays<T>(g, doff, soff, n))
-        return launch_gather<T, Pol>(g, lg, doff, soff, n, s);
+        return phase == 1 ? OSC_OK
+                          : launch_gather<T, Pol>(g, lg, doff, soff, n, s);
     }
     // Small calls: at the default shape they would fill fewer workgroups
     // than the chip has CUs; one unit per thread with 8 slices in flight
@@ -1494,8 +1502,11 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
     if (chosen) shape = split_walk_shape<T, Pol>(n, shape);
     uint8_t *fl = nullptr;
     if constexpr (Pol::SPECIAL)
-      fl = span_flags(s, ((n * sizeof(T) + 15) / 16 + kBlock - 1) / kBlock);
+      if (phase < 0)
+        fl = span_flags(s, ((n * sizeof(T) + 15) / 16 + kBlock - 1) / kBlock);
     if (!Pol::SPECIAL || fl != nullptr) {
+      if (phase == 1)
+        return replicate_tuned(g, doff, n * sizeof(T) / 16 * 16, fl, s);
       if constexpr (Pol::TREE) {
         if (shape == 3)
           launch_shape<T, Pol, VB, shape_upt(3), shape_infl(3), true>(g, lg, doff, soff, n, lds, s, fl);
@@ -1510,9 +1521,11 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
         }
       }
       if (hipGetLastError() != hipSuccess) return OSC_ERR_HIP;
+      if (phase == 0) return OSC_OK;
       return replicate_tuned(g, doff, n * sizeof(T) / 16 * 16, fl, s);
     }
   }
+  if (phase == 1) return OSC_OK;  // one kernel: issued whole under phase 0
   if (!vec_ok) {
     launch_shape<T, Pol, 1, 1, 8>(g, lg, doff, soff, n, lds, s);
   } else if constexpr (Pol::TREE) {

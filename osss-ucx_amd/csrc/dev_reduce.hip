@@ -33,6 +33,7 @@ bool g_split_partial = true;
 bool g_split_walk_rule = true;
 thread_local LaunchNote t_launch[kMaxLaunchNames];
 thread_local int t_nlaunch = 0;
+thread_local int t_split_phase = -1;
 
 // Span flags of split float MIN/MAX reductions: one device buffer per
 // stream, since calls on different streams may run at the same time while
@@ -247,6 +248,10 @@ extern "C" void oscd_release_span_flags(void) {
     osc::g_flag_cap = 0;
   }
   osc::replicate_tune_reset();
+}
+
+extern "C" void oscd_set_split_phase(int phase) {
+  osc::t_split_phase = phase == 0 || phase == 1 ? phase : -1;
 }
 
 extern "C" void oscd_set_split_partial(int on) {

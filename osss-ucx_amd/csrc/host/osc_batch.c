@@ -37,6 +37,7 @@ struct osc_batch {
   double alg_bytes;
   /* parallel branches of the captured graph: streams, fork/join events */
   int nbranch;
+  int phases; /* osc_g.batch_phases when the graph was captured */
   void *aux[OSC_BATCH_MAX_BRANCHES - 1];
   void *ev_fork;
   void *ev_join[OSC_BATCH_MAX_BRANCHES - 1];
@@ -173,7 +174,8 @@ int osc_batch_launch(osc_batch *b, void *stream) {
   if (b == NULL) return osc_set_error(OSC_ERR_ARG, "null batch");
   if (stream == NULL) stream = osc_g.stream;
   if (b->graph != NULL &&
-      (b->graph_stream != stream || b->nbranch != osc_g.batch_branches)) {
+      (b->graph_stream != stream || b->nbranch != osc_g.batch_branches ||
+       b->phases != osc_g.batch_phases)) {
     oscd_graph_destroy(b->graph);
     b->graph = NULL;
   }
@@ -195,6 +197,7 @@ int osc_batch_launch(osc_batch *b, void *stream) {
     if (rc2 != OSC_OK) return osc_hip_error(rc2, "hipGraphInstantiate");
     b->graph = exec;
     b->graph_stream = stream;
+    b->phases = osc_g.batch_phases;
   }
   rc = oscd_graph_launch(b->graph, stream);
   return rc == OSC_OK ? OSC_OK : osc_hip_error(rc, "hipGraphLaunch");

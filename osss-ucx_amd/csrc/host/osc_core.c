@@ -98,6 +98,10 @@ int osc_init(int npes, size_t heap_bytes_per_pe, int device) {
     osc_g.batch_branches = (br != NULL && br[0] != '\0') ? atoi(br) : 4;
     if (osc_g.batch_branches < 1) osc_g.batch_branches = 1;
     if (osc_g.batch_branches > 8) osc_g.batch_branches = 8;
+    {
+      const char *ph = getenv("OSC_BATCH_PHASES");
+      osc_g.batch_phases = (ph != NULL && ph[0] != '\0') ? atoi(ph) != 0 : 1;
+    }
   }
   osc_g.wait_spin_us =
       (spin != NULL && spin[0] != '\0') ? (unsigned)strtoul(spin, NULL, 10) : 500u;
@@ -201,6 +205,12 @@ int osc_get_reduce_shape(void) { return osc_g.reduce_shape; }
 int osc_set_batch_branches(int n) {
   if (n < 1 || n > 8) return osc_set_error(OSC_ERR_ARG, "batch branches %d (1..8)", n);
   osc_g.batch_branches = n;
+  return OSC_OK;
+}
+int osc_set_batch_phases(int on) {
+  if (on != 0 && on != 1)
+    return osc_set_error(OSC_ERR_ARG, "batch phases %d (0 or 1)", on);
+  osc_g.batch_phases = on;
   return OSC_OK;
 }
 int osc_set_reduce_split(int group) {

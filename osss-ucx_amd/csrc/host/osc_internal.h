@@ -50,6 +50,8 @@ typedef struct osc_state {
   int fp_default;
   int reduce_shape; /* -1 = per-policy default; OSC_REDUCE_SHAPE overrides */
   int batch_branches; /* parallel branches per wave of a batch's graph */
+  int batch_phases;   /* waves in two phases: split walks, then replicates
+                         and data movement (osc_set_batch_phases) */
   unsigned wait_spin_us; /* blocking calls: bounded spin on a completion
                             slot before hipStreamSynchronize (0: never) */
   void *aux_dev;    /* 256 device bytes for single-call descriptors */

@@ -107,6 +107,11 @@ void oscd_set_split_partial(int on);
 /* walk shape of split calls the library picks: 1 (the default) = by bytes
  * per member, 0 = the policy's split shape from 8 MiB, 1x8 below */
 void oscd_set_split_walk_rule(int on);
+/* which part of a reduction the calling thread's next oscd_reduce calls
+ * issue: -1 (default) all; 0 the walk of a split call, or the whole call
+ * when it does not split; 1 only the replicate of a split call.  Float
+ * MIN/MAX do not split under 0 and 1. */
+void oscd_set_split_phase(int phase);
 /* large-reduction split: 0 = one kernel (every workgroup stores to all
  * members), else the walk to member 0 + oscd_replicate with that group
  * size (-1, the default: picked by size) */

@@ -100,6 +100,7 @@ def lib() -> C.CDLL:
         "osc_set_reduce_shape": (i, [i]),
         "osc_get_reduce_shape": (i, []),
         "osc_set_batch_branches": (i, [i]),
+        "osc_set_batch_phases": (i, [i]),
         "osc_set_reduce_split": (i, [i]),
         "osc_set_move_interleave": (i, [i]),
         "osc_reduce_split_group": (i, [i, sz, C.POINTER(C.c_int)]),

@@ -162,6 +162,54 @@ def test_batched_4096_calls_on_128_pes():
         world.close()
 
 
+@pytest.mark.parametrize("phases", [1, 0])
+def test_batch_phases_split_reductions_and_movers(phases):
+    """Waves of split reductions (1-4 MiB per member on 32-64 members, whole
+    and partial last units, float min/max among them) and large fcollect /
+    alltoall calls: with the wave in two phases (walks, then replicates with
+    the data movement) or in one, the captured graph gives the one-by-one
+    digests on every replay, with every member's destinations poisoned
+    before each replay."""
+    world = osc_calls.CallWorld(npes=64, max_reduce=1 << 19,
+                                move_bytes=8 << 20, dest_bytes=256 << 20)
+    h = world.h
+    osc.check(h.L.osc_set_batch_phases(phases), "phases")
+    try:
+        rng = np.random.default_rng(77 + phases)
+        calls = []
+        for k in range(24):
+            op, dt = osc_calls.MATRIX[k % len(osc_calls.MATRIX)]
+            P = int(rng.integers(32, 65))
+            n = int(rng.integers(1 << 17, 1 << 19)) | (k & 1)
+            calls.append(osc_calls.reduce_call(world, op, dt, P,
+                                               int(rng.integers(0, 65 - P)),
+                                               n))
+        for k in range(6):
+            P = int(rng.integers(16, 65))
+            kind = osc.CALL_ALLTOALL if k % 2 else osc.CALL_FCOLLECT
+            blk = min(96 << 10, (8 << 20) // P // 16 * 16)
+            calls.append(osc_calls.move_call(world, kind, P,
+                                             int(rng.integers(0, 65 - P)),
+                                             blk))
+        calls, nw = osc.plan(calls, world.dest_off, world.dest_bytes)
+        dest = h.L.osc_heap_base() + world.dest_off
+        for wv in range(nw):
+            wave = [c for c in calls if c.wave == wv]
+            b = osc.Batch(wave)
+            b.launch_serial()
+            d_serial = b.digests()
+            for rep in range(2):
+                h.fill(None, dest, world.dest_bytes // 8, "u64", "bits",
+                       seed=300 + rep)
+                b.launch()
+                assert np.array_equal(b.digests(), d_serial), (wv, rep)
+            b.close()
+    finally:
+        h.L.osc_set_batch_phases(1)
+        world.close()
+    assert h.L.osc_set_batch_phases(2) != 0
+
+
 # ---- served: 16 teams of 8..128 PEs ------------------------------------------
 
 def test_served_16_teams_of_8_to_128_pes():

@@ -17,3 +17,11 @@ def test_reduce_split_switch():
         assert L.osc_set_reduce_split(bad) != 0
     for good in (0, 1, 2, 4, 8, 16, -1):
         assert L.osc_set_reduce_split(good) == 0
+
+
+def test_batch_phases_switch():
+    L = osc.lib()
+    for bad in (2, -1):
+        assert L.osc_set_batch_phases(bad) != 0
+    assert L.osc_set_batch_phases(0) == 0
+    assert L.osc_set_batch_phases(1) == 0
