@@ -325,9 +325,11 @@ extern thread_local int t_nlaunch;
 // Which part of a reduction this thread's next launches issue
 // (oscd_set_split_phase): -1 all of it; 0 the walk of a split call, or the
 // whole call when it does not split; 1 the replicate of a split call and
-// nothing else.  Float MIN/MAX do not split under 0 and 1 (their span flags
-// belong to one stream).
+// nothing else.  Under 0 and 1 float MIN/MAX split only with span flags of
+// their own (t_split_flags: one byte per 256 units of the call, the same
+// for both phases), since a stream's flags serve one call at a time.
 extern thread_local int t_split_phase;
+extern thread_local uint8_t *t_split_flags;
 inline void note_launch(const char *what, const char *pretty) {
   if (t_nlaunch < kMaxLaunchNames) t_launch[t_nlaunch++] = {what, pretty};
 }
@@ -1502,8 +1504,9 @@ static int launch(const oscd_geom *g, int lg, uint64_t doff, uint64_t soff,
     if (chosen) shape = split_walk_shape<T, Pol>(n, shape);
     uint8_t *fl = nullptr;
     if constexpr (Pol::SPECIAL)
-      if (phase < 0)
-        fl = span_flags(s, ((n * sizeof(T) + 15) / 16 + kBlock - 1) / kBlock);
+      fl = phase < 0
+               ? span_flags(s, ((n * sizeof(T) + 15) / 16 + kBlock - 1) / kBlock)
+               : t_split_flags;
     if (!Pol::SPECIAL || fl != nullptr) {
       if (phase == 1)
         return replicate_tuned(g, doff, n * sizeof(T) / 16 * 16, fl, s);

@@ -34,6 +34,7 @@ bool g_split_walk_rule = true;
 thread_local LaunchNote t_launch[kMaxLaunchNames];
 thread_local int t_nlaunch = 0;
 thread_local int t_split_phase = -1;
+thread_local uint8_t *t_split_flags = nullptr;
 
 // Span flags of split float MIN/MAX reductions: one device buffer per
 // stream, since calls on different streams may run at the same time while
@@ -252,6 +253,10 @@ extern "C" void oscd_release_span_flags(void) {
 
 extern "C" void oscd_set_split_phase(int phase) {
   osc::t_split_phase = phase == 0 || phase == 1 ? phase : -1;
+}
+
+extern "C" void oscd_set_split_flags(void *flags) {
+  osc::t_split_flags = (uint8_t *)flags;
 }
 
 extern "C" void oscd_set_split_partial(int on) {

@@ -38,6 +38,10 @@ struct osc_batch {
   /* parallel branches of the captured graph: streams, fork/join events */
   int nbranch;
   int phases; /* osc_g.batch_phases when the graph was captured */
+  /* span flags of the float MIN/MAX reductions that may split in a
+   * two-phase wave: call i's at d_flags + flag_off[i] (UINT64_MAX: none) */
+  uint8_t *d_flags;
+  uint64_t *flag_off;
   void *aux[OSC_BATCH_MAX_BRANCHES - 1];
   void *ev_fork;
   void *ev_join[OSC_BATCH_MAX_BRANCHES - 1];
@@ -146,6 +150,26 @@ int osc_batch_create(const osc_call *calls, int ncalls, osc_batch **out) {
   }
   rc = osc_plan_build(&b->plan, calls, ncalls, 1);
   if (rc != OSC_OK) goto fail;
+  b->flag_off = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)ncalls);
+  if (b->flag_off == NULL) {
+    rc = osc_set_error(OSC_ERR_NOMEM, "batch");
+    goto fail;
+  }
+  {
+    uint64_t total = 0;
+    for (i = 0; i < ncalls; ++i) {
+      const uint64_t fb = osc_call_flag_bytes(&calls[i]);
+      b->flag_off[i] = fb ? total : UINT64_MAX;
+      total += fb;
+    }
+    if (total > 0) {
+      rc = oscd_malloc((void **)&b->d_flags, total);
+      if (rc != OSC_OK) {
+        osc_hip_error(rc, "batch span flags");
+        goto fail;
+      }
+    }
+  }
   rc = oscd_malloc((void **)&b->d_calls, sizeof(oscd_bcall) * (size_t)ncalls);
   if (rc == OSC_OK)
     rc = oscd_malloc((void **)&b->d_tiles,
@@ -188,7 +212,7 @@ int osc_batch_launch(osc_batch *b, void *stream) {
     if (rc != OSC_OK) return osc_hip_error(rc, "hipStreamBeginCapture");
     rc = osc_plan_enqueue_branches(&b->plan, b->calls, b->d_calls, b->d_tiles,
                                    stream, b->nbranch, b->aux, b->ev_fork,
-                                   b->ev_join);
+                                   b->ev_join, b->d_flags, b->flag_off);
     rc2 = oscd_graph_end(stream, &exec);
     if (rc != OSC_OK) {
       if (exec) oscd_graph_destroy(exec);
@@ -251,6 +275,8 @@ int osc_batch_destroy(osc_batch *b) {
   if (b->d_calls) oscd_free(b->d_calls);
   if (b->d_tiles) oscd_free(b->d_tiles);
   if (b->d_digest) oscd_free(b->d_digest);
+  if (b->d_flags) oscd_free(b->d_flags);
+  free(b->flag_off);
   osc_plan_free(&b->plan);
   free(b->calls);
   free(b);

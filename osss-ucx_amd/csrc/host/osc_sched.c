@@ -403,10 +403,20 @@ static int in_pass(const osc_launch *L, const osc_call *calls, int pass) {
 
 /* One pass over the launches [i0, i1) of a wave: forked onto nbranch
  * streams, largest first onto the least loaded branch, joined at the end. */
+uint64_t osc_call_flag_bytes(const osc_call *c) {
+  const uint64_t bytes = c->count * (uint64_t)osc_dtype_size(c->dtype);
+  if (c->kind != OSC_CALL_REDUCE ||
+      (c->dtype != OSC_T_F32 && c->dtype != OSC_T_F64) ||
+      (c->op != OSC_OP_MIN && c->op != OSC_OP_MAX) || bytes < (1u << 20))
+    return 0;
+  return ((bytes + 15) / 16 + 255) / 256;
+}
+
 static int enqueue_pass(const osc_plan *p, int i0, int i1, int pass,
                         const osc_call *calls, const oscd_bcall *d_calls,
                         const oscd_tile *d_tiles, void *stream, int nbranch,
-                        void *const *aux, void *ev_fork, void *const *ev_join) {
+                        void *const *aux, void *ev_fork, void *const *ev_join,
+                        uint8_t *d_flags, const uint64_t *flag_off) {
   double load[OSC_MAX_BRANCHES] = {0};
   int i, b, rc;
   char *done = (char *)calloc((size_t)(i1 - i0), 1);
@@ -435,11 +445,17 @@ static int enqueue_pass(const osc_plan *p, int i0, int i1, int pass,
         calls[p->launches[best].call].kind == OSC_CALL_REDUCE)
       w *= 0.5;
     load[bb] += w;
-    if (pass >= 0 && is_reduce_launch(&p->launches[best], calls))
+    if (pass >= 0 && is_reduce_launch(&p->launches[best], calls)) {
+      const osc_launch *L = &p->launches[best];
       oscd_set_split_phase(pass);
+      if (L->kind == OSC_LAUNCH_DIRECT && d_flags != NULL &&
+          flag_off != NULL && flag_off[L->call] != UINT64_MAX)
+        oscd_set_split_flags(d_flags + flag_off[L->call]);
+    }
     rc = enqueue_one(&p->launches[best], calls, d_calls, d_tiles,
                      bb == 0 ? stream : aux[bb - 1]);
     oscd_set_split_phase(-1);
+    oscd_set_split_flags(NULL);
   }
   free(done);
   for (b = 1; b < nbranch && rc == OSC_OK; ++b) {
@@ -453,7 +469,8 @@ int osc_plan_enqueue_branches(const osc_plan *p, const osc_call *calls,
                               const oscd_bcall *d_calls,
                               const oscd_tile *d_tiles, void *stream,
                               int nbranch, void *const *aux, void *ev_fork,
-                              void *const *ev_join) {
+                              void *const *ev_join, uint8_t *d_flags,
+                              const uint64_t *flag_off) {
   int i0 = 0, rc = OSC_OK;
   if (nbranch > OSC_MAX_BRANCHES) nbranch = OSC_MAX_BRANCHES;
   if (nbranch <= 1) return osc_plan_enqueue(p, calls, d_calls, d_tiles, stream);
@@ -468,16 +485,16 @@ int osc_plan_enqueue_branches(const osc_plan *p, const osc_call *calls,
        * reductions that stay whole), then their replicates with the data
        * movement, so each phase's traffic runs mostly one way.  Measured on
        * MI355X (the 4096-call mix on 128 members, alternated in one
-       * process): 108.2-108.7 ms against 109.5-110.3 ms in one phase; the
+       * process): 107.8-107.9 ms against 110.1-110.2 ms in one phase; the
        * large alltoalls in the first phase instead measured the same. */
       rc = enqueue_pass(p, i0, i1, 0, calls, d_calls, d_tiles, stream,
-                        nbranch, aux, ev_fork, ev_join);
+                        nbranch, aux, ev_fork, ev_join, d_flags, flag_off);
       if (rc == OSC_OK)
         rc = enqueue_pass(p, i0, i1, 1, calls, d_calls, d_tiles, stream,
-                          nbranch, aux, ev_fork, ev_join);
+                          nbranch, aux, ev_fork, ev_join, d_flags, flag_off);
     } else {
       rc = enqueue_pass(p, i0, i1, -1, calls, d_calls, d_tiles, stream,
-                        nbranch, aux, ev_fork, ev_join);
+                        nbranch, aux, ev_fork, ev_join, NULL, NULL);
     }
     i0 = i1;
   }

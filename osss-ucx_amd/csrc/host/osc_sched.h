@@ -59,7 +59,10 @@ int osc_plan_build_into(osc_plan *p, const osc_call *calls, int ncalls,
 int osc_plan_enqueue(const osc_plan *p, const osc_call *calls,
                      const oscd_bcall *d_calls, const oscd_tile *d_tiles,
                      void *stream);
-/* The same, with the launches of each wave spread over nbranch streams:
+/* The same, with the launches of each wave spread over nbranch streams
+ * (float MIN/MAX reductions with flag_off[call] != UINT64_MAX use
+ * d_flags + flag_off[call] as their span flags in two-phase waves; either
+ * may be NULL):
  * `stream` and aux[0 .. nbranch-2], forked from `stream` at the start of
  * the wave through ev_fork and joined back at its end through
  * ev_join[0 .. nbranch-2] (launches of one wave are independent: disjoint
@@ -70,7 +73,11 @@ int osc_plan_enqueue_branches(const osc_plan *p, const osc_call *calls,
                               const oscd_bcall *d_calls,
                               const oscd_tile *d_tiles, void *stream,
                               int nbranch, void *const *aux, void *ev_fork,
-                              void *const *ev_join);
+                              void *const *ev_join, uint8_t *d_flags,
+                              const uint64_t *flag_off);
+/* Bytes of span flags a float MIN/MAX reduction may need in a two-phase
+ * wave (0 for every other call). */
+uint64_t osc_call_flag_bytes(const osc_call *c);
 /* One call through the per-call kernels (the "one by one" path). */
 int osc_call_launch_single(const osc_call *c, void *stream);
 

@@ -109,9 +109,12 @@ void oscd_set_split_partial(int on);
 void oscd_set_split_walk_rule(int on);
 /* which part of a reduction the calling thread's next oscd_reduce calls
  * issue: -1 (default) all; 0 the walk of a split call, or the whole call
- * when it does not split; 1 only the replicate of a split call.  Float
- * MIN/MAX do not split under 0 and 1. */
+ * when it does not split; 1 only the replicate of a split call.  Under 0
+ * and 1 float MIN/MAX split only with the span flags set by
+ * oscd_set_split_flags (one byte per 256 16-byte units of the call, the
+ * same buffer for both phases; NULL: no split). */
 void oscd_set_split_phase(int phase);
+void oscd_set_split_flags(void *flags);
 /* large-reduction split: 0 = one kernel (every workgroup stores to all
  * members), else the walk to member 0 + oscd_replicate with that group
  * size (-1, the default: picked by size) */
