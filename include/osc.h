@@ -162,7 +162,8 @@ int osc_set_reduce_split(int group);
  * NULL) receives the interleave width. */
 int osc_reduce_split_group(int npes, size_t nbytes, int *width);
 /* fcollect, broadcast and alltoall blocks of 16-byte units from 64 KiB per
- * member: 1 (default) = the replicate's interleaved write order, 8
+ * member (alltoall from 24 KiB when P^2 x block exceeds 256 MiB): 1
+ * (default) = the replicate's interleaved write order, 8
  * workgroups per super-chunk; 0 = the chunked orders smaller blocks take
  * (destination-major for fcollect/broadcast, pair-major for alltoall).
  * Results are identical either way.  OSC_MOVE_INTERLEAVE sets it at
